@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark: 4-hop walks from 4096 sources on an R-MAT scale-22 graph, one MI355X.
+
+A step is one complete walk: X_0 (4096 one-hot rows) -> X_1 .. X_4, each hop a
+symbolic pass and a numeric pass over the device-resident A^T bit tiles; every X_k stays
+on the device. Graph generation and the tile build are setup (untimed).
+
+Prints ONE JSON line (rank 0). Extra keys: tile_products_per_s, per_hop, roofline
+(dominant kernel: the numeric walk kernel, HIP events on its launch stream over the
+timed region), cpu_baseline (the same tile algorithm on the host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "walk queries/s + tile-products/s, 4-hop R-MAT-22, learned+batched vs plain, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
+    """Time the oracle's tile algorithm (OpenMP) on every `stride`-th strip of each hop,
+    using the GPU's X_{k-1} as input, and scale each hop by total / sampled A^T bits.
+    Also checks the sampled strips against the GPU frontier (bit-exact)."""
+    ath = at.download()
+    ns = ath.n_strips
+    strips = np.arange(0, ns, stride)
+    est_total = 0.0
+    sample_s = 0.0
+    for k in range(1, hops + 1):
+        x_in = w.frontier(k - 1)
+        x_ref = w.frontier(k)
+        x_out = np.zeros_like(x_in)
+        t0 = time.perf_counter()
+        _, visited = O.walk_tile_hop(ns, ath.offset_row, ath.col_idx, ath.data, x_in, 0, ns, threads, x_out=x_out,
+                                     stride=stride)
+        dt = time.perf_counter() - t0
+        sample_s += dt
+        rows = (strips[:, None] * 64 + np.arange(64)[None, :]).ravel()
+        if not np.array_equal(x_out[rows], x_ref[rows]):
+            raise SystemExit(f"cpu_baseline: CPU tile algorithm disagrees with the GPU at hop {k}")
+        est_total += dt * (n_edges / max(visited, 1))
+        log(f"cpu hop {k}: {dt:.2f}s on {len(strips)} strips, {visited} A^T bits -> est {dt * n_edges / max(visited, 1):.2f}s")
+    return {
+        "value": n_src / est_total,
+        "unit": "source-walks/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle tile algorithm (OpenMP, {threads} threads) on every {stride}-th output strip of each of "
+                  f"the {hops} hops ({len(strips)} strips/hop, {sample_s:.1f}s measured), scaled per hop by total/"
+                  f"sampled A^T bits; sampled strips checked bit-exact against the GPU",
+        "est_seconds_per_walk": est_total,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--sources", type=int, default=4096)
+    ap.add_argument("--hops", type=int, default=4)
+    ap.add_argument("--cpu-stride", type=int, default=64, help="CPU baseline samples every n-th strip")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import matrixgraph_amd as M
+
+    ctx = M.Context(0)
+    t0 = time.perf_counter()
+    el = M.EdgeList.rmat(ctx, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
+    n_vertices, n_edges = el.info()
+    at = el.to_tiled_matrix(transpose=True)
+    el.close()
+    ns, n_tiles, _ = at.metadata()
+    log(f"graph: 2^{args.scale} vertices, {n_edges} edges, {n_tiles} A^T tiles, setup {time.perf_counter() - t0:.1f}s")
+    sources = M.sample_sources(n_vertices, args.sources, args.seed)
+    w = M.Walk(ctx, at, args.sources, args.hops, M.WALK_PRUNE if args.prune else 0)
+
+    for _ in range(args.warmup):
+        w.run(sources, args.hops)
+    ctx.synchronize()
+    w.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        w.run(sources, args.hops)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    w.set_timing(False)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    per_hop = []
+    total_tp = 0
+    sum_bytes = 0
+    sum_numeric_ms = 0.0
+    for k in range(1, args.hops + 1):
+        st = w.hop_stats(k)
+        sym_ms, num_ms, n_runs = w.kernel_times(k)
+        avg_num = num_ms / max(n_runs, 1)
+        avg_sym = sym_ms / max(n_runs, 1)
+        total_tp += st["tile_products"]
+        sum_bytes += st["alg_bytes"]
+        sum_numeric_ms += avg_num
+        per_hop.append({
+            "hop": k, "tile_products": st["tile_products"], "alg_bytes": st["alg_bytes"],
+            "left_tiles": st["left_tiles"], "right_tiles_used": st["right_tiles_used"],
+            "out_tiles": st["out_tiles"], "out_tiles_numeric": st["out_tiles_numeric"],
+            "popcount": st["popcount"], "edges_visited": st["edges_visited"],
+            "numeric_kernel_ms": round(avg_num, 4), "symbolic_ms": round(avg_sym, 4),
+            "numeric_GBps_alg": round(st["alg_bytes"] / (avg_num * 1e-3) / 1e9, 1) if avg_num > 0 else None,
+        })
+        log(f"hop {k}: TP={st['tile_products']} B_alg={st['alg_bytes'] / 1e9:.2f}GB numeric={avg_num:.3f}ms "
+            f"sym={avg_sym:.3f}ms popcount={st['popcount']}")
+    n_launch = args.hops
+    achieved = (sum_bytes / n_launch) / ((sum_numeric_ms / n_launch) * 1e-3) / 1e9 if sum_numeric_ms > 0 else 0.0
+    traffic = None
+    tfile = ROOT / "profiles" / "walk_numeric_rmat22_traffic.json"
+    if tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune:
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    roofline = {
+        "kernel": "walk_numeric_kernel<64>",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": sum_bytes // n_launch,
+        "avg_launch_ms": round(sum_numeric_ms / n_launch, 4),
+    }
+
+    result = {
+        "metric": METRIC,
+        "value": round(args.sources / (ms_per_step * 1e-3), 1),
+        "unit": "source-walks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"rmat{args.scale}_walk{args.hops}hop_{args.sources}src",
+            "graph": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, (a,b,c,d)=(0.57,0.19,0.19,0.05), "
+                     f"seed {args.seed}, duplicates/self-loops removed, {n_edges} edges, {n_tiles} tiles",
+            "sources": args.sources,
+            "hops": args.hops,
+            "tile_size": 64,
+            "structural_mode": "prune" if args.prune else "canonical",
+            "parallelism": "single-gpu",
+        },
+        "walk_queries_per_s": round(1.0 / (ms_per_step * 1e-3), 3),
+        "tile_products_per_s": round(total_tp / (ms_per_step * 1e-3), 1),
+        "tile_products_per_walk": total_tp,
+        "per_hop": per_hop,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        threads = O.max_threads()
+        result["cpu_baseline"] = cpu_baseline(M, O, ctx, w, at, n_edges, args.hops, args.sources, args.cpu_stride,
+                                              threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

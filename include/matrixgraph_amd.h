@@ -1,0 +1,263 @@
+/*
+ * matrixgraph_amd.h — C-ABI of the MI355X (gfx950) bit-tiled Boolean walk engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (MG_OK == 0). No entry point calls exit(); on failure the status is non-zero
+ * and mg_last_error() returns a message (thread-local, valid until the next
+ * failing call on the same thread).
+ *
+ * Tile layout (identical to the reference's BitTiledMatrix, tile_size 64):
+ *   tile_offset_row[n_strips+1]  u32 row pointer of the tile-CSR, [0] == 0
+ *   tile_row_idx[n_nz_tile]      u32 tile row of each tile
+ *   tile_col_idx[n_nz_tile]      u32 tile column of each tile
+ *   data[64 * n_nz_tile]         u64; tile t = words [64t, 64t+64);
+ *                                word i = row i, bit j (LSB = column 0) = column j
+ *   tiles listed in ascending (row, col) order.
+ * Reference: core/data_structures/bit_tiled_matrix.cuh:17-72,
+ *            core/data_structures/bit_tiled_matrix.cu:33-71,113-123,
+ *            core/util/format_converter.cuh:292-415.
+ *
+ * Products follow the reference convention: the right operand is passed
+ * TRANSPOSED (Bt = B^T), C = A * B over the Boolean semiring
+ * (core/task/gpu_task/ppr_query.cu:63-71, :153-210).
+ *
+ * Threading: a context is bound to one HIP device and one stream; it is not
+ * thread-safe (one context per host thread). Calls on a context are ordered
+ * on its stream; functions that return host values synchronise that stream.
+ */
+#ifndef MATRIXGRAPH_AMD_H_
+#define MATRIXGRAPH_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MG_OK = 0,
+  MG_ERR_INVALID = 1,     /* bad argument, shape or handle */
+  MG_ERR_HIP = 2,         /* HIP runtime error */
+  MG_ERR_OOM = 3,         /* device or host allocation failed */
+  MG_ERR_UNSUPPORTED = 4, /* valid in the reference but not supported here (e.g. tile_size != 64) */
+  MG_ERR_IO = 5,          /* file missing, truncated or malformed */
+  MG_ERR_INTERNAL = 6
+};
+
+#define MG_TILE_SIZE 64u
+
+typedef struct mg_context mg_context;
+typedef struct mg_tiled_matrix mg_tiled_matrix;
+typedef struct mg_edge_list mg_edge_list;
+typedef struct mg_walk mg_walk;
+
+/* ---------------------------------------------------------------------------
+ * Library and context
+ * ------------------------------------------------------------------------- */
+
+const char* mg_version(void);
+const char* mg_last_error(void);
+
+/* device < 0: take MATRIXGRAPH_HIP_DEVICE from the environment, else 0
+ * (replaces set_device_from_env, core/go_api/matrixgraph_go_api.cu:23-36). */
+int mg_context_create(int device, mg_context** out);
+int mg_context_destroy(mg_context* ctx);
+/* stream: a hipStream_t, or NULL for the context's own stream. */
+int mg_context_set_stream(mg_context* ctx, void* stream);
+int mg_context_synchronize(mg_context* ctx);
+int mg_context_device(const mg_context* ctx, int* device, int* n_compute_units);
+
+/* ---------------------------------------------------------------------------
+ * Layer 1 — the reference's kernel-wrapper entry points, one for one.
+ * Replaces sics::matrixgraph::core::task::kernel::MatrixOperationsKernelWrapper
+ * (core/task/gpu_task/kernel/matrix_operations.cuh:14-124). All buffers are
+ * DEVICE pointers owned by the caller; outputs that accumulate must be
+ * zeroed by the caller exactly as with the reference (Init(size) zero-fills,
+ * core/data_structures/device_buffer.cuh:92-106). stream is a hipStream_t
+ * (NULL = default stream). Launches are asynchronous.
+ * ------------------------------------------------------------------------- */
+
+/* Boolean product of bit-packed layout matrices, inner-product form:
+ *   c[i][j] |= OR_t a[i][t] & b[j][t]   (a: m x k, b: n x k, c: m x n, row-major bit-packed,
+ *   element (i,j) of an r x s matrix at bit i*s+j).
+ * Replaces MatrixBitAnd (matrix_operations.cuh:29-34, matrix_operations.cu:256-280,623-644). */
+int mg_matrix_bit_and(void* stream, const uint64_t* matrix_a, const uint64_t* matrix_b,
+                      uint64_t* matrix_c, uint32_t m, uint32_t k, uint32_t n);
+
+/* *count += popcount of bits [0, size) of matrix.
+ * Replaces MatrixBitCount (matrix_operations.cuh:36-40, matrix_operations.cu:282-296,646-661). */
+int mg_matrix_bit_count(void* stream, const uint64_t* matrix, uint64_t* count, uint64_t size);
+
+/* From an n_strips x n_strips layout bitmap: tile_count_row[x] = number of set bits in
+ * row x (per-row COUNTS, as the reference writes into its "tile_offset_row" buffer; the
+ * caller forms the prefix sum), tile_row_idx/tile_col_idx[rank] = (x, y) for the
+ * rank-th set bit in row-major order.
+ * Replaces InitBitTiledMatrixMetadataByLayoutMatrix (matrix_operations.cuh:42-48,
+ * matrix_operations.cu:298-316,663-686); n_strips is passed explicitly (the reference
+ * passes tile_size and loops over it). */
+int mg_init_bit_tiled_matrix_metadata_by_layout_matrix(void* stream, const uint64_t* layout_matrix,
+                                                       uint32_t n_strips, uint32_t* tile_count_row,
+                                                       uint32_t* tile_row_idx, uint32_t* tile_col_idx);
+
+/* Numeric pass: for every tile i of C (x, y) = (row_idx_c[i], col_idx_c[i]),
+ *   data_c[i] |= OR over t with A(x,t) and Bt(y,t) present of  A(x,t) (x) Bt(y,t)
+ * where (P = L (x) Rt)[r] bit c = ((L[r] & Rt[c]) != 0). OR-accumulates into data_c.
+ * tile_size must be 64. layout_matrix_c and tile_row_idx_a/b may be NULL (unused).
+ * Replaces FillTiles (matrix_operations.cuh:50-65, matrix_operations.cu:318-366,688-738). */
+int mg_fill_tiles(void* stream, size_t tile_size, size_t n_strips, size_t n_nz_tile_a,
+                  size_t n_nz_tile_b, size_t n_nz_tile_c, const uint64_t* layout_matrix_c,
+                  const uint32_t* tile_offset_row_a, const uint32_t* tile_offset_row_b,
+                  const uint32_t* tile_offset_row_c, const uint32_t* tile_row_idx_a,
+                  const uint32_t* tile_row_idx_b, const uint32_t* tile_row_idx_c,
+                  const uint32_t* tile_col_idx_a, const uint32_t* tile_col_idx_b,
+                  const uint32_t* tile_col_idx_c, const uint64_t* data_a, const uint64_t* data_b,
+                  uint64_t* data_c);
+
+/* *count += popcount of the 64 * n_nz_tile words of data.
+ * Replaces Count (matrix_operations.cuh:115-118, matrix_operations.cu:494-503,867-884). */
+int mg_count(void* stream, size_t tile_size, size_t n_strips, size_t n_nz_tile, const uint64_t* data,
+             uint64_t* count);
+
+/* ---------------------------------------------------------------------------
+ * Layer 2 — device-resident bit-tiled matrices (one block of a grid).
+ * Replaces the host-pinned BitTiledMatrix (bit_tiled_matrix.cuh:17-72) for GPU work.
+ * ------------------------------------------------------------------------- */
+
+int mg_tiled_matrix_upload(mg_context* ctx, uint32_t n_strips, uint32_t n_nz_tile, uint32_t tile_size,
+                           const uint32_t* tile_offset_row, const uint32_t* tile_row_idx,
+                           const uint32_t* tile_col_idx, const uint64_t* data, mg_tiled_matrix** out);
+int mg_tiled_matrix_metadata(const mg_tiled_matrix* m, uint32_t* n_strips, uint32_t* n_nz_tile,
+                             uint32_t* tile_size);
+/* Copies any non-NULL array back to the host. nz_tile_bm receives
+ * n_strips*n_strips/64 + 1 words (bit x*n_strips+y set iff tile (x,y) present),
+ * the layout of meta_buf/nz_tile_bm.bin (core/io/bit_tiled_matrix_io.cu:62-64). */
+int mg_tiled_matrix_download(mg_context* ctx, const mg_tiled_matrix* m, uint32_t* tile_offset_row,
+                             uint32_t* tile_row_idx, uint32_t* tile_col_idx, uint64_t* data,
+                             uint64_t* nz_tile_bm);
+/* Device addresses of the four arrays (borrowed; valid while m lives). */
+int mg_tiled_matrix_device_arrays(const mg_tiled_matrix* m, const uint32_t** tile_offset_row,
+                                  const uint32_t** tile_row_idx, const uint32_t** tile_col_idx,
+                                  const uint64_t** data);
+/* Order-independent 64-bit digest of (index, data): equal matrices give equal digests. */
+int mg_tiled_matrix_digest(mg_context* ctx, const mg_tiled_matrix* m, uint64_t* digest);
+/* Total set bits (the reference's total_count, ppr_query.cu:874-956). */
+int mg_tiled_matrix_popcount(mg_context* ctx, const mg_tiled_matrix* m, uint64_t* count);
+int mg_tiled_matrix_destroy(mg_tiled_matrix* m);
+
+typedef struct mg_product_stats {
+  uint64_t tile_products;  /* (r,k,c) with A(r,k) and B(k,c) both present */
+  uint64_t out_tiles;      /* structural tiles of C */
+  uint64_t right_tiles_used;
+  uint64_t alg_bytes;      /* 520*(T_A + T_B,used + T_C) + 4*(2*n_strips + 2) */
+} mg_product_stats;
+
+/* Symbolic pass: allocates C with the structural tile set of A*B (tile (x,y) present iff
+ * some t has A(x,t) and Bt(y,t) present), tiles zeroed. Both operands share n_strips.
+ * Replaces PPRQuery::InitResultMatrix (ppr_query.cu:73-457): MatrixBitAnd + MatrixBitCount +
+ * InitBitTiledMatrixMetadataByLayoutMatrix + host prefix sum. stats may be NULL. */
+int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                         mg_tiled_matrix** c, mg_product_stats* stats);
+/* Numeric pass into a C produced by mg_multiply_symbolic (OR-accumulates).
+ * Replaces PPRQuery::FillTilesUnifiedMemory (ppr_query.cu:459-780). */
+int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                        mg_tiled_matrix* c);
+
+/* ---------------------------------------------------------------------------
+ * Edge lists and the edge-list -> bit-tile converter (input production).
+ * Replaces Edges + Edgelist2BitTiledMatrix (core/data_structures/edgelist.h:22-233,
+ * core/util/format_converter.cuh:292-415) and Edges::Transpose (edgelist.cu:367-379).
+ * Device edge lists are kept sorted by (src, dst) and duplicate-free.
+ * ------------------------------------------------------------------------- */
+
+/* R-MAT, 2^scale vertices, edge_factor * 2^scale draws; quadrant probabilities a, b, c
+ * (d = 1-a-b-c); duplicates and self-loops removed. Counter-based SplitMix64 streams. */
+int mg_edges_rmat(mg_context* ctx, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                  uint64_t seed, mg_edge_list** out);
+/* Directed G(n, p), n = 2^log2_n, p = p_num / 2^p_log2_den exactly; no self-loops. */
+int mg_edges_er(mg_context* ctx, uint32_t log2_n, uint64_t p_num, uint32_t p_log2_den, uint64_t seed,
+                mg_edge_list** out);
+/* n_comm planted directed communities on n_vertices vertices: community c covers
+ * [c*stride, c*stride+comm_size); each ordered pair u != v inside one is an edge with
+ * p = p_num / 2^p_log2_den exactly. */
+int mg_edges_communities(mg_context* ctx, uint32_t n_vertices, uint32_t n_comm, uint32_t comm_size,
+                         uint32_t stride, uint64_t p_num, uint32_t p_log2_den, uint64_t seed,
+                         mg_edge_list** out);
+/* Union (set semantics) of two edge lists on the same vertex count. */
+int mg_edges_union(mg_context* ctx, const mg_edge_list* a, const mg_edge_list* b, mg_edge_list** out);
+/* Host edges -> device list, sorted and de-duplicated; self-loops KEPT (the reference's
+ * converter keeps them). */
+int mg_edges_upload(mg_context* ctx, uint32_t n_vertices, uint64_t n_edges, const uint32_t* src,
+                    const uint32_t* dst, mg_edge_list** out);
+int mg_edges_info(const mg_edge_list* el, uint32_t* n_vertices, uint64_t* n_edges);
+int mg_edges_download(mg_context* ctx, const mg_edge_list* el, uint32_t* src, uint32_t* dst);
+int mg_edges_digest(mg_context* ctx, const mg_edge_list* el, uint64_t* digest);
+/* transpose != 0 builds A^T (tile (y,x) from edge (x,y)), as Edges::Transpose then convert.
+ * n_strips = ceil(n_vertices / 64), block scope = n_vertices (n_chunks = 1). */
+int mg_edges_to_tiled_matrix(mg_context* ctx, const mg_edge_list* el, int transpose,
+                             mg_tiled_matrix** out);
+int mg_edges_destroy(mg_edge_list* el);
+
+/* count distinct vertices in [0, n_vertices) by Floyd's sampling over SplitMix64
+ * (pure host function, no GPU). Order of out[] = row order of X_0. */
+int mg_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t seed, uint32_t* out);
+
+/* ---------------------------------------------------------------------------
+ * Walks: X_{k+1} = X_k * A with the right operand supplied as A^T, X_0 one-hot rows
+ * (exact-length walks; X_k is not OR-ed into X_{k+1}). Generalises the
+ * PPRQuery pipeline (ppr_query.cu:958-989) to a hop loop with X_k device-resident.
+ *
+ * Device layout of X_k: vertex-major ("strip-interleaved") — vertex v's source set is
+ * W = words_per_vertex consecutive u64 (bit s%64 of word s/64 = source s). A strip of
+ * 64 vertices is 64*W words: the 64 bit tiles X_k(x, strip) for x < W, word-interleaved.
+ * mg_walk_export returns X_k in the reference tile layout and orientation.
+ * ------------------------------------------------------------------------- */
+
+enum {
+  MG_WALK_PRUNE = 1u  /* structural tile set of X_{k+1} built from numerically
+                         non-empty X_k tiles instead of structural ones */
+};
+
+typedef struct mg_hop_stats {
+  uint64_t tile_products;     /* structural (r,k,c) triples of this hop */
+  uint64_t left_tiles;        /* T_L: structural tiles of X_{k} */
+  uint64_t right_tiles_used;  /* T_R,used: tiles of A in rows referenced by X_k */
+  uint64_t out_tiles;         /* T_C: structural tiles of X_{k+1} */
+  uint64_t out_tiles_numeric; /* numerically non-empty tiles of X_{k+1} */
+  uint64_t popcount;          /* set bits of X_{k+1} */
+  uint64_t alg_bytes;         /* 520*(T_L+T_R,used+T_C) + 4*(2*rows+2), rows = tile rows of X */
+  uint64_t edges_visited;     /* set bits of A read by the numeric pass */
+  double symbolic_ms;         /* device time of the last logged run (timing enabled), else 0 */
+  double numeric_ms;          /* numeric kernel only (walk_numeric), same run */
+} mg_hop_stats;
+
+/* Allocates X_0..X_max_hops (W = ceil(max_sources/64) rounded up to a power of two,
+ * max_sources <= 4096) plus the strip scheduler for at. at must outlive w. */
+int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sources,
+                   uint32_t max_hops, uint32_t flags, mg_walk** out);
+/* Runs hops 1..hops from the given distinct sources (async; stats after sync). */
+int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, const uint32_t* sources, uint32_t hops);
+/* enable != 0 starts (and clears) a log of HIP events around the symbolic pass and the
+ * numeric kernel of every hop of every later run; enable == 0 stops logging. */
+int mg_walk_set_timing(mg_walk* w, int enable);
+/* Sums over the logged runs that reached `hop`: symbolic-pass and numeric-kernel device
+ * milliseconds, and the number of such runs. Synchronises. */
+int mg_walk_kernel_times(mg_context* ctx, mg_walk* w, uint32_t hop, double* symbolic_ms, double* numeric_ms,
+                         uint32_t* n_runs);
+/* hop in [1, hops]; synchronises. */
+int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_hop_stats* out);
+int mg_walk_words_per_vertex(const mg_walk* w, uint32_t* words);
+/* hop in [0, hops]; xt receives n_strips*64*W words (vertex-major). */
+int mg_walk_frontier_download(mg_context* ctx, const mg_walk* w, uint32_t hop, uint64_t* xt);
+/* Order-independent digest of X_hop restricted to the first n_sources source bits. */
+int mg_walk_frontier_digest(mg_context* ctx, const mg_walk* w, uint32_t hop, uint64_t* digest);
+/* X_hop in the reference layout: n_strips x n_strips tile grid (padded square), tile rows
+ * 0..ceil(n_sources/64)-1 used, structural tile set (or numeric with MG_WALK_PRUNE). */
+int mg_walk_export(mg_context* ctx, const mg_walk* w, uint32_t hop, mg_tiled_matrix** out);
+int mg_walk_destroy(mg_walk* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATRIXGRAPH_AMD_H_ */

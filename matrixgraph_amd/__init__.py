@@ -1,0 +1,371 @@
+"""MI355X-native bit-tiled Boolean walk engine (Python mirror of the C-ABI).
+
+Objects here own C handles; every computation runs in libmatrixgraph_amd.so on the GPU.
+Host-side containers (HostTiledMatrix) only hold arrays in the reference's layout
+(core/data_structures/bit_tiled_matrix.cuh:17-72).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+
+TILE = 64
+
+
+class MGError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed (status {code}): {msg}")
+        self.code = code
+
+
+def _check(code: int, where: str) -> None:
+    if code != 0:
+        lib = _capi.load()
+        raise MGError(code, where, lib.mg_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def lib():
+    return _capi.load()
+
+
+def version() -> str:
+    return lib().mg_version().decode()
+
+
+# ----------------------------------------------------------------------------
+# Host mirror of BitTiledMatrix
+# ----------------------------------------------------------------------------
+@dataclass
+class HostTiledMatrix:
+    """One block of bit tiles in the reference layout (tile_size 64).
+
+    offset_row[n_strips+1], row_idx[n], col_idx[n] (uint32); data[n, 64] (uint64), word i of
+    a tile = row i, bit j = column j (core/util/format_converter.cuh:408-410).
+    """
+
+    n_strips: int
+    offset_row: np.ndarray
+    row_idx: np.ndarray
+    col_idx: np.ndarray
+    data: np.ndarray
+    tile_size: int = TILE
+
+    @property
+    def n_nz_tile(self) -> int:
+        return int(self.row_idx.shape[0])
+
+    @classmethod
+    def empty(cls, n_strips: int) -> "HostTiledMatrix":
+        return cls(n_strips, np.zeros(n_strips + 1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                   np.zeros((0, TILE), np.uint64))
+
+    def nz_tile_bm(self) -> np.ndarray:
+        """meta_buf/nz_tile_bm.bin words: n_strips^2/64 + 1 u64 (core/io/bit_tiled_matrix_io.cu:62-64)."""
+        bits = self.n_strips * self.n_strips
+        bm = np.zeros(bits // 64 + 1, np.uint64)
+        pos = self.row_idx.astype(np.uint64) * np.uint64(self.n_strips) + self.col_idx.astype(np.uint64)
+        np.bitwise_or.at(bm, (pos >> np.uint64(6)).astype(np.int64), np.uint64(1) << (pos & np.uint64(63)))
+        return bm
+
+    def to_dense(self, n: int | None = None) -> np.ndarray:
+        """Dense uint8 n x n adjacency (n defaults to n_strips*64)."""
+        n = self.n_strips * TILE if n is None else n
+        full = np.zeros((self.n_strips * TILE, self.n_strips * TILE), np.uint8)
+        if self.n_nz_tile:
+            bits = np.unpackbits(self.data.view(np.uint8).reshape(self.n_nz_tile, TILE, 8), axis=2,
+                                 bitorder="little").reshape(self.n_nz_tile, TILE, TILE)
+            for t in range(self.n_nz_tile):
+                x, y = int(self.row_idx[t]), int(self.col_idx[t])
+                full[x * TILE:(x + 1) * TILE, y * TILE:(y + 1) * TILE] = bits[t]
+        return full[:n, :n]
+
+    @classmethod
+    def from_dense(cls, dense: np.ndarray, keep_empty=None) -> "HostTiledMatrix":
+        """Tiles with at least one set bit (plus any (x, y) in keep_empty), row-major."""
+        n = dense.shape[0]
+        ns = (n + TILE - 1) // TILE
+        pad = np.zeros((ns * TILE, ns * TILE), np.uint8)
+        pad[:n, :n] = dense
+        blocks = pad.reshape(ns, TILE, ns, TILE).transpose(0, 2, 1, 3)
+        occ = blocks.any(axis=(2, 3))
+        if keep_empty is not None:
+            for x, y in keep_empty:
+                occ[x, y] = True
+        xs, ys = np.nonzero(occ)
+        data = np.packbits(blocks[xs, ys].astype(np.uint8), axis=2, bitorder="little").reshape(len(xs), TILE * 8)
+        data = data.view(np.uint64).reshape(len(xs), TILE).copy()
+        off = np.zeros(ns + 1, np.uint32)
+        np.add.at(off, xs + 1, 1)
+        off = np.cumsum(off).astype(np.uint32)
+        return cls(ns, off, xs.astype(np.uint32), ys.astype(np.uint32), data)
+
+
+# ----------------------------------------------------------------------------
+# Context and device objects
+# ----------------------------------------------------------------------------
+class Context:
+    def __init__(self, device: int = -1):
+        h = C.c_void_p()
+        _check(lib().mg_context_create(device, C.byref(h)), "mg_context_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mg_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        _check(lib().mg_context_synchronize(self.h), "mg_context_synchronize")
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib().mg_context_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None),
+               "mg_context_set_stream")
+
+    @property
+    def n_compute_units(self) -> int:
+        d, n = C.c_int(), C.c_int()
+        _check(lib().mg_context_device(self.h, C.byref(d), C.byref(n)), "mg_context_device")
+        return n.value
+
+
+class TiledMatrix:
+    """Device-resident bit-tiled matrix (one block)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+
+    @classmethod
+    def upload(cls, ctx: Context, m: HostTiledMatrix) -> "TiledMatrix":
+        off = np.ascontiguousarray(m.offset_row, np.uint32)
+        row = np.ascontiguousarray(m.row_idx, np.uint32)
+        col = np.ascontiguousarray(m.col_idx, np.uint32)
+        data = np.ascontiguousarray(m.data, np.uint64)
+        h = C.c_void_p()
+        _check(lib().mg_tiled_matrix_upload(ctx.h, m.n_strips, m.n_nz_tile, m.tile_size, _ptr(off), _ptr(row),
+                                            _ptr(col), _ptr(data), C.byref(h)), "mg_tiled_matrix_upload")
+        return cls(ctx, h)
+
+    def metadata(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().mg_tiled_matrix_metadata(self.h, C.byref(a), C.byref(b), C.byref(c)), "mg_tiled_matrix_metadata")
+        return a.value, b.value, c.value
+
+    @property
+    def n_strips(self) -> int:
+        return self.metadata()[0]
+
+    @property
+    def n_nz_tile(self) -> int:
+        return self.metadata()[1]
+
+    def download(self) -> HostTiledMatrix:
+        ns, n, _ = self.metadata()
+        off = np.empty(ns + 1, np.uint32)
+        row = np.empty(n, np.uint32)
+        col = np.empty(n, np.uint32)
+        data = np.empty((n, TILE), np.uint64)
+        _check(lib().mg_tiled_matrix_download(self.ctx.h, self.h, _ptr(off), _ptr(row), _ptr(col), _ptr(data), None),
+               "mg_tiled_matrix_download")
+        return HostTiledMatrix(ns, off, row, col, data)
+
+    def device_arrays(self):
+        p = [C.c_void_p() for _ in range(4)]
+        _check(lib().mg_tiled_matrix_device_arrays(self.h, *[C.byref(x) for x in p]), "mg_tiled_matrix_device_arrays")
+        return tuple(x.value for x in p)
+
+    def digest(self) -> int:
+        d = C.c_uint64()
+        _check(lib().mg_tiled_matrix_digest(self.ctx.h, self.h, C.byref(d)), "mg_tiled_matrix_digest")
+        return d.value
+
+    def popcount(self) -> int:
+        d = C.c_uint64()
+        _check(lib().mg_tiled_matrix_popcount(self.ctx.h, self.h, C.byref(d)), "mg_tiled_matrix_popcount")
+        return d.value
+
+    def close(self):
+        if self.h:
+            lib().mg_tiled_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def multiply_symbolic(ctx: Context, a: TiledMatrix, bt: TiledMatrix):
+    h = C.c_void_p()
+    st = _capi.ProductStats()
+    _check(lib().mg_multiply_symbolic(ctx.h, a.h, bt.h, C.byref(h), C.byref(st)), "mg_multiply_symbolic")
+    return TiledMatrix(ctx, h), {f: getattr(st, f) for f, _ in _capi.ProductStats._fields_}
+
+
+def multiply_numeric(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: TiledMatrix) -> None:
+    _check(lib().mg_multiply_numeric(ctx.h, a.h, bt.h, c.h), "mg_multiply_numeric")
+
+
+def multiply(ctx: Context, a: TiledMatrix, bt: TiledMatrix):
+    """C = A * B (B supplied transposed): symbolic then numeric."""
+    c, st = multiply_symbolic(ctx, a, bt)
+    multiply_numeric(ctx, a, bt, c)
+    return c, st
+
+
+class EdgeList:
+    """Device edge list, sorted by (src, dst), duplicate-free."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+
+    @classmethod
+    def rmat(cls, ctx, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+        h = C.c_void_p()
+        _check(lib().mg_edges_rmat(ctx.h, scale, edge_factor, a, b, c, seed, C.byref(h)), "mg_edges_rmat")
+        return cls(ctx, h)
+
+    @classmethod
+    def er(cls, ctx, log2_n, p_num, p_log2_den, seed):
+        h = C.c_void_p()
+        _check(lib().mg_edges_er(ctx.h, log2_n, p_num, p_log2_den, seed, C.byref(h)), "mg_edges_er")
+        return cls(ctx, h)
+
+    @classmethod
+    def communities(cls, ctx, n_vertices, n_comm, comm_size, stride, p_num, p_log2_den, seed):
+        h = C.c_void_p()
+        _check(lib().mg_edges_communities(ctx.h, n_vertices, n_comm, comm_size, stride, p_num, p_log2_den, seed,
+                                          C.byref(h)), "mg_edges_communities")
+        return cls(ctx, h)
+
+    @classmethod
+    def upload(cls, ctx, n_vertices, src, dst):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        h = C.c_void_p()
+        _check(lib().mg_edges_upload(ctx.h, n_vertices, len(src), _ptr(src), _ptr(dst), C.byref(h)), "mg_edges_upload")
+        return cls(ctx, h)
+
+    def union(self, other: "EdgeList") -> "EdgeList":
+        h = C.c_void_p()
+        _check(lib().mg_edges_union(self.ctx.h, self.h, other.h, C.byref(h)), "mg_edges_union")
+        return EdgeList(self.ctx, h)
+
+    def info(self):
+        n, m = C.c_uint32(), C.c_uint64()
+        _check(lib().mg_edges_info(self.h, C.byref(n), C.byref(m)), "mg_edges_info")
+        return n.value, m.value
+
+    def download(self):
+        _, m = self.info()
+        src = np.empty(m, np.uint32)
+        dst = np.empty(m, np.uint32)
+        _check(lib().mg_edges_download(self.ctx.h, self.h, _ptr(src), _ptr(dst)), "mg_edges_download")
+        return src, dst
+
+    def digest(self) -> int:
+        d = C.c_uint64()
+        _check(lib().mg_edges_digest(self.ctx.h, self.h, C.byref(d)), "mg_edges_digest")
+        return d.value
+
+    def to_tiled_matrix(self, transpose: bool = False) -> TiledMatrix:
+        h = C.c_void_p()
+        _check(lib().mg_edges_to_tiled_matrix(self.ctx.h, self.h, 1 if transpose else 0, C.byref(h)),
+               "mg_edges_to_tiled_matrix")
+        return TiledMatrix(self.ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().mg_edges_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sample_sources(n_vertices: int, count: int, seed: int) -> np.ndarray:
+    out = np.empty(count, np.uint32)
+    _check(lib().mg_sample_sources(n_vertices, count, seed, _ptr(out)), "mg_sample_sources")
+    return out
+
+
+WALK_PRUNE = 1
+
+
+class Walk:
+    """X_{k+1} = X_k * A from one-hot X_0, right operand A^T (device-resident frontiers)."""
+
+    def __init__(self, ctx: Context, at: TiledMatrix, max_sources: int, max_hops: int, flags: int = 0):
+        h = C.c_void_p()
+        _check(lib().mg_walk_create(ctx.h, at.h, max_sources, max_hops, flags, C.byref(h)), "mg_walk_create")
+        self.ctx, self.at, self.h = ctx, at, h
+        self.n_strips = at.n_strips
+        w = C.c_uint32()
+        _check(lib().mg_walk_words_per_vertex(h, C.byref(w)), "mg_walk_words_per_vertex")
+        self.words_per_vertex = w.value
+
+    def set_timing(self, on: bool):
+        _check(lib().mg_walk_set_timing(self.h, 1 if on else 0), "mg_walk_set_timing")
+
+    def run(self, sources, hops: int) -> None:
+        src = np.ascontiguousarray(sources, np.uint32)
+        self._n_sources = len(src)
+        _check(lib().mg_walk_run(self.ctx.h, self.h, len(src), _ptr(src), hops), "mg_walk_run")
+
+    def hop_stats(self, hop: int) -> dict:
+        st = _capi.HopStats()
+        _check(lib().mg_walk_hop_stats(self.ctx.h, self.h, hop, C.byref(st)), "mg_walk_hop_stats")
+        return {f: getattr(st, f) for f, _ in _capi.HopStats._fields_}
+
+    def kernel_times(self, hop: int):
+        """(symbolic_ms_sum, numeric_ms_sum, n_runs) over runs logged since set_timing(True)."""
+        a, b, n = C.c_double(), C.c_double(), C.c_uint32()
+        _check(lib().mg_walk_kernel_times(self.ctx.h, self.h, hop, C.byref(a), C.byref(b), C.byref(n)),
+               "mg_walk_kernel_times")
+        return a.value, b.value, n.value
+
+    def frontier(self, hop: int) -> np.ndarray:
+        """Vertex-major X_hop: shape (n_vertices, words_per_vertex) uint64."""
+        out = np.empty((self.n_strips * TILE, self.words_per_vertex), np.uint64)
+        _check(lib().mg_walk_frontier_download(self.ctx.h, self.h, hop, _ptr(out)), "mg_walk_frontier_download")
+        return out
+
+    def frontier_digest(self, hop: int) -> int:
+        d = C.c_uint64()
+        _check(lib().mg_walk_frontier_digest(self.ctx.h, self.h, hop, C.byref(d)), "mg_walk_frontier_digest")
+        return d.value
+
+    def export(self, hop: int) -> TiledMatrix:
+        h = C.c_void_p()
+        _check(lib().mg_walk_export(self.ctx.h, self.h, hop, C.byref(h)), "mg_walk_export")
+        return TiledMatrix(self.ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().mg_walk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

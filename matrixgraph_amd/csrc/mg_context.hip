@@ -1,0 +1,114 @@
+// Context lifecycle, error reporting and host-only helpers of the C-ABI.
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_set>
+
+#include "mg_device.hpp"
+#include "mg_internal.hpp"
+
+namespace mgamd {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace mgamd
+
+using namespace mgamd;
+
+extern "C" const char* mg_version(void) { return "matrixgraph_amd 0.1.0 (gfx950)"; }
+
+extern "C" const char* mg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int mg_context_create(int device, mg_context** out) {
+  return guarded([&] {
+    MG_REQUIRE(out != nullptr, "mg_context_create: out is NULL");
+    *out = nullptr;
+    if (device < 0) {
+      // Same role as MATRIXGRAPH_CUDA_DEVICE in core/go_api/matrixgraph_go_api.cu:23-36.
+      const char* env = std::getenv("MATRIXGRAPH_HIP_DEVICE");
+      device = env ? std::atoi(env) : 0;
+    }
+    int n_dev = 0;
+    MG_HIP(hipGetDeviceCount(&n_dev));
+    if (device >= n_dev)
+      invalid("mg_context_create: device " + std::to_string(device) + " not present (" +
+              std::to_string(n_dev) + " devices)");
+    DeviceGuard g(device);
+    hipDeviceProp_t prop;
+    MG_HIP(hipGetDeviceProperties(&prop, device));
+    auto* ctx = new mg_context();
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete ctx;
+      MG_HIP(e);
+    }
+    ctx->stream = ctx->own_stream;
+    try {
+      ctx->counters.alloc(64);
+    } catch (...) {
+      (void)hipStreamDestroy(ctx->own_stream);
+      delete ctx;
+      throw;
+    }
+    *out = ctx;
+  });
+}
+
+extern "C" int mg_context_destroy(mg_context* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->counters.release();
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+  });
+}
+
+extern "C" int mg_context_set_stream(mg_context* ctx, void* stream) {
+  return guarded([&] {
+    MG_REQUIRE(ctx != nullptr, "mg_context_set_stream: ctx is NULL");
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  });
+}
+
+extern "C" int mg_context_synchronize(mg_context* ctx) {
+  return guarded([&] {
+    MG_REQUIRE(ctx != nullptr, "mg_context_synchronize: ctx is NULL");
+    DeviceGuard g(ctx->device);
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int mg_context_device(const mg_context* ctx, int* device, int* n_compute_units) {
+  return guarded([&] {
+    MG_REQUIRE(ctx != nullptr, "mg_context_device: ctx is NULL");
+    if (device) *device = ctx->device;
+    if (n_compute_units) *n_compute_units = ctx->n_cu;
+  });
+}
+
+// Floyd's algorithm (Bentley & Floyd, "Programming pearls: a sample of
+// brilliance", CACM 1987): for j = n-count .. n-1 draw r in [0, j]; take r unless
+// already taken, else take j. Draw i uses SplitMix64 counter i of the
+// "sources" stream; r = (draw >> 11) % (j+1).
+extern "C" int mg_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t seed, uint32_t* out) {
+  return guarded([&] {
+    MG_REQUIRE(out != nullptr || count == 0, "mg_sample_sources: out is NULL");
+    MG_REQUIRE(count <= n_vertices, "mg_sample_sources: count exceeds n_vertices");
+    const uint64_t key = stream_key(seed, kTagSources);
+    std::unordered_set<uint32_t> taken;
+    taken.reserve(count * 2 + 1);
+    uint32_t k = 0;
+    for (uint64_t j = static_cast<uint64_t>(n_vertices) - count; j < n_vertices; ++j, ++k) {
+      const uint64_t r = (sm64_at(key, k) >> 11) % (j + 1);
+      const uint32_t pick = taken.count(static_cast<uint32_t>(r)) ? static_cast<uint32_t>(j)
+                                                                   : static_cast<uint32_t>(r);
+      taken.insert(pick);
+      out[k] = pick;
+    }
+  });
+}

@@ -1,0 +1,464 @@
+// General bit-tiled Boolean product C = A * B with B supplied transposed (Bt):
+//   - layout-level entry points of the reference wrapper (MatrixBitAnd,
+//     InitBitTiledMatrixMetadataByLayoutMatrix, FillTiles);
+//   - the handle-based symbolic pass (structural tile set of C, tile-product count)
+//     and numeric pass.
+// Tile product on a wave64: lane r holds L[r] (row r of the left tile), lane c holds
+// Rt[c] (row c of the transposed right tile); output row r bit c = (L[r] & Rt[c]) != 0.
+// The loop runs only over non-zero Rt rows (wave ballot), so sparse tiles cost a few
+// instructions; dense tiles (64 non-zero rows) take 64 iterations.
+#include <string>
+
+#include "mg_device.hpp"
+#include "mg_internal.hpp"
+
+namespace mgamd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ inline uint64_t tile_product(uint64_t l_row, uint64_t rt_row) {
+  uint64_t nz = ballot(rt_row != 0);
+  if (ballot(l_row != 0) == 0) return 0;
+  uint64_t out = 0;
+  while (nz) {
+    const int c = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+    nz &= nz - 1;
+    const uint64_t w = read_lane(rt_row, c);
+    out |= static_cast<uint64_t>((l_row & w) != 0) << c;
+  }
+  return out;
+}
+
+// Index of key in sorted a[lo, hi), or -1.
+__device__ inline int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t key) {
+  uint32_t l = lo, h = hi;
+  while (l < h) {
+    const uint32_t mid = l + ((h - l) >> 1);
+    if (a[mid] < key) l = mid + 1; else h = mid;
+  }
+  return (l < hi && a[l] == key) ? static_cast<int64_t>(l) : -1;
+}
+
+// One wave per tile of C: intersect A.row(x) with Bt.row(y) on the tile column,
+// OR the tile products, OR the result into data_c (FillTiles semantics,
+// core/task/gpu_task/kernel/matrix_operations.cu:318-366 with the intended
+// offsets of B at :356 and the intended 64x64 product of :212-233).
+__global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
+    uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
+    const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
+    const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
+    uint64_t* __restrict__ data_c) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_c; i += n_waves) {
+    const uint32_t x = row_c[i];
+    const uint32_t y = col_c[i];
+    const uint32_t a0 = off_a[x], a1 = off_a[x + 1];
+    const uint32_t b0 = off_bt[y], b1 = off_bt[y + 1];
+    const bool a_short = (a1 - a0) <= (b1 - b0);
+    const uint32_t s0 = a_short ? a0 : b0, s1 = a_short ? a1 : b1;
+    const uint32_t l0 = a_short ? b0 : a0, l1 = a_short ? b1 : a1;
+    const uint32_t* s_col = a_short ? col_a : col_bt;
+    const uint32_t* l_col = a_short ? col_bt : col_a;
+    uint64_t acc = 0;
+    for (uint32_t base = s0; base < s1; base += kWave) {
+      const uint32_t j = base + lane;
+      int64_t pos = -1;
+      if (j < s1) pos = find_sorted(l_col, l0, l1, s_col[j]);
+      uint64_t hit = ballot(pos >= 0);
+      while (hit) {
+        const int k = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+        hit &= hit - 1;
+        const uint32_t js = base + k;
+        const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), k));
+        const uint64_t ia = a_short ? js : jl;
+        const uint64_t ib = a_short ? jl : js;
+        const uint64_t l_row = data_a[ia * kTile + lane];
+        const uint64_t rt_row = data_bt[ib * kTile + lane];
+        acc |= tile_product(l_row, rt_row);
+      }
+    }
+    uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
+    if (acc) *dst |= acc;
+  }
+}
+
+// len (<= 64) bits of a bit-packed array starting at bit `start`.
+__device__ inline uint64_t bit_window(const uint64_t* __restrict__ p, uint64_t start, uint32_t len) {
+  const uint64_t w = start >> 6;
+  const uint32_t s = static_cast<uint32_t>(start & 63);
+  uint64_t v = p[w] >> s;
+  if (s != 0 && s + len > 64) v |= p[w + 1] << (64 - s);
+  if (len < 64) v &= (1ull << len) - 1;
+  return v;
+}
+
+// MatrixBitAnd (matrix_operations.cu:256-280), intended semantics: wave per
+// (row i of a, 64 rows of b); c bit i*n+j |= any(a row i & b row j).
+__global__ void __launch_bounds__(kBlock) bit_and_kernel(const uint64_t* __restrict__ a,
+                                                         const uint64_t* __restrict__ b,
+                                                         unsigned long long* __restrict__ c, uint32_t m,
+                                                         uint32_t k, uint32_t n) {
+  const uint32_t lane = lane_id();
+  const uint32_t nj = (n + 63) / 64;
+  const uint64_t n_tasks = static_cast<uint64_t>(m) * nj;
+  const uint64_t n_waves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  for (uint64_t task = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; task < n_tasks; task += n_waves) {
+    const uint32_t i = static_cast<uint32_t>(task / nj);
+    const uint32_t jc = static_cast<uint32_t>(task % nj);
+    const uint32_t j = jc * 64 + lane;
+    bool hit = false;
+    if (j < n) {
+      for (uint32_t t0 = 0; t0 < k && !hit; t0 += 64) {
+        const uint32_t len = (k - t0) < 64 ? (k - t0) : 64;
+        const uint64_t wa = bit_window(a, static_cast<uint64_t>(i) * k + t0, len);
+        const uint64_t wb = bit_window(b, static_cast<uint64_t>(j) * k + t0, len);
+        hit = (wa & wb) != 0;
+      }
+    }
+    const uint64_t mask = ballot(hit);
+    if (lane == 0 && mask) {
+      const uint64_t start = static_cast<uint64_t>(i) * n + static_cast<uint64_t>(jc) * 64;
+      const uint32_t s = static_cast<uint32_t>(start & 63);
+      atomicOr(c + (start >> 6), static_cast<unsigned long long>(mask << s));
+      if (s != 0 && (mask >> (64 - s)) != 0)
+        atomicOr(c + (start >> 6) + 1, static_cast<unsigned long long>(mask >> (64 - s)));
+    }
+  }
+}
+
+// Layout row counts: cnt[x] = set bits in row x of an n x n layout.
+__global__ void __launch_bounds__(kBlock) layout_row_count_kernel(const uint64_t* __restrict__ layout,
+                                                                  uint32_t n, uint32_t* __restrict__ cnt) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t x = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; x < n; x += n_waves) {
+    uint64_t c = 0;
+    for (uint32_t y0 = lane * 64; y0 < n; y0 += 64 * kWave) {
+      const uint32_t len = (n - y0) < 64 ? (n - y0) : 64;
+      c += __popcll(bit_window(layout, static_cast<uint64_t>(x) * n + y0, len));
+    }
+    c = wave_sum(c);
+    if (lane == 0) cnt[x] = static_cast<uint32_t>(c);
+  }
+}
+
+// Rank emission: (row_idx, col_idx)[rank] for every set bit, rank = row-major order.
+__global__ void __launch_bounds__(kBlock) layout_emit_kernel(const uint64_t* __restrict__ layout, uint32_t n,
+                                                             const uint32_t* __restrict__ row_start,
+                                                             uint32_t* __restrict__ row_idx,
+                                                             uint32_t* __restrict__ col_idx) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t x = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; x < n; x += n_waves) {
+    uint32_t rank = row_start[x];
+    for (uint32_t y0 = 0; y0 < n; y0 += 64) {
+      const uint32_t len = (n - y0) < 64 ? (n - y0) : 64;
+      const uint64_t w = bit_window(layout, static_cast<uint64_t>(x) * n + y0, len);  // uniform
+      if (w == 0) continue;
+      if ((w >> lane) & 1) {
+        const uint32_t r = rank + __popcll(w & ((1ull << lane) - 1));
+        row_idx[r] = x;
+        col_idx[r] = y0 + lane;
+      }
+      rank += __popcll(w);
+    }
+  }
+}
+
+__global__ void add_counts_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
+}
+
+// --- symbolic pass ---------------------------------------------------------
+
+// Row form of B from Bt: histogram of Bt's column index.
+__global__ void col_hist_kernel(const uint32_t* __restrict__ col, uint32_t n, uint32_t* __restrict__ hist) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(hist + col[i], 1u);
+}
+__global__ void col_scatter_kernel(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, uint32_t n,
+                                   const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+                                   uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t t = col[i];
+    out[start[t] + atomicAdd(cursor + t, 1u)] = row[i];
+  }
+}
+__global__ void mark_referenced_kernel(const uint32_t* __restrict__ col, uint32_t n, uint8_t* __restrict__ ref) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ref[col[i]] = 1;
+}
+__global__ void __launch_bounds__(kBlock) used_right_kernel(const uint8_t* __restrict__ ref,
+                                                            const uint32_t* __restrict__ off_b, uint32_t n,
+                                                            unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    if (ref[t]) local += off_b[t + 1] - off_b[t];
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+}
+
+// Block-wide exclusive scan of one u32 per thread (256 threads).
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d, kWave));
+    if (lane >= static_cast<uint32_t>(d)) incl += o;
+  }
+  if (lane == kWave - 1) s_tmp[wave] = incl;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int w = 0; w < kWavesPerBlock; ++w) {
+    const uint32_t t = s_tmp[w];
+    if (w < static_cast<int>(wave)) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - v;
+}
+
+// Per output row x: union over t in A.row(x) of B.row(t) in an LDS bitmap.
+// EMIT = false: count[x] = |union|, tp[x] = sum |B.row(t)|.
+// EMIT = true : write the union as ascending col_idx at off_c[x], row_idx = x.
+template <bool EMIT>
+__global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
+    uint32_t n_strips, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ col_a,
+    const uint32_t* __restrict__ off_b, const uint32_t* __restrict__ col_b, uint32_t* __restrict__ count,
+    unsigned long long* __restrict__ tp, const uint32_t* __restrict__ off_c, uint32_t* __restrict__ row_c,
+    uint32_t* __restrict__ col_c) {
+  extern __shared__ uint64_t s_bm[];
+  __shared__ uint32_t s_tmp[kWavesPerBlock];
+  __shared__ unsigned long long s_tp;
+  const uint32_t n_words = (n_strips + 63) / 64;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  for (uint32_t x = blockIdx.x; x < n_strips; x += gridDim.x) {
+    for (uint32_t w = threadIdx.x; w < n_words; w += kBlock) s_bm[w] = 0;
+    if (threadIdx.x == 0) s_tp = 0;
+    __syncthreads();
+    const uint32_t a0 = off_a[x], a1 = off_a[x + 1];
+    uint64_t my_tp = 0;
+    for (uint32_t k = a0 + wave; k < a1; k += kWavesPerBlock) {
+      const uint32_t t = col_a[k];
+      const uint32_t r0 = off_b[t], r1 = off_b[t + 1];
+      my_tp += r1 - r0;
+      for (uint32_t j = r0 + lane; j < r1; j += kWave) {
+        const uint32_t y = col_b[j];
+        atomicOr(reinterpret_cast<unsigned long long*>(s_bm + (y >> 6)), 1ull << (y & 63));
+      }
+    }
+    if (!EMIT && lane == 0 && my_tp) atomicAdd(&s_tp, static_cast<unsigned long long>(my_tp));
+    __syncthreads();
+    // Contiguous word range per thread so emitted columns come out ascending.
+    const uint32_t per = (n_words + kBlock - 1) / kBlock;
+    const uint32_t w0 = threadIdx.x * per;
+    const uint32_t w1 = (w0 + per < n_words) ? w0 + per : n_words;
+    uint32_t c = 0;
+    for (uint32_t w = w0; w < w1; ++w) c += __popcll(s_bm[w]);
+    uint32_t total = 0;
+    const uint32_t before = block_exclusive_scan(c, s_tmp, &total);
+    if (!EMIT) {
+      if (threadIdx.x == 0) {
+        count[x] = total;
+        tp[x] = s_tp;
+      }
+    } else {
+      uint32_t pos = off_c[x] + before;
+      for (uint32_t w = w0; w < w1; ++w) {
+        uint64_t bits = s_bm[w];
+        while (bits) {
+          const int b = __ffsll(static_cast<unsigned long long>(bits)) - 1;
+          bits &= bits - 1;
+          row_c[pos] = x;
+          col_c[pos] = w * 64 + b;
+          ++pos;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) sum_u64_kernel(const unsigned long long* __restrict__ v, uint32_t n,
+                                                         unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) local += v[i];
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+}
+
+constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
+
+void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const uint32_t* off_bt,
+                       const uint32_t* row_c, const uint32_t* col_c, const uint32_t* col_a,
+                       const uint32_t* col_bt, const uint64_t* data_a, const uint64_t* data_bt,
+                       uint64_t* data_c) {
+  if (n_c == 0) return;
+  fill_tiles_kernel<<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(n_c, off_a, off_bt, row_c, col_c, col_a,
+                                                                            col_bt, data_a, data_bt, data_c);
+  MG_LAUNCH_CHECK();
+}
+
+}  // namespace
+}  // namespace mgamd
+
+using namespace mgamd;
+
+extern "C" int mg_matrix_bit_and(void* stream, const uint64_t* matrix_a, const uint64_t* matrix_b,
+                                 uint64_t* matrix_c, uint32_t m, uint32_t k, uint32_t n) {
+  return guarded([&] {
+    MG_REQUIRE(matrix_a && matrix_b && matrix_c, "mg_matrix_bit_and: NULL buffer");
+    if (m == 0 || n == 0 || k == 0) return;
+    const uint64_t tasks = static_cast<uint64_t>(m) * ((n + 63) / 64);
+    bit_and_kernel<<<grid_for(tasks, kWavesPerBlock, 65536), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+        matrix_a, matrix_b, reinterpret_cast<unsigned long long*>(matrix_c), m, k, n);
+    MG_LAUNCH_CHECK();
+  });
+}
+
+extern "C" int mg_init_bit_tiled_matrix_metadata_by_layout_matrix(void* stream, const uint64_t* layout_matrix,
+                                                                  uint32_t n_strips, uint32_t* tile_count_row,
+                                                                  uint32_t* tile_row_idx, uint32_t* tile_col_idx) {
+  return guarded([&] {
+    MG_REQUIRE(layout_matrix && tile_count_row && tile_row_idx && tile_col_idx,
+               "mg_init_bit_tiled_matrix_metadata_by_layout_matrix: NULL buffer");
+    if (n_strips == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // Temporary row counts and their exclusive scan (the reference computes ranks
+    // with an O(words) prefix popcount per bit instead, matrix_operations.cu:149-174).
+    DevBuf<uint32_t> cnt(n_strips), start(n_strips + 1ull);
+    layout_row_count_kernel<<<grid_for(n_strips, kWavesPerBlock, 65536), kBlock, 0, s>>>(layout_matrix, n_strips,
+                                                                                          cnt.get());
+    MG_LAUNCH_CHECK();
+    mg_context tmp;  // scan helper needs only the stream
+    tmp.stream = s;
+    exclusive_scan_u32(&tmp, cnt.get(), start.get(), n_strips);
+    layout_emit_kernel<<<grid_for(n_strips, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+        layout_matrix, n_strips, start.get(), tile_row_idx, tile_col_idx);
+    MG_LAUNCH_CHECK();
+    add_counts_kernel<<<grid_for(n_strips, 256), 256, 0, s>>>(tile_count_row, cnt.get(), n_strips);
+    MG_LAUNCH_CHECK();
+    MG_HIP(hipStreamSynchronize(s));  // temporaries are freed on return
+  });
+}
+
+extern "C" int mg_fill_tiles(void* stream, size_t tile_size, size_t n_strips, size_t n_nz_tile_a,
+                             size_t n_nz_tile_b, size_t n_nz_tile_c, const uint64_t* layout_matrix_c,
+                             const uint32_t* tile_offset_row_a, const uint32_t* tile_offset_row_b,
+                             const uint32_t* tile_offset_row_c, const uint32_t* tile_row_idx_a,
+                             const uint32_t* tile_row_idx_b, const uint32_t* tile_row_idx_c,
+                             const uint32_t* tile_col_idx_a, const uint32_t* tile_col_idx_b,
+                             const uint32_t* tile_col_idx_c, const uint64_t* data_a, const uint64_t* data_b,
+                             uint64_t* data_c) {
+  (void)layout_matrix_c; (void)tile_row_idx_a; (void)tile_row_idx_b; (void)tile_offset_row_c;
+  (void)n_nz_tile_a; (void)n_nz_tile_b; (void)n_strips;
+  return guarded([&] {
+    if (tile_size != kTile) fail(MG_ERR_UNSUPPORTED, "mg_fill_tiles: tile_size must be 64");
+    if (n_nz_tile_c == 0) return;
+    MG_REQUIRE(tile_offset_row_a && tile_offset_row_b && tile_row_idx_c && tile_col_idx_a && tile_col_idx_b &&
+                   tile_col_idx_c && data_a && data_b && data_c,
+               "mg_fill_tiles: NULL buffer");
+    MG_REQUIRE(n_nz_tile_c < (1ull << 32), "mg_fill_tiles: n_nz_tile_c exceeds u32");
+    launch_fill_tiles(static_cast<hipStream_t>(stream), static_cast<uint32_t>(n_nz_tile_c), tile_offset_row_a,
+                      tile_offset_row_b, tile_row_idx_c, tile_col_idx_c, tile_col_idx_a, tile_col_idx_b, data_a,
+                      data_b, data_c);
+  });
+}
+
+extern "C" int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                    mg_tiled_matrix** c_out, mg_product_stats* stats) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && c_out, "mg_multiply_symbolic: NULL argument");
+    *c_out = nullptr;
+    MG_REQUIRE(a->n_strips == bt->n_strips, "mg_multiply_symbolic: operands differ in n_strips");
+    const uint32_t n = a->n_strips;
+    const uint32_t n_words = (n + 63) / 64;
+    if (n_words > kMaxLdsBitmapWords)
+      fail(MG_ERR_UNSUPPORTED, "mg_multiply_symbolic: n_strips > 524288 not supported");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    // Row form of B: B.row(t) = { y : Bt(y,t) present }.
+    DevBuf<uint32_t> hist(n), off_b(n + 1ull), cursor(n), col_b(bt->n_nz_tile ? bt->n_nz_tile : 1);
+    MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * n, s));
+    MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * n, s));
+    if (bt->n_nz_tile) {
+      col_hist_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile, hist.get());
+      MG_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(ctx, hist.get(), off_b.get(), n);
+    if (bt->n_nz_tile) {
+      col_scatter_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->row_idx.get(), bt->col_idx.get(),
+                                                                      bt->n_nz_tile, off_b.get(), cursor.get(),
+                                                                      col_b.get());
+      MG_LAUNCH_CHECK();
+    }
+    // Count pass.
+    DevBuf<uint32_t> cnt(n), off_c(n + 1ull);
+    DevBuf<unsigned long long> tp(n), sums(2);
+    MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 2, s));
+    const size_t lds = sizeof(uint64_t) * n_words;
+    const unsigned grid = grid_for(n, 1, 16384);
+    symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), off_b.get(),
+                                                         col_b.get(), cnt.get(), tp.get(), nullptr, nullptr,
+                                                         nullptr);
+    MG_LAUNCH_CHECK();
+    exclusive_scan_u32(ctx, cnt.get(), off_c.get(), n);
+    sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(tp.get(), n, sums.get());
+    MG_LAUNCH_CHECK();
+    DevBuf<uint8_t> ref(n);
+    MG_HIP(hipMemsetAsync(ref.get(), 0, n, s));
+    if (a->n_nz_tile) {
+      mark_referenced_kernel<<<grid_for(a->n_nz_tile, 256), 256, 0, s>>>(a->col_idx.get(), a->n_nz_tile, ref.get());
+      MG_LAUNCH_CHECK();
+    }
+    used_right_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(ref.get(), off_b.get(), n, sums.get() + 1);
+    MG_LAUNCH_CHECK();
+    uint32_t n_c = 0;
+    unsigned long long h_sums[2] = {0, 0};
+    MG_HIP(hipMemcpyAsync(&n_c, off_c.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipMemcpyAsync(h_sums, sums.get(), sizeof(h_sums), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    auto c = std::make_unique<mg_tiled_matrix>();
+    c->device = ctx->device;
+    alloc_tiled_matrix(c.get(), n, n_c);
+    MG_HIP(hipMemcpyAsync(c->offset_row.get(), off_c.get(), sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToDevice,
+                          s));
+    if (n_c) {
+      MG_HIP(hipMemsetAsync(c->data.get(), 0, c->data.bytes(), s));
+      symbolic_row_kernel<true><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), off_b.get(),
+                                                          col_b.get(), nullptr, nullptr, off_c.get(),
+                                                          c->row_idx.get(), c->col_idx.get());
+      MG_LAUNCH_CHECK();
+    }
+    MG_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->tile_products = h_sums[0];
+      stats->right_tiles_used = h_sums[1];
+      stats->out_tiles = n_c;
+      stats->alg_bytes = 520ull * (a->n_nz_tile + h_sums[1] + n_c) + 4ull * (2ull * n + 2);
+    }
+    *c_out = c.release();
+  });
+}
+
+extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                   mg_tiled_matrix* c) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && c, "mg_multiply_numeric: NULL argument");
+    MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
+               "mg_multiply_numeric: operands differ in n_strips");
+    DeviceGuard g(ctx->device);
+    launch_fill_tiles(ctx->stream, c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(),
+                      c->col_idx.get(), a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(),
+                      c->data.get());
+  });
+}

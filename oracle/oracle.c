@@ -1,0 +1,530 @@
+/*
+ * CPU oracle (test infrastructure only; see oracle.h for what pins each function).
+ * Plain C11 + OpenMP. Never linked into the product library.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* RNG and digests                                                            */
+/* ------------------------------------------------------------------------- */
+
+uint64_t or_sm64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t or_sm64_at(uint64_t key, uint64_t i) { return or_sm64_mix(key + (i + 1) * 0x9E3779B97F4A7C15ull); }
+
+uint64_t or_stream_key(uint64_t seed, uint64_t tag) {
+  return or_sm64_mix(or_sm64_mix(seed ^ 0x6A09E667F3BCC909ull) + tag);
+}
+
+uint64_t or_digest_term(uint64_t index, uint64_t value) {
+  return or_sm64_mix(index * 0xD1B54A32D192ED03ull ^ or_sm64_mix(value + 0x8CB92BA72F3D8DD7ull));
+}
+
+int or_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static void set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* Radix sort of u64 keys (LSD, 16-bit digits), used by generators/converter.  */
+/* ------------------------------------------------------------------------- */
+
+static void radix_sort_u64(uint64_t* keys, uint64_t n, int bits) {
+  if (n < 2) return;
+  uint64_t* tmp = (uint64_t*)malloc(sizeof(uint64_t) * n);
+  uint64_t* cnt = (uint64_t*)malloc(sizeof(uint64_t) * 65536);
+  uint64_t* src = keys;
+  uint64_t* dst = tmp;
+  for (int shift = 0; shift < bits; shift += 16) {
+    memset(cnt, 0, sizeof(uint64_t) * 65536);
+    for (uint64_t i = 0; i < n; ++i) cnt[(src[i] >> shift) & 0xFFFF]++;
+    uint64_t sum = 0;
+    for (int d = 0; d < 65536; ++d) {
+      const uint64_t c = cnt[d];
+      cnt[d] = sum;
+      sum += c;
+    }
+    for (uint64_t i = 0; i < n; ++i) dst[cnt[(src[i] >> shift) & 0xFFFF]++] = src[i];
+    uint64_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != keys) memcpy(keys, src, sizeof(uint64_t) * n);
+  free(tmp);
+  free(cnt);
+}
+
+static uint64_t unique_sorted(uint64_t* keys, uint64_t n) {
+  if (n == 0) return 0;
+  uint64_t w = 1;
+  for (uint64_t i = 1; i < n; ++i)
+    if (keys[i] != keys[w - 1]) keys[w++] = keys[i];
+  return w;
+}
+
+uint64_t or_sort_unique_edges(uint64_t n, uint32_t* src, uint32_t* dst) {
+  uint64_t* k = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+  for (uint64_t i = 0; i < n; ++i) k[i] = ((uint64_t)src[i] << 32) | dst[i];
+  radix_sort_u64(k, n, 64);
+  const uint64_t u = unique_sorted(k, n);
+  for (uint64_t i = 0; i < u; ++i) {
+    src[i] = (uint32_t)(k[i] >> 32);
+    dst[i] = (uint32_t)k[i];
+  }
+  free(k);
+  return u;
+}
+
+uint64_t or_edges_digest(uint64_t n, const uint32_t* src, const uint32_t* dst) {
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) acc += or_digest_term(((uint64_t)src[i] << 32) | dst[i], 0xE0E0ull);
+  return acc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Generators                                                                 */
+/* ------------------------------------------------------------------------- */
+
+static uint32_t prob_threshold(double p) {
+  const double t = floor(p * 4294967296.0);
+  if (!(t > 0.0)) return 0;
+  if (t >= 4294967295.0) return 4294967295u;
+  return (uint32_t)t;
+}
+
+/* R-MAT (Chakrabarti, Zhan, Faloutsos, SDM 2004): per draw e and level l (MSB first),
+ * a 32-bit uniform r = sm64_at(K, e*scale + l) >> 32 picks the quadrant
+ * a (0,0) / b (0,1) / c (1,0) / d (1,1) against cumulative thresholds. The configs
+ * ask for no noise and no vertex permutation; self-loops and duplicates are removed. */
+int64_t or_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                      uint32_t* src, uint32_t* dst, uint64_t cap) {
+  const uint64_t n_draws = (uint64_t)edge_factor << scale;
+  const uint64_t key = or_stream_key(seed, 1);
+  const uint32_t ta = prob_threshold(a), tab = prob_threshold(a + b), tabc = prob_threshold(a + b + c);
+  uint64_t* k = (uint64_t*)malloc(sizeof(uint64_t) * (n_draws ? n_draws : 1));
+  uint64_t m = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < (int64_t)n_draws; ++e) {
+    uint32_t u = 0, v = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      const uint32_t r = (uint32_t)(or_sm64_at(key, (uint64_t)e * scale + l) >> 32);
+      const uint32_t bit = 1u << (scale - 1 - l);
+      if (r < ta) {
+      } else if (r < tab) {
+        v |= bit;
+      } else if (r < tabc) {
+        u |= bit;
+      } else {
+        u |= bit;
+        v |= bit;
+      }
+    }
+    k[e] = (u == v) ? ~0ull : (((uint64_t)u << 32) | v);
+  }
+  radix_sort_u64(k, n_draws, 64);
+  m = unique_sorted(k, n_draws);
+  if (m > 0 && k[m - 1] == ~0ull) --m;
+  if (m > cap) {
+    free(k);
+    return -1;
+  }
+  for (uint64_t i = 0; i < m; ++i) {
+    src[i] = (uint32_t)(k[i] >> 32);
+    dst[i] = (uint32_t)k[i];
+  }
+  free(k);
+  return (int64_t)m;
+}
+
+/* 64 independent Bernoulli(p_num / 2^m) bits: lane i compares its m-bit uniform U_i
+ * (bit m-1-j = bit i of draw j) with p_num, most significant bit first. */
+static uint64_t bernoulli_word(uint64_t key, uint64_t word_ctr, uint64_t p_num, uint32_t m) {
+  uint64_t lt = 0, eq = ~0ull;
+  for (uint32_t j = 0; j < m; ++j) {
+    const uint32_t bitpos = m - 1 - j;
+    const uint64_t tb = ((p_num >> bitpos) & 1) ? ~0ull : 0ull;
+    const uint64_t r = or_sm64_at(key, word_ctr * m + j);
+    lt |= eq & ~r & tb;
+    eq &= ~(r ^ tb);
+    if (eq == 0) break;
+  }
+  return lt;
+}
+
+/* Rows u (in increasing order), columns [c0, c1), no diagonal. */
+static int64_t bernoulli_rows(uint32_t n_vertices, uint64_t key, uint64_t p_num, uint32_t m, uint32_t n_rows,
+                              uint32_t comm_size, uint32_t stride, uint32_t* src, uint32_t* dst, uint64_t cap) {
+  const uint32_t n_blocks = (n_vertices + 63) / 64;
+  uint64_t* cnt = (uint64_t*)calloc((size_t)n_rows + 1, sizeof(uint64_t));
+  for (int pass = 0; pass < 2; ++pass) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t row = 0; row < (int64_t)n_rows; ++row) {
+      uint32_t u, c0, c1;
+      if (comm_size == 0) {
+        u = (uint32_t)row;
+        c0 = 0;
+        c1 = n_vertices;
+      } else {
+        const uint32_t cidx = (uint32_t)row / comm_size;
+        c0 = cidx * stride;
+        u = c0 + (uint32_t)row % comm_size;
+        c1 = c0 + comm_size;
+      }
+      uint64_t pos = pass ? cnt[row] : 0, total = 0;
+      for (uint32_t j = c0 / 64; j < (c1 + 63) / 64; ++j) {
+        uint64_t w = bernoulli_word(key, (uint64_t)u * n_blocks + j, p_num, m);
+        const uint32_t lo = j * 64;
+        if (c0 > lo) w &= ~0ull << (c0 - lo);
+        if (c1 < lo + 64) w &= (1ull << (c1 - lo)) - 1;
+        if (u >= lo && u < lo + 64) w &= ~(1ull << (u - lo));
+        if (pass == 0) {
+          total += (uint64_t)__builtin_popcountll(w);
+        } else {
+          while (w) {
+            const int bb = __builtin_ctzll(w);
+            w &= w - 1;
+            if (pos < cap) {
+              src[pos] = u;
+              dst[pos] = lo + (uint32_t)bb;
+            }
+            ++pos;
+          }
+        }
+      }
+      if (pass == 0) cnt[row + 1] = total;
+    }
+    if (pass == 0) {
+      for (uint32_t r = 0; r < n_rows; ++r) cnt[r + 1] += cnt[r];
+      if (cnt[n_rows] > cap) {
+        free(cnt);
+        return -1;
+      }
+    }
+  }
+  const int64_t total = (int64_t)cnt[n_rows];
+  free(cnt);
+  return total;
+}
+
+int64_t or_er_edges(uint32_t log2_n, uint64_t p_num, uint32_t p_log2_den, uint64_t seed, uint32_t* src,
+                    uint32_t* dst, uint64_t cap) {
+  const uint32_t n = 1u << log2_n;
+  return bernoulli_rows(n, or_stream_key(seed, 2), p_num, p_log2_den, n, 0, 0, src, dst, cap);
+}
+
+int64_t or_community_edges(uint32_t n_vertices, uint32_t n_comm, uint32_t comm_size, uint32_t stride,
+                           uint64_t p_num, uint32_t p_log2_den, uint64_t seed, uint32_t* src, uint32_t* dst,
+                           uint64_t cap) {
+  return bernoulli_rows(n_vertices, or_stream_key(seed, 3), p_num, p_log2_den, n_comm * comm_size, comm_size,
+                        stride, src, dst, cap);
+}
+
+/* Floyd's sampling (Bentley & Floyd, CACM 1987) over the "sources" stream. */
+void or_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t seed, uint32_t* out) {
+  const uint64_t key = or_stream_key(seed, 4);
+  uint32_t cap = 16;
+  while (cap < 4 * count) cap <<= 1;
+  uint32_t* table = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+  memset(table, 0xFF, sizeof(uint32_t) * cap);
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint64_t j = (uint64_t)n_vertices - count + k;
+    const uint32_t r = (uint32_t)((or_sm64_at(key, k) >> 11) % (j + 1));
+    /* probe for r */
+    uint32_t h = (uint32_t)(or_sm64_mix(r) & (cap - 1));
+    int found = 0;
+    while (table[h] != 0xFFFFFFFFu) {
+      if (table[h] == r) {
+        found = 1;
+        break;
+      }
+      h = (h + 1) & (cap - 1);
+    }
+    const uint32_t pick = found ? (uint32_t)j : r;
+    h = (uint32_t)(or_sm64_mix(pick) & (cap - 1));
+    while (table[h] != 0xFFFFFFFFu) h = (h + 1) & (cap - 1);
+    table[h] = pick;
+    out[k] = pick;
+  }
+  free(table);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Edge list -> tiles (format_converter.cuh:292-415 with n_chunks = 1):        */
+/* tile (x, y) = (u / 64, v / 64), bit (u % 64) * 64 + (v % 64), i.e. word     */
+/* u % 64, bit v % 64; tiles in row-major (x, y) order; off = prefix of row     */
+/* counts (format_converter.cuh:333-337).                                      */
+/* ------------------------------------------------------------------------- */
+
+static uint64_t* tile_keys(uint32_t n_strips, uint64_t n, const uint32_t* src, const uint32_t* dst, int transpose) {
+  uint64_t* k = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t a = src[i], b = dst[i];
+    if (transpose) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint64_t tile = (uint64_t)(a >> 6) * n_strips + (b >> 6);
+    k[i] = (tile << 12) | ((uint64_t)(a & 63) << 6) | (b & 63);
+  }
+  radix_sort_u64(k, n, 64);
+  return k;
+}
+
+uint64_t or_count_tiles(uint32_t n_vertices, uint64_t n_edges, const uint32_t* src, const uint32_t* dst,
+                        int transpose) {
+  const uint32_t n_strips = (n_vertices + 63) / 64;
+  uint64_t* k = tile_keys(n_strips, n_edges, src, dst, transpose);
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n_edges; ++i)
+    if (i == 0 || (k[i] >> 12) != (k[i - 1] >> 12)) ++t;
+  free(k);
+  return t;
+}
+
+void or_edges_to_tiles(uint32_t n_vertices, uint64_t n_edges, const uint32_t* src, const uint32_t* dst,
+                       int transpose, uint32_t* off, uint32_t* row, uint32_t* col, uint64_t* data) {
+  const uint32_t n_strips = (n_vertices + 63) / 64;
+  uint64_t* k = tile_keys(n_strips, n_edges, src, dst, transpose);
+  memset(off, 0, sizeof(uint32_t) * ((size_t)n_strips + 1));
+  int64_t t = -1;
+  for (uint64_t i = 0; i < n_edges; ++i) {
+    const uint64_t tile = k[i] >> 12;
+    if (i == 0 || tile != (k[i - 1] >> 12)) {
+      ++t;
+      row[t] = (uint32_t)(tile / n_strips);
+      col[t] = (uint32_t)(tile % n_strips);
+      memset(data + (uint64_t)t * 64, 0, 64 * sizeof(uint64_t));
+      off[row[t] + 1]++;
+    }
+    data[(uint64_t)t * 64 + ((k[i] >> 6) & 63)] |= 1ull << (k[i] & 63);
+  }
+  for (uint32_t x = 0; x < n_strips; ++x) off[x + 1] += off[x];
+  free(k);
+}
+
+uint64_t or_tiles_digest(uint32_t n_strips, uint32_t n_nz_tile, const uint32_t* row, const uint32_t* col,
+                         const uint64_t* data) {
+  uint64_t acc = 0;
+  for (uint32_t t = 0; t < n_nz_tile; ++t) {
+    const uint64_t key = (uint64_t)row[t] * n_strips + col[t];
+    acc += or_digest_term(~key, 0x5EEDull);
+    for (uint32_t w = 0; w < 64; ++w) {
+      const uint64_t v = data[(uint64_t)t * 64 + w];
+      if (v) acc += or_digest_term(key * 64 + w, v);
+    }
+  }
+  return acc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* O1 / O2 / O3                                                               */
+/* ------------------------------------------------------------------------- */
+
+void or_bool_matmul_bruteforce(uint32_t n, const uint8_t* a, const uint8_t* b, uint8_t* c) {
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = 0; j < n; ++j) {
+      uint8_t v = 0;
+      for (uint32_t k = 0; k < n; ++k) v |= (uint8_t)(a[(size_t)i * n + k] & b[(size_t)k * n + j]);
+      c[(size_t)i * n + j] = v;
+    }
+}
+
+/* Dense bitset product: row i of c = OR of rows k of b over the set bits k of row i of a. */
+void or_bitset_matmul(uint32_t m, uint32_t k, uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* c,
+                      int threads) {
+  const uint32_t ka = (k + 63) / 64, nb = (n + 63) / 64;
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    uint64_t* ci = c + (uint64_t)i * nb;
+    memset(ci, 0, sizeof(uint64_t) * nb);
+    for (uint32_t wa = 0; wa < ka; ++wa) {
+      uint64_t bits = a[(uint64_t)i * ka + wa];
+      while (bits) {
+        const uint32_t kk = wa * 64 + (uint32_t)__builtin_ctzll(bits);
+        bits &= bits - 1;
+        const uint64_t* bk = b + (uint64_t)kk * nb;
+        for (uint32_t w = 0; w < nb; ++w) ci[w] |= bk[w];
+      }
+    }
+  }
+}
+
+/* Vertex-major frontier (bit-parallel over sources): y[v] = OR_{u in in(v)} x[u]. */
+void or_frontier_step(uint32_t n, const uint64_t* in_off, const uint32_t* in_src, uint32_t W, const uint64_t* x,
+                      uint64_t* y, int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t v = 0; v < (int64_t)n; ++v) {
+    uint64_t* yv = y + (uint64_t)v * W;
+    memset(yv, 0, sizeof(uint64_t) * W);
+    for (uint64_t e = in_off[v]; e < in_off[v + 1]; ++e) {
+      const uint64_t* xu = x + (uint64_t)in_src[e] * W;
+      for (uint32_t w = 0; w < W; ++w) yv[w] |= xu[w];
+    }
+  }
+}
+
+uint64_t or_frontier_digest(uint64_t n_vertices, uint32_t W, uint32_t n_words, const uint64_t* x) {
+  uint64_t acc = 0;
+  for (uint64_t v = 0; v < n_vertices; ++v)
+    for (uint32_t w = 0; w < W && w < n_words; ++w) {
+      const uint64_t val = x[v * W + w];
+      if (val) acc += or_digest_term(v * 64 + w, val);
+    }
+  return acc;
+}
+
+void or_in_csr(uint32_t n_vertices, uint64_t n_edges, const uint32_t* src, const uint32_t* dst, uint64_t* in_off,
+               uint32_t* in_src) {
+  memset(in_off, 0, sizeof(uint64_t) * ((size_t)n_vertices + 1));
+  for (uint64_t i = 0; i < n_edges; ++i) in_off[dst[i] + 1]++;
+  for (uint32_t v = 0; v < n_vertices; ++v) in_off[v + 1] += in_off[v];
+  uint64_t* cur = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n_vertices + 1));
+  memcpy(cur, in_off, sizeof(uint64_t) * ((size_t)n_vertices + 1));
+  for (uint64_t i = 0; i < n_edges; ++i) in_src[cur[dst[i]]++] = src[i];
+  free(cur);
+}
+
+/* ------------------------------------------------------------------------- */
+/* O4: tile algorithm                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* Symbolic (MatrixBitAnd + MatrixBitCount + InitBitTiledMatrixMetadataByLayoutMatrix,
+ * matrix_operations.cu:256-316, intended): C(x,y) present iff A.row(x) and Bt.row(y)
+ * share a tile column; listed row-major. */
+int64_t or_tile_symbolic(uint32_t n_strips, const uint32_t* a_off, const uint32_t* a_col, const uint32_t* bt_off,
+                         const uint32_t* bt_col, uint32_t* c_off, uint32_t* c_row, uint32_t* c_col, uint64_t cap,
+                         uint64_t* tile_products) {
+  uint64_t* b_row_len = (uint64_t*)calloc(n_strips, sizeof(uint64_t));
+  for (uint32_t i = 0; i < bt_off[n_strips]; ++i) b_row_len[bt_col[i]]++;
+  uint64_t tp = 0, n = 0;
+  c_off[0] = 0;
+  for (uint32_t x = 0; x < n_strips; ++x) {
+    for (uint32_t i = a_off[x]; i < a_off[x + 1]; ++i) tp += b_row_len[a_col[i]];
+    for (uint32_t y = 0; y < n_strips; ++y) {
+      uint32_t i = a_off[x], j = bt_off[y];
+      int hit = 0;
+      while (i < a_off[x + 1] && j < bt_off[y + 1]) {
+        if (a_col[i] < bt_col[j]) ++i;
+        else if (a_col[i] > bt_col[j]) ++j;
+        else { hit = 1; break; }
+      }
+      if (hit) {
+        if (n < cap) {
+          c_row[n] = x;
+          c_col[n] = y;
+        }
+        ++n;
+      }
+    }
+    c_off[x + 1] = (uint32_t)n;
+  }
+  free(b_row_len);
+  if (tile_products) *tile_products = tp;
+  return n > cap ? -1 : (int64_t)n;
+}
+
+/* Numeric (fill_tiles_kernel + single_thread_matrix_bit_and, matrix_operations.cu:318-366,
+ * 212-233, intended): C(x,y)[r] bit c |= ((A(x,t)[r] & Bt(y,t)[c]) != 0) for every common t. */
+void or_tile_numeric(uint32_t n_strips, const uint32_t* a_off, const uint32_t* a_col, const uint64_t* a_data,
+                     const uint32_t* bt_off, const uint32_t* bt_col, const uint64_t* bt_data, uint32_t n_c,
+                     const uint32_t* c_row, const uint32_t* c_col, uint64_t* c_data, int threads) {
+  (void)n_strips;
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < (int64_t)n_c; ++q) {
+    const uint32_t x = c_row[q], y = c_col[q];
+    uint64_t* ct = c_data + (uint64_t)q * 64;
+    uint32_t i = a_off[x], j = bt_off[y];
+    while (i < a_off[x + 1] && j < bt_off[y + 1]) {
+      if (a_col[i] < bt_col[j]) {
+        ++i;
+      } else if (a_col[i] > bt_col[j]) {
+        ++j;
+      } else {
+        const uint64_t* at = a_data + (uint64_t)i * 64;
+        const uint64_t* bt = bt_data + (uint64_t)j * 64;
+        for (uint32_t r = 0; r < 64; ++r)
+          for (uint32_t c = 0; c < 64; ++c)
+            if (at[r] & bt[c]) ct[r] |= 1ull << c;
+        ++i;
+        ++j;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Walk hop on tiles (the CPU baseline): same algorithm as the GPU kernel.    */
+/* ------------------------------------------------------------------------- */
+
+uint64_t or_walk_tile_hop(uint32_t n_strips, const uint32_t* at_off, const uint32_t* at_col, const uint64_t* at_data,
+                          uint32_t W, const uint64_t* x_in, uint64_t* x_out, uint32_t strip_begin,
+                          uint32_t strip_end, uint32_t stride, int threads) {
+  (void)n_strips;
+  uint64_t visited = 0;
+  if (stride == 0) stride = 1;
+  const int64_t n_sel = strip_end > strip_begin ? ((int64_t)strip_end - strip_begin + stride - 1) / stride : 0;
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : visited)
+  for (int64_t i = 0; i < n_sel; ++i) {
+    const int64_t y = (int64_t)strip_begin + i * stride;
+    uint64_t* acc = x_out + (uint64_t)y * 64 * W;
+    memset(acc, 0, sizeof(uint64_t) * 64 * W);
+    for (uint32_t p = at_off[y]; p < at_off[y + 1]; ++p) {
+      const uint64_t t = at_col[p];
+      const uint64_t* tile = at_data + (uint64_t)p * 64;
+      for (uint32_t r = 0; r < 64; ++r) {
+        uint64_t bits = tile[r];
+        while (bits) {
+          const uint32_t tp = (uint32_t)__builtin_ctzll(bits);
+          bits &= bits - 1;
+          ++visited;
+          const uint64_t* src = x_in + (t * 64 + tp) * W;
+          uint64_t* dst = acc + (uint64_t)r * W;
+          for (uint32_t w = 0; w < W; ++w) dst[w] |= src[w];
+        }
+      }
+    }
+  }
+  return visited;
+}
+
+uint64_t or_walk_symbolic(uint32_t n_strips, const uint32_t* at_off, const uint32_t* at_col, const uint64_t* s_in,
+                          uint64_t* s_out) {
+  uint64_t tp = 0;
+  for (uint32_t y = 0; y < n_strips; ++y) {
+    uint64_t s = 0;
+    for (uint32_t p = at_off[y]; p < at_off[y + 1]; ++p) {
+      s |= s_in[at_col[p]];
+      tp += (uint64_t)__builtin_popcountll(s_in[at_col[p]]);
+    }
+    s_out[y] = s;
+  }
+  return tp;
+}

@@ -1,0 +1,90 @@
+"""GPU input production vs the CPU oracle: generators, set operations, converter (bit-exact)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def mg():
+    import matrixgraph_amd
+
+    return matrixgraph_amd
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 1), (14, 1), (16, 7)])
+def test_rmat_matches_oracle(ctx, scale, seed):
+    el = mg().EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed)
+    s, d = el.download()
+    os_, od = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, seed)
+    assert np.array_equal(s, os_) and np.array_equal(d, od)
+    assert el.digest() == O.edges_digest(os_, od)
+
+
+def test_er_and_communities_match_oracle(ctx):
+    el = mg().EdgeList.er(ctx, 12, 1, 6, 5)
+    s, d = el.download()
+    os_, od = O.er_edges(12, 1, 6, 5)
+    assert np.array_equal(s, os_) and np.array_equal(d, od)
+    cm = mg().EdgeList.communities(ctx, 16384, 4, 512, 4096, 1288490189, 32, 3)
+    cs, cd = cm.download()
+    ocs, ocd = O.community_edges(16384, 4, 512, 4096, 1288490189, 32, 3)
+    assert np.array_equal(cs, ocs) and np.array_equal(cd, ocd)
+    rm = mg().EdgeList.rmat(ctx, 14, 16, 0.57, 0.19, 0.19, 3)
+    un = rm.union(cm)
+    us, ud = un.download()
+    rs, rd = O.rmat_edges(14, 16, 0.57, 0.19, 0.19, 3)
+    es, ed = O.sort_unique_edges(np.concatenate([rs, ocs]), np.concatenate([rd, ocd]))
+    assert np.array_equal(us, es) and np.array_equal(ud, ed)
+
+
+def test_upload_keeps_self_loops_and_dedups(ctx):
+    g = json.loads((GOLDEN / "toy_graph.json").read_text())
+    e = np.array(g["edges"] + g["edges"][:5], np.uint32)
+    el = mg().EdgeList.upload(ctx, g["n_vertices"], e[:, 0], e[:, 1])
+    s, d = el.download()
+    es, ed = O.sort_unique_edges(e[:, 0], e[:, 1])
+    assert len(s) == 25 and np.array_equal(s, es) and np.array_equal(d, ed)
+    assert np.any(s == d)
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_tiles_match_oracle(ctx, scale):
+    el = mg().EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 1)
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, 1)
+    n = 1 << scale
+    for transpose in (False, True):
+        m = el.to_tiled_matrix(transpose).download()
+        ns, off, row, col, data = O.edges_to_tiles(n, s, d, transpose)
+        assert m.n_strips == ns
+        assert np.array_equal(m.offset_row, off) and np.array_equal(m.row_idx, row)
+        assert np.array_equal(m.col_idx, col) and np.array_equal(m.data, data)
+
+
+def test_tile_upload_roundtrip_and_validation(ctx):
+    M = mg()
+    s, d = O.rmat_edges(10, 16, 0.57, 0.19, 0.19, 4)
+    ns, off, row, col, data = O.edges_to_tiles(1024, s, d)
+    h = M.HostTiledMatrix(ns, off, row, col, data)
+    t = M.TiledMatrix.upload(ctx, h)
+    back = t.download()
+    assert np.array_equal(back.data, data) and np.array_equal(back.offset_row, off)
+    assert t.digest() == O.tiles_digest(ns, row, col, data)
+    assert t.popcount() == len(s)
+    bad = M.HostTiledMatrix(ns, off.copy(), row, col.copy(), data)
+    bad.col_idx[0] = ns  # out of range
+    with pytest.raises(M.MGError) as e:
+        M.TiledMatrix.upload(ctx, bad)
+    assert e.value.code == 1
+    bad2 = M.HostTiledMatrix(ns, off, row, col, data, tile_size=32)
+    with pytest.raises(M.MGError) as e:
+        M.TiledMatrix.upload(ctx, bad2)
+    assert e.value.code == 4
+    # empty matrix
+    empty = M.TiledMatrix.upload(ctx, M.HostTiledMatrix.empty(4))
+    assert empty.n_nz_tile == 0 and empty.popcount() == 0

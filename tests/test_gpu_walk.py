@@ -1,0 +1,179 @@
+"""GPU walks X_{k+1} = X_k * A (A supplied transposed) vs the CPU oracles: dense frontier
+content (bit-exact), structural tile sets and tile-product counts, the reference-layout
+export, pruning, edge cases and the full-size R-MAT-22 4-hop walk (digests)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def mg():
+    import matrixgraph_amd
+
+    return matrixgraph_amd
+
+
+def setup_graph(ctx, scale, seed, ef=16):
+    M = mg()
+    el = M.EdgeList.rmat(ctx, scale, ef, 0.57, 0.19, 0.19, seed)
+    at = el.to_tiled_matrix(transpose=True)
+    s, d = O.rmat_edges(scale, ef, 0.57, 0.19, 0.19, seed)
+    return el, at, s, d
+
+
+def s_masks(n_strips, sources):
+    s = np.zeros(n_strips, np.uint64)
+    for i, v in enumerate(np.asarray(sources, np.int64)):
+        s[v // 64] |= np.uint64(1) << np.uint64(i // 64)
+    return s
+
+
+def numeric_masks(x):
+    """Per strip: bit w set iff some vertex of the strip has a bit in source word w."""
+    ns = x.shape[0] // 64
+    nz = (x.reshape(ns, 64, x.shape[1]) != 0).any(axis=1)
+    return (nz.astype(np.uint64) << np.arange(x.shape[1], dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+
+
+def popcount(a):
+    return int(np.unpackbits(np.ascontiguousarray(a).view(np.uint8)).sum())
+
+
+@pytest.mark.parametrize("scale,seed,n_src,hops", [(10, 1, 64, 4), (12, 4, 1, 3), (12, 5, 65, 4), (14, 2, 4096, 4),
+                                                   (14, 6, 1000, 2)])
+def test_walk_matches_oracle(ctx, scale, seed, n_src, hops):
+    M = mg()
+    n = 1 << scale
+    el, at, s, d = setup_graph(ctx, scale, seed)
+    sources = O.sample_sources(n, n_src, seed)
+    w = M.Walk(ctx, at, n_src, hops)
+    w.run(sources, hops)
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=w.words_per_vertex)
+    ns = n // 64
+    ath = at.download()
+    s_cur = s_masks(ns, sources)
+    for k in range(1, hops + 1):
+        got = w.frontier(k)
+        assert np.array_equal(got, xs[k]), f"hop {k}"
+        st = w.hop_stats(k)
+        s_next, tp = O.walk_symbolic(ns, ath.offset_row, ath.col_idx, s_cur)
+        assert st["tile_products"] == tp
+        assert st["left_tiles"] == sum(bin(int(v)).count("1") for v in s_cur)
+        assert st["out_tiles"] == sum(bin(int(v)).count("1") for v in s_next)
+        assert st["right_tiles_used"] == int(np.count_nonzero(s_cur[ath.col_idx]))
+        assert st["popcount"] == popcount(xs[k])
+        assert st["out_tiles_numeric"] == sum(bin(int(v)).count("1") for v in numeric_masks(xs[k]))
+        assert st["out_tiles_numeric"] <= st["out_tiles"]
+        assert w.frontier_digest(k) == O.frontier_digest(xs[k], n_src)
+        s_cur = s_next
+
+
+def test_walk_export_matches_tile_oracle(ctx):
+    """X_k exported in the reference layout equals the O4 tile product X_{k-1} * A."""
+    M = mg()
+    scale, seed, n_src = 11, 3, 130
+    n = 1 << scale
+    el, at, s, d = setup_graph(ctx, scale, seed)
+    sources = O.sample_sources(n, n_src, 9)
+    w = M.Walk(ctx, at, n_src, 3)
+    w.run(sources, 3)
+    a = O.edges_to_tiles(n, s, d)
+    at_h = O.edges_to_tiles(n, s, d, transpose=True)
+    ns = a[0]
+    x_prev = w.export(0).download()
+    # X_0 export: one-hot rows
+    dense0 = x_prev.to_dense(n)[:n_src]
+    assert np.array_equal(np.nonzero(dense0)[1], sources) and dense0.sum() == n_src
+    for k in range(1, 4):
+        xk = w.export(k).download()
+        c_off, c_row, c_col, tp = O.tile_symbolic(ns, x_prev.offset_row, x_prev.col_idx, at_h[1], at_h[3])
+        assert np.array_equal(xk.row_idx, c_row) and np.array_equal(xk.col_idx, c_col)
+        assert np.array_equal(xk.offset_row, c_off)
+        c_data = O.tile_numeric(ns, (x_prev.offset_row, x_prev.row_idx, x_prev.col_idx, x_prev.data), at_h[1:],
+                                c_row, c_col)
+        assert np.array_equal(xk.data, c_data)
+        assert w.hop_stats(k)["tile_products"] == tp
+        x_prev = xk
+
+
+def test_walk_prune_same_content(ctx):
+    M = mg()
+    el, at, s, d = setup_graph(ctx, 13, 8)
+    sources = O.sample_sources(1 << 13, 300, 1)
+    w0 = M.Walk(ctx, at, 300, 4)
+    w1 = M.Walk(ctx, at, 300, 4, flags=M.WALK_PRUNE)
+    w0.run(sources, 4)
+    w1.run(sources, 4)
+    for k in range(1, 5):
+        assert np.array_equal(w0.frontier(k), w1.frontier(k))
+        s0, s1 = w0.hop_stats(k), w1.hop_stats(k)
+        assert s1["out_tiles"] <= s0["out_tiles"] and s1["tile_products"] <= s0["tile_products"]
+        assert s1["out_tiles_numeric"] == s0["out_tiles_numeric"]
+
+
+def test_walk_cycle_closed_form_and_empty_graph(ctx):
+    M = mg()
+    n = 8192
+    v = np.arange(n, dtype=np.uint32)
+    cyc = M.EdgeList.upload(ctx, n, v, (v + 1) % n).to_tiled_matrix(transpose=True)
+    sources = np.arange(0, n, 3, dtype=np.uint32)[:2000]
+    w = M.Walk(ctx, cyc, 2000, 5)
+    w.run(sources, 5)
+    for k in range(1, 6):
+        x = w.frontier(k)
+        assert w.hop_stats(k)["popcount"] == 2000
+        idx = (sources.astype(np.int64) + k) % n
+        for i in range(0, 2000, 97):
+            assert (int(x[idx[i], i // 64]) >> (i % 64)) & 1
+    empty = M.EdgeList.upload(ctx, 4096, np.zeros(0, np.uint32), np.zeros(0, np.uint32)).to_tiled_matrix(True)
+    we = M.Walk(ctx, empty, 10, 2)
+    we.run(np.arange(10, dtype=np.uint32), 2)
+    assert not we.frontier(1).any() and we.hop_stats(1)["tile_products"] == 0
+
+
+def test_walk_rerun_smaller_and_errors(ctx):
+    M = mg()
+    n = 1 << 12
+    el, at, s, d = setup_graph(ctx, 12, 11)
+    w = M.Walk(ctx, at, 4096, 3)
+    big = O.sample_sources(n, 4096, 1)
+    w.run(big, 3)
+    small = O.sample_sources(n, 10, 2)
+    w.run(small, 2)  # stale bits of the larger run must not leak
+    xs = O.walk_frontiers(n, s, d, small, 2, W=w.words_per_vertex)
+    assert np.array_equal(w.frontier(2), xs[2])
+    with pytest.raises(M.MGError):
+        w.run(np.array([1, 1], np.uint32), 1)
+    with pytest.raises(M.MGError):
+        w.run(small, 4)
+    with pytest.raises(M.MGError):
+        w.run(np.array([n], np.uint32), 1)
+    with pytest.raises(M.MGError):
+        w.hop_stats(3)
+
+
+@pytest.mark.skipif(os.environ.get("MG_SKIP_FULL") == "1", reason="full-size check disabled by MG_SKIP_FULL=1")
+def test_walk_rmat22_full_scale_digests(ctx):
+    """Configuration: R-MAT scale 22 (seed 2), 4096 sources (seed 2), 4 hops; every hop's
+    frontier digest equals the CPU oracle's (O3 vertex-major frontier)."""
+    M = mg()
+    scale, seed, n_src, hops = 22, 2, 4096, 4
+    n = 1 << scale
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed)
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, seed)
+    assert el.digest() == O.edges_digest(s, d)
+    at = el.to_tiled_matrix(transpose=True)
+    sources = O.sample_sources(n, n_src, seed)
+    w = M.Walk(ctx, at, n_src, hops)
+    w.run(sources, hops)
+    off, isrc = O.in_csr(n, s, d)
+    del s, d
+    x = O.one_hot_frontier(n, sources, 64)
+    for k in range(1, hops + 1):
+        x = O.frontier_step(n, off, isrc, x)
+        assert w.frontier_digest(k) == O.frontier_digest(x, n_src), f"hop {k}"
+        assert w.hop_stats(k)["popcount"] == popcount(x)
