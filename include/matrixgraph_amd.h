@@ -124,9 +124,31 @@ int mg_count(void* stream, size_t tile_size, size_t n_strips, size_t n_nz_tile, 
  * Replaces the host-pinned BitTiledMatrix (bit_tiled_matrix.cuh:17-72) for GPU work.
  * ------------------------------------------------------------------------- */
 
+/* data == NULL uploads the index with all tiles zero (a structural result to be filled). */
 int mg_tiled_matrix_upload(mg_context* ctx, uint32_t n_strips, uint32_t n_nz_tile, uint32_t tile_size,
                            const uint32_t* tile_offset_row, const uint32_t* tile_row_idx,
                            const uint32_t* tile_col_idx, const uint64_t* data, mg_tiled_matrix** out);
+/* ---------------------------------------------------------------------------
+ * On-disk format (host only, no GPU): the block layout of BitTiledMatrixIO
+ * (core/io/bit_tiled_matrix_io.cu:30-136): dir/meta.yaml +
+ * dir/meta_buf/{row_idx,col_idx,tile_offset_row,nz_tile_bm,data}.bin, native
+ * little-endian, no header. dir must end with '/', as in the reference (file names are
+ * appended). Missing, truncated or malformed files give MG_ERR_IO.
+ * ------------------------------------------------------------------------- */
+int mg_io_read_block_metadata(const char* dir, uint32_t* n_strips, uint32_t* n_nz_tile, uint32_t* tile_size);
+/* Arrays sized from mg_io_read_block_metadata; NULL arrays are skipped. nz_tile_bm gets
+ * n_strips*n_strips/64 + 1 words. A block with n_nz_tile 0 yields a zero row pointer. */
+int mg_io_read_block(const char* dir, uint32_t* tile_offset_row, uint32_t* tile_row_idx, uint32_t* tile_col_idx,
+                     uint64_t* data, uint64_t* nz_tile_bm);
+/* Writes meta.yaml and the five .bin files (nz_tile_bm derived from the index). Empty
+ * blocks are written too (the reference dereferences a NULL bitmap there). */
+int mg_io_write_block(const char* dir, uint32_t n_strips, uint32_t n_nz_tile, uint32_t tile_size,
+                      const uint32_t* tile_offset_row, const uint32_t* tile_row_idx, const uint32_t* tile_col_idx,
+                      const uint64_t* data);
+/* Grid root meta.yaml (GridGraphMetadata, core/io/grid_bit_tiled_matrix_io.cu:22-67);
+ * blocks live in dir/block{x*n_chunks+y}/. */
+int mg_io_read_grid_metadata(const char* dir, uint32_t* n_chunks, uint32_t* n_vertices, uint32_t* n_edges);
+int mg_io_write_grid_metadata(const char* dir, uint32_t n_chunks, uint32_t n_vertices, uint32_t n_edges);
 int mg_tiled_matrix_metadata(const mg_tiled_matrix* m, uint32_t* n_strips, uint32_t* n_nz_tile,
                              uint32_t* tile_size);
 /* Copies any non-NULL array back to the host. nz_tile_bm receives

@@ -369,3 +369,50 @@ class Walk:
             self.close()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------------------
+# On-disk format (host only): core/io/bit_tiled_matrix_io.cu:30-136
+# ----------------------------------------------------------------------------
+def _dir(path) -> bytes:
+    p = str(path)
+    if not p.endswith("/"):
+        p += "/"
+    return p.encode()
+
+
+def write_block(path, m: HostTiledMatrix) -> None:
+    off = np.ascontiguousarray(m.offset_row, np.uint32)
+    row = np.ascontiguousarray(m.row_idx, np.uint32)
+    col = np.ascontiguousarray(m.col_idx, np.uint32)
+    data = np.ascontiguousarray(m.data, np.uint64)
+    _check(lib().mg_io_write_block(_dir(path), m.n_strips, m.n_nz_tile, m.tile_size, _ptr(off), _ptr(row), _ptr(col),
+                                   _ptr(data)), "mg_io_write_block")
+
+
+def read_block(path) -> HostTiledMatrix:
+    ns, nt, ts = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().mg_io_read_block_metadata(_dir(path), C.byref(ns), C.byref(nt), C.byref(ts)),
+           "mg_io_read_block_metadata")
+    off = np.empty(ns.value + 1, np.uint32)
+    row = np.empty(nt.value, np.uint32)
+    col = np.empty(nt.value, np.uint32)
+    data = np.empty((nt.value, TILE), np.uint64)
+    _check(lib().mg_io_read_block(_dir(path), _ptr(off), _ptr(row), _ptr(col), _ptr(data), None), "mg_io_read_block")
+    return HostTiledMatrix(ns.value, off, row, col, data, ts.value)
+
+
+def write_grid(path, blocks, n_vertices: int, n_edges: int) -> None:
+    """blocks: list of n_chunks^2 HostTiledMatrix, gid = x*n_chunks + y."""
+    n_chunks = int(round(len(blocks) ** 0.5))
+    assert n_chunks * n_chunks == len(blocks)
+    for gid, b in enumerate(blocks):
+        write_block(f"{str(path).rstrip('/')}/block{gid}/", b)
+    _check(lib().mg_io_write_grid_metadata(_dir(path), n_chunks, n_vertices, n_edges), "mg_io_write_grid_metadata")
+
+
+def read_grid(path):
+    nc, nv, ne = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().mg_io_read_grid_metadata(_dir(path), C.byref(nc), C.byref(nv), C.byref(ne)), "mg_io_read_grid_metadata")
+    blocks = [read_block(f"{str(path).rstrip('/')}/block{g}/") for g in range(nc.value * nc.value)]
+    return blocks, {"n_chunks": nc.value, "n_vertices": nv.value, "n_edges": ne.value}

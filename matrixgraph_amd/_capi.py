@@ -63,6 +63,11 @@ SIGNATURES = {
     "mg_tiled_matrix_digest": (i32, [vp, vp, P_u64]),
     "mg_tiled_matrix_popcount": (i32, [vp, vp, P_u64]),
     "mg_tiled_matrix_destroy": (i32, [vp]),
+    "mg_io_read_block_metadata": (i32, [C.c_char_p, P_u32, P_u32, P_u32]),
+    "mg_io_read_block": (i32, [C.c_char_p, vp, vp, vp, vp, vp]),
+    "mg_io_write_block": (i32, [C.c_char_p, u32, u32, u32, vp, vp, vp, vp]),
+    "mg_io_read_grid_metadata": (i32, [C.c_char_p, P_u32, P_u32, P_u32]),
+    "mg_io_write_grid_metadata": (i32, [C.c_char_p, u32, u32, u32]),
     "mg_multiply_symbolic": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(ProductStats)]),
     "mg_multiply_numeric": (i32, [vp, vp, vp, vp]),
     # edges

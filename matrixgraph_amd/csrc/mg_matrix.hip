@@ -96,7 +96,6 @@ extern "C" int mg_tiled_matrix_upload(mg_context* ctx, uint32_t n_strips, uint32
       fail(MG_ERR_UNSUPPORTED, "mg_tiled_matrix_upload: tile_size " + std::to_string(tile_size) +
                                    " unsupported (only 64)");
     MG_REQUIRE(n_strips > 0, "mg_tiled_matrix_upload: n_strips == 0");
-    MG_REQUIRE(n_nz_tile == 0 || data != nullptr, "mg_tiled_matrix_upload: data is NULL");
     validate_host_index(n_strips, n_nz_tile, tile_offset_row, tile_row_idx, tile_col_idx);
     DeviceGuard g(ctx->device);
     auto m = std::make_unique<mg_tiled_matrix>();
@@ -109,8 +108,11 @@ extern "C" int mg_tiled_matrix_upload(mg_context* ctx, uint32_t n_strips, uint32
                             hipMemcpyHostToDevice, ctx->stream));
       MG_HIP(hipMemcpyAsync(m->col_idx.get(), tile_col_idx, sizeof(uint32_t) * n_nz_tile,
                             hipMemcpyHostToDevice, ctx->stream));
-      MG_HIP(hipMemcpyAsync(m->data.get(), data, sizeof(uint64_t) * kTile * n_nz_tile,
-                            hipMemcpyHostToDevice, ctx->stream));
+      if (data)
+        MG_HIP(hipMemcpyAsync(m->data.get(), data, sizeof(uint64_t) * kTile * n_nz_tile,
+                              hipMemcpyHostToDevice, ctx->stream));
+      else
+        MG_HIP(hipMemsetAsync(m->data.get(), 0, sizeof(uint64_t) * kTile * n_nz_tile, ctx->stream));
     }
     MG_HIP(hipStreamSynchronize(ctx->stream));
     *out = m.release();

@@ -29,7 +29,8 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 constexpr uint32_t kChunkTiles = 1024;   // scheduler target tiles per chunk
 constexpr uint32_t kChunkStrips = 32;    // at most this many strips per chunk
-constexpr int kQueue = 8;                // gathers issued together per wave
+constexpr int kLoadsInFlight = 8;        // frontier gathers per wave flush
+constexpr int kTileBatch = 8;            // A^T tile loads in flight per wave
 
 enum Counter { kTP = 0, kLeft, kRightUsed, kOut, kOutNumeric, kPop, kEdges, kNumCounters };
 
@@ -101,35 +102,73 @@ __global__ void __launch_bounds__(kBlock) walk_symbolic_kernel(uint32_t n_strips
 }
 
 // --- numeric ----------------------------------------------------------------
-// Gather queue of one wave: lane j holds job j (source vertex, target-row mask).
+// Gather queue of one wave: lane j holds job j (source vertex u and the mask of the
+// strip's target rows that have u as in-neighbour); len is wave-uniform.
 struct GatherQueue {
-  uint32_t u;     // per-lane slot
-  uint64_t rows;  // per-lane slot
-  uint32_t len;   // wave-uniform
+  uint32_t u;
+  uint64_t rows;
+  uint32_t len;
+};
+
+// A job gathers the W words of one source vertex: 64/W jobs share one 64-lane load,
+// and a flush keeps at most kLoadsInFlight loads in flight.
+template <int W>
+struct QueueShape {
+  static constexpr int kJobsPerLoad = 64 / W;
+  static constexpr int kCapacity = (kLoadsInFlight * kJobsPerLoad < 64) ? kLoadsInFlight * kJobsPerLoad : 64;
+  static constexpr int kLoads = (kCapacity + kJobsPerLoad - 1) / kJobsPerLoad;
 };
 
 template <int W>
 __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ x_in,
                                    unsigned long long* __restrict__ acc, uint32_t lane) {
-  static_assert(W == 64, "flush_queue: one source word per lane");
-  uint64_t v[kQueue];
+  using S = QueueShape<W>;
+  uint64_t v[S::kLoads];
+  if constexpr (W == 64) {
+    // One job per load: its source vertex is wave-uniform.
 #pragma unroll
-  for (int j = 0; j < kQueue; ++j) {
-    v[j] = 0;
-    if (static_cast<uint32_t>(j) < q.len) {
-      const uint32_t u = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(q.u), j));
-      v[j] = x_in[static_cast<uint64_t>(u) * W + lane];
+    for (int j = 0; j < S::kLoads; ++j) {
+      v[j] = 0;
+      if (static_cast<uint32_t>(j) < q.len) {
+        const uint32_t u = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(q.u), j));
+        v[j] = x_in[static_cast<uint64_t>(u) * W + lane];
+      }
     }
-  }
 #pragma unroll
-  for (int j = 0; j < kQueue; ++j) {
-    if (static_cast<uint32_t>(j) < q.len) {
-      uint64_t rows = read_lane(q.rows, j);
+    for (int j = 0; j < S::kLoads; ++j) {
+      if (static_cast<uint32_t>(j) < q.len) {
+        uint64_t rows = read_lane(q.rows, j);
+        const uint64_t val = v[j];
+        while (rows) {
+          const int r = __ffsll(static_cast<unsigned long long>(rows)) - 1;
+          rows &= rows - 1;
+          if (val) atomicOr(acc + r * W + lane, static_cast<unsigned long long>(val));
+        }
+      }
+    }
+  } else {
+    // Lane = (job slot, word): several jobs per load, per-lane source index.
+    const uint32_t w = lane % W;
+    const uint32_t slot = lane / W;
+#pragma unroll
+    for (int j = 0; j < S::kLoads; ++j) {
+      const uint32_t job = j * S::kJobsPerLoad + slot;
+      const uint32_t u = static_cast<uint32_t>(__shfl(static_cast<int>(q.u), static_cast<int>(job & 63), kWave));
+      v[j] = (job < q.len) ? x_in[static_cast<uint64_t>(u) * W + w] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < S::kLoads; ++j) {
+      const uint32_t job = j * S::kJobsPerLoad + slot;
+      const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q.rows)),
+                                                       static_cast<int>(job & 63), kWave));
+      const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(q.rows >> 32)),
+                                                       static_cast<int>(job & 63), kWave));
+      uint64_t rows = (job < q.len && v[j] != 0) ? ((static_cast<uint64_t>(hi) << 32) | lo) : 0;
       const uint64_t val = v[j];
       while (rows) {
         const int r = __ffsll(static_cast<unsigned long long>(rows)) - 1;
         rows &= rows - 1;
-        if (val) atomicOr(acc + r * W + lane, static_cast<unsigned long long>(val));
+        atomicOr(acc + r * W + w, static_cast<unsigned long long>(val));
       }
     }
   }
@@ -143,7 +182,7 @@ __device__ inline void push_job(GatherQueue& q, uint32_t u, uint64_t rows, uint3
     q.u = u;
     q.rows = rows;
   }
-  if (++q.len == kQueue) flush_queue<W>(q, x_in, acc, lane);
+  if (++q.len == QueueShape<W>::kCapacity) flush_queue<W>(q, x_in, acc, lane);
 }
 
 // Extract the gather jobs of one A^T tile: for each source column t' with a set bit
@@ -176,6 +215,7 @@ __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueu
   }
 }
 
+// One workgroup per scheduler chunk; per strip a 64 x W-word LDS accumulator.
 template <int W>
 __global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
     const Chunk* __restrict__ chunks, const uint32_t* __restrict__ off, const uint32_t* __restrict__ col,
@@ -200,30 +240,43 @@ __global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
     __syncthreads();
 
     GatherQueue q{0, 0, 0};
-    // Tiles p = seg0 + wave + kWaves*i, four per step so their loads overlap.
-    uint32_t p = read_first_lane(seg0 + wave);
-    for (; p + 3 * kWaves < seg1; p += 4 * kWaves) {
-      uint32_t t[4];
-      uint64_t b[4], vm[4];
+    // Wave `wave` scans tiles p = seg0 + wave + kWaves*(64*g + lane): one coalesced load of
+    // 64 tile columns and one gather of their source-strip masks per 64 tiles; the active
+    // tiles then go in groups of kTileBatch whose 512-B loads are in flight together.
+    for (uint32_t g0 = seg0 + wave; g0 < seg1; g0 += kWaves * 64) {
+      const uint32_t p = g0 + kWaves * lane;
+      const bool valid = p < seg1;
+      const uint32_t t_l = valid ? col[p] : 0;
+      const uint64_t vm_l = valid ? v_in[t_l] : 0;
+      uint64_t act = ballot(vm_l != 0);
+      while (act) {
+        int ks[kTileBatch];
+        int cnt = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) t[j] = col[p + j * kWaves];
+        for (int j = 0; j < kTileBatch; ++j) {
+          ks[j] = 0;
+          if (act) {
+            ks[j] = __ffsll(static_cast<unsigned long long>(act)) - 1;
+            act &= act - 1;
+            cnt = j + 1;
+          }
+        }
+        // A^T tiles are streamed once per hop: non-temporal loads keep them from
+        // evicting the frontier words that the gathers re-read.
+        uint64_t b[kTileBatch];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) vm[j] = v_in[t[j]];
-      // A^T tiles are streamed once per hop: non-temporal loads keep them from
-      // evicting the frontier words that the gathers re-read.
+        for (int j = 0; j < kTileBatch; ++j)
+          b[j] = (j < cnt) ? __builtin_nontemporal_load(
+                                 data + static_cast<uint64_t>(g0 + kWaves * static_cast<uint32_t>(ks[j])) * kTile + lane)
+                           : 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        b[j] = vm[j] ? __builtin_nontemporal_load(data + static_cast<uint64_t>(p + j * kWaves) * kTile + lane) : 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (vm[j]) tile_jobs<W>(t[j], b[j], vm[j], q, lane, x_in, acc, edges);
-    }
-    for (; p < seg1; p += kWaves) {
-      const uint32_t t = col[p];
-      const uint64_t vm = v_in[t];
-      if (vm) {
-        const uint64_t b = __builtin_nontemporal_load(data + static_cast<uint64_t>(p) * kTile + lane);
-        tile_jobs<W>(t, b, vm, q, lane, x_in, acc, edges);
+        for (int j = 0; j < kTileBatch; ++j) {
+          if (j < cnt) {
+            const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), ks[j]));
+            const uint64_t vm = read_lane(vm_l, ks[j]);
+            tile_jobs<W>(t, b[j], vm, q, lane, x_in, acc, edges);
+          }
+        }
       }
     }
     if (q.len) flush_queue<W>(q, x_in, acc, lane);
@@ -442,24 +495,37 @@ struct mg_walk {
 
 namespace {
 
-void launch_numeric(mg_context* ctx, mg_walk* w, uint32_t k, hipEvent_t ev_begin, hipEvent_t ev_end) {
-  hipStream_t st = ctx->stream;
+template <int W>
+void launch_numeric_w(hipStream_t st, mg_walk* w, uint32_t k, hipEvent_t ev_begin, hipEvent_t ev_end) {
   const mg_tiled_matrix* at = w->at;
   unsigned long long* ctr = w->counters.get() + (k + 1) * kNumCounters;
   if (w->n_split) {
-    zero_strips_kernel<64><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get());
+    zero_strips_kernel<W><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get());
     MG_LAUNCH_CHECK();
   }
   if (ev_begin) MG_HIP(hipEventRecord(ev_begin, st));
-  walk_numeric_kernel<64><<<w->n_chunks, kBlock, 0, st>>>(
+  walk_numeric_kernel<W><<<w->n_chunks, kBlock, 0, st>>>(
       w->chunks.get(), at->offset_row.get(), at->col_idx.get(), at->data.get(), w->x[k].get(), w->vmask[k].get(),
       w->x[k + 1].get(), w->nmask[k + 1].get(), w->vmask[k + 1].get(), ctr);
   MG_LAUNCH_CHECK();
   if (ev_end) MG_HIP(hipEventRecord(ev_end, st));
   if (w->n_split) {
-    finalize_strips_kernel<64><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get(),
-                                                             w->nmask[k + 1].get(), w->vmask[k + 1].get(), ctr);
+    finalize_strips_kernel<W><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get(),
+                                                            w->nmask[k + 1].get(), w->vmask[k + 1].get(), ctr);
     MG_LAUNCH_CHECK();
+  }
+}
+
+void launch_numeric(mg_context* ctx, mg_walk* w, uint32_t k, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  switch (w->W) {
+    case 1: launch_numeric_w<1>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 2: launch_numeric_w<2>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 4: launch_numeric_w<4>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 8: launch_numeric_w<8>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 16: launch_numeric_w<16>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 32: launch_numeric_w<32>(ctx->stream, w, k, ev_begin, ev_end); break;
+    case 64: launch_numeric_w<64>(ctx->stream, w, k, ev_begin, ev_end); break;
+    default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex " + std::to_string(w->W));
   }
 }
 
@@ -478,7 +544,9 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     w->device = ctx->device;
     w->at = at;
     w->n_strips = at->n_strips;
-    w->W = 64;  // one source word per lane (4096 sources per frontier)
+    // Words per vertex: the smallest power of two covering max_sources (64 sources per word).
+    w->W = 1;
+    while (w->W * 64 < max_sources) w->W <<= 1;
     w->max_sources = max_sources;
     w->max_hops = max_hops;
     w->flags = flags;
