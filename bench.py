@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=64, help="CPU baseline samples every n-th strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (5 times): the known-size reference for "
+                         "FETCH_SIZE / WRITE_SIZE counter scaling")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -88,6 +91,14 @@ def main():
     import matrixgraph_amd as M
 
     ctx = M.Context(0)
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src, dst = M.DeviceBuffer(ctx, nbytes), M.DeviceBuffer(ctx, nbytes)
+        for _ in range(5):
+            M._check(M.lib().mg_buffer_copy(ctx.h, dst.ptr, src.ptr, nbytes), "mg_buffer_copy")
+        ctx.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+        return
     t0 = time.perf_counter()
     el = M.EdgeList.rmat(ctx, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
     n_vertices, n_edges = el.info()

@@ -67,6 +67,18 @@ int mg_context_destroy(mg_context* ctx);
 int mg_context_set_stream(mg_context* ctx, void* stream);
 int mg_context_synchronize(mg_context* ctx);
 int mg_context_device(const mg_context* ctx, int* device, int* n_compute_units);
+/* The context's hipStream_t (for layer-1 calls ordered with the context's work). */
+int mg_context_stream(const mg_context* ctx, void** stream);
+
+/* Device buffers for C callers of layer 1, with DeviceOwnedBuffer semantics
+ * (core/data_structures/device_buffer.cuh:15-151): mg_buffer_alloc zero-fills (Init(size)),
+ * upload/download copy host <-> device on the context's stream and synchronise. */
+int mg_buffer_alloc(mg_context* ctx, size_t bytes, void** dptr);
+int mg_buffer_free(mg_context* ctx, void* dptr);
+int mg_buffer_upload(mg_context* ctx, void* dptr, const void* host, size_t bytes);
+int mg_buffer_download(mg_context* ctx, void* host, const void* dptr, size_t bytes);
+/* Device-to-device copy on the context's stream (asynchronous). */
+int mg_buffer_copy(mg_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---------------------------------------------------------------------------
  * Layer 1 — the reference's kernel-wrapper entry points, one for one.

@@ -142,6 +142,49 @@ class Context:
         return n.value
 
 
+    @property
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().mg_context_stream(self.h, C.byref(s)), "mg_context_stream")
+        return s.value or 0
+
+
+class DeviceBuffer:
+    """Zero-filled device allocation (DeviceOwnedBuffer semantics) for layer-1 calls."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        _check(lib().mg_buffer_alloc(ctx.h, nbytes, C.byref(p)), "mg_buffer_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx: Context, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        b = cls(ctx, a.nbytes)
+        _check(lib().mg_buffer_upload(ctx.h, C.c_void_p(b.ptr), _ptr(a), a.nbytes), "mg_buffer_upload")
+        return b
+
+    def to_array(self, dtype, count=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dt)
+        _check(lib().mg_buffer_download(self.ctx.h, _ptr(out), C.c_void_p(self.ptr), out.nbytes), "mg_buffer_download")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().mg_buffer_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class TiledMatrix:
     """Device-resident bit-tiled matrix (one block)."""
 

@@ -49,6 +49,12 @@ SIGNATURES = {
     "mg_context_set_stream": (i32, [vp, vp]),
     "mg_context_synchronize": (i32, [vp]),
     "mg_context_device": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
+    "mg_context_stream": (i32, [vp, C.POINTER(vp)]),
+    "mg_buffer_alloc": (i32, [vp, sz, C.POINTER(vp)]),
+    "mg_buffer_free": (i32, [vp, vp]),
+    "mg_buffer_upload": (i32, [vp, vp, vp, sz]),
+    "mg_buffer_download": (i32, [vp, vp, vp, sz]),
+    "mg_buffer_copy": (i32, [vp, vp, vp, sz]),
     # layer 1 (device pointers)
     "mg_matrix_bit_and": (i32, [vp, vp, vp, vp, u32, u32, u32]),
     "mg_matrix_bit_count": (i32, [vp, vp, vp, u64]),

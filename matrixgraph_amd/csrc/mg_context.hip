@@ -112,3 +112,64 @@ extern "C" int mg_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t s
     }
   });
 }
+
+extern "C" int mg_context_stream(const mg_context* ctx, void** stream) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && stream, "mg_context_stream: NULL argument");
+    *stream = ctx->stream;
+  });
+}
+
+extern "C" int mg_buffer_alloc(mg_context* ctx, size_t bytes, void** dptr) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && dptr, "mg_buffer_alloc: NULL argument");
+    *dptr = nullptr;
+    DeviceGuard g(ctx->device);
+    void* p = nullptr;
+    MG_HIP(hipMalloc(&p, bytes ? bytes : 1));
+    const hipError_t e = hipMemsetAsync(p, 0, bytes ? bytes : 1, ctx->stream);
+    if (e == hipSuccess) {
+      MG_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+      (void)hipFree(p);
+      MG_HIP(e);
+    }
+    *dptr = p;
+  });
+}
+
+extern "C" int mg_buffer_free(mg_context* ctx, void* dptr) {
+  return guarded([&] {
+    MG_REQUIRE(ctx != nullptr, "mg_buffer_free: NULL ctx");
+    if (!dptr) return;
+    DeviceGuard g(ctx->device);
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+    MG_HIP(hipFree(dptr));
+  });
+}
+
+extern "C" int mg_buffer_upload(mg_context* ctx, void* dptr, const void* host, size_t bytes) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && dptr && (host || bytes == 0), "mg_buffer_upload: NULL argument");
+    DeviceGuard g(ctx->device);
+    if (bytes) MG_HIP(hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int mg_buffer_download(mg_context* ctx, void* host, const void* dptr, size_t bytes) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && dptr && (host || bytes == 0), "mg_buffer_download: NULL argument");
+    DeviceGuard g(ctx->device);
+    if (bytes) MG_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int mg_buffer_copy(mg_context* ctx, void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && dst && src, "mg_buffer_copy: NULL argument");
+    DeviceGuard g(ctx->device);
+    if (bytes) MG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}

@@ -91,28 +91,20 @@ def test_empty_operand(ctx):
 # ---------------------------------------------------------------------------
 # Layer 1: reference-wrapper entry points on raw device buffers
 # ---------------------------------------------------------------------------
+_CTX = None
+
+
 def _dev(arr):
-    import torch
-
-    a = np.ascontiguousarray(arr)
-    if a.dtype == np.uint64:
-        return torch.from_numpy(a.view(np.int64).copy()).cuda()
-    if a.dtype == np.uint32:
-        return torch.from_numpy(a.view(np.int32).copy()).cuda()
-    raise TypeError(a.dtype)
+    return mg().DeviceBuffer.from_array(_CTX, arr)
 
 
-def _host(t, dtype):
-    import torch
-
-    torch.cuda.synchronize()
-    return t.cpu().numpy().view(dtype)
+def _host(buf, dtype):
+    _CTX.synchronize()
+    return buf.to_array(dtype)
 
 
 def _stream():
-    import torch
-
-    return torch.cuda.current_stream().cuda_stream
+    return _CTX.stream
 
 
 def _bits(words, nbits):
@@ -127,6 +119,8 @@ def _pack(bits):
 
 
 def test_layer1_bit_and_and_bit_count(ctx):
+    global _CTX
+    _CTX = ctx
     lib = mg().lib()
     rng = np.random.default_rng(7)
     m, k, n = 70, 100, 90
@@ -137,16 +131,19 @@ def test_layer1_bit_and_and_bit_count(ctx):
     prior = np.zeros(m * n, np.uint8)
     prior[rng.integers(0, m * n, 50)] = 1
     dc = _dev(_pack(prior))
-    assert lib.mg_matrix_bit_and(_stream(), da.data_ptr(), db.data_ptr(), dc.data_ptr(), m, k, n) == 0
+    assert lib.mg_matrix_bit_and(_stream(), da.ptr, db.ptr, dc.ptr, m, k, n) == 0
     got = _bits(_host(dc, np.uint64), m * n).reshape(m, n)
     assert np.array_equal(got, expect | prior.reshape(m, n))  # OR-accumulates like atomicOr
     words = rng.integers(0, 2**63, 10, dtype=np.int64).astype(np.uint64)
     cnt = _dev(np.array([5], np.uint64))
-    assert lib.mg_matrix_bit_count(_stream(), _dev(words).data_ptr(), cnt.data_ptr(), 600) == 0
+    dw = _dev(words)  # keep the buffer alive until the launch has completed
+    assert lib.mg_matrix_bit_count(_stream(), dw.ptr, cnt.ptr, 600) == 0
     assert int(_host(cnt, np.uint64)[0]) == 5 + int(_bits(words, 600).sum())
 
 
 def test_layer1_init_metadata(ctx):
+    global _CTX
+    _CTX = ctx
     lib = mg().lib()
     rng = np.random.default_rng(8)
     ns = 100
@@ -157,13 +154,15 @@ def test_layer1_init_metadata(ctx):
     dr = _dev(np.zeros(len(xs), np.uint32))
     dcol = _dev(np.zeros(len(xs), np.uint32))
     assert lib.mg_init_bit_tiled_matrix_metadata_by_layout_matrix(
-        _stream(), dl.data_ptr(), ns, cnt.data_ptr(), dr.data_ptr(), dcol.data_ptr()) == 0
+        _stream(), dl.ptr, ns, cnt.ptr, dr.ptr, dcol.ptr) == 0
     assert np.array_equal(_host(cnt, np.uint32), lay.sum(axis=1).astype(np.uint32))
     assert np.array_equal(_host(dr, np.uint32), xs.astype(np.uint32))
     assert np.array_equal(_host(dcol, np.uint32), ys.astype(np.uint32))
 
 
 def test_layer1_fill_tiles_and_count(ctx):
+    global _CTX
+    _CTX = ctx
     lib = mg().lib()
     s, d = O.rmat_edges(12, 16, 0.57, 0.19, 0.19, 9)
     a = O.edges_to_tiles(4096, s, d)
@@ -176,14 +175,14 @@ def test_layer1_fill_tiles_and_count(ctx):
     ns = a[0]
     bufs = [_dev(v) for v in (a[1], at[1], c_off, a[2], at[2], c_row, a[3], at[3], c_col, a[4], at[4], prior)]
     (a_off, b_off, co, a_row, b_row, cr, a_col, b_col, cc, a_dat, b_dat, dc) = bufs
-    rc = lib.mg_fill_tiles(_stream(), 64, ns, len(a[2]), len(at[2]), len(c_row), None, a_off.data_ptr(),
-                           b_off.data_ptr(), co.data_ptr(), a_row.data_ptr(), b_row.data_ptr(), cr.data_ptr(),
-                           a_col.data_ptr(), b_col.data_ptr(), cc.data_ptr(), a_dat.data_ptr(), b_dat.data_ptr(),
-                           dc.data_ptr())
+    rc = lib.mg_fill_tiles(_stream(), 64, ns, len(a[2]), len(at[2]), len(c_row), None, a_off.ptr,
+                           b_off.ptr, co.ptr, a_row.ptr, b_row.ptr, cr.ptr,
+                           a_col.ptr, b_col.ptr, cc.ptr, a_dat.ptr, b_dat.ptr,
+                           dc.ptr)
     assert rc == 0
     got = _host(dc, np.uint64).reshape(-1, 64)
     assert np.array_equal(got, c_data | prior)
     cnt = _dev(np.zeros(1, np.uint64))
-    assert lib.mg_count(_stream(), 64, ns, len(c_row), dc.data_ptr(), cnt.data_ptr()) == 0
+    assert lib.mg_count(_stream(), 64, ns, len(c_row), dc.ptr, cnt.ptr) == 0
     assert int(_host(cnt, np.uint64)[0]) == int(np.unpackbits((c_data | prior).view(np.uint8)).sum())
     assert lib.mg_fill_tiles(_stream(), 32, ns, 0, 0, 1, None, *([None] * 12)) == 4
