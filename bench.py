@@ -66,6 +66,87 @@ def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
     }
 
 
+def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int):
+    """Uneven query batch: 512 queries over 8 graphs (R-MAT scales 16..20, three ER
+    n=2^18 p=2^-13), sizes log-uniform in [1, 4096], hops uniform in [1, 6].
+    Plain = every query run alone (its own walk, words per vertex sized to the query);
+    batched = one captured HIP graph over panels, launch options from the autotuner.
+    Per-query X_hops digests must agree between the two forms."""
+    from matrixgraph_amd import autotune
+    from matrixgraph_amd import workloads as WL
+
+    specs = WL.batch_graph_specs(seed0)
+    t0 = time.perf_counter()
+    built = [WL.build_graph(ctx, s) for s in specs]
+    graphs = [g for g, _ in built]
+    queries = WL.batch_queries(specs, n_queries, query_seed)
+    log(f"batch: {len(specs)} graphs built in {time.perf_counter() - t0:.1f}s, {len(queries)} queries")
+
+    walks = {}
+
+    def walk_for(q):
+        W = 1
+        while W * 64 < len(q.sources):
+            W *= 2
+        key = (q.graph, W)
+        if key not in walks:
+            walks[key] = M.Walk(ctx, graphs[q.graph], 64 * W, 6)
+        return walks[key]
+
+    def run_plain():
+        for q in queries:
+            walk_for(q).run(q.sources, q.hops)
+
+    run_plain()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run_plain()
+    ctx.synchronize()
+    plain_s = (time.perf_counter() - t0) / reps
+    plain_digests = []
+    for q in queries:  # last run of each (graph, W) walk is not the query's: rerun singly for digests
+        w = walk_for(q)
+        w.run(q.sources, q.hops)
+        plain_digests.append(w.frontier_digest(q.hops))
+    for w in walks.values():
+        w.close()
+    walks.clear()
+    log(f"batch plain: {plain_s * 1e3:.1f} ms per batch ({n_queries / plain_s:.1f} queries/s)")
+
+    best, table = autotune.tune(ctx, graphs, queries, reps=2, log=log)
+    b = M.WalkBatch(ctx, graphs, queries, packing=best["packing"], chunk_tiles=best["chunk_tiles"], use_graph=True)
+    b.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    ctx.synchronize()
+    batched_s = (time.perf_counter() - t0) / reps
+    exact = all(b.query_digest(i) == plain_digests[i] for i in range(len(queries)))
+    tot = b.totals()
+    n_panels, max_hops, n_chunks = b.info()
+    b.close()
+    log(f"batch batched: {batched_s * 1e3:.1f} ms per batch ({n_queries / batched_s:.1f} queries/s), "
+        f"{n_panels} panels, bit-exact={exact}")
+    if not exact:
+        raise SystemExit("batch: batched digests differ from the per-query runs")
+    return {
+        "workload": f"{n_queries} walk queries over {len(specs)} graphs (" + "; ".join(s.describe() for s in specs)
+                    + f"), query seed {query_seed}, sizes log-uniform [1,4096], hops uniform [1,6]",
+        "plain_queries_per_s": round(n_queries / plain_s, 2),
+        "batched_queries_per_s": round(n_queries / batched_s, 2),
+        "batched_vs_plain": round(plain_s / batched_s, 3),
+        "plain_ms": round(plain_s * 1e3, 3),
+        "batched_ms": round(batched_s * 1e3, 3),
+        "batched_tile_products_per_s": round(tot["tile_products"] / batched_s, 1),
+        "panels": n_panels,
+        "chunks": n_chunks,
+        "autotune": {"best": best, "table": table},
+        "bit_exact": exact,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +159,9 @@ def main():
     ap.add_argument("--hops", type=int, default=4)
     ap.add_argument("--cpu-stride", type=int, default=64, help="CPU baseline samples every n-th strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
+    ap.add_argument("--batch-queries", type=int, default=512)
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (5 times): the known-size reference for "
@@ -199,6 +283,9 @@ def main():
         threads = O.max_threads()
         result["cpu_baseline"] = cpu_baseline(M, O, ctx, w, at, n_edges, args.hops, args.sources, args.cpu_stride,
                                               threads)
+    if not args.no_batch:
+        w.close()
+        result["batch"] = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)
     print(json.dumps(result), flush=True)
 
 

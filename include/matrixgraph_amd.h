@@ -290,6 +290,77 @@ int mg_walk_frontier_digest(mg_context* ctx, const mg_walk* w, uint32_t hop, uin
 int mg_walk_export(mg_context* ctx, const mg_walk* w, uint32_t hop, mg_tiled_matrix** out);
 int mg_walk_destroy(mg_walk* w);
 
+/* ---------------------------------------------------------------------------
+ * Batched walks: many queries of uneven source-set size and hop count over several
+ * graphs in one launch sequence, captured once as a HIP graph and replayed by
+ * mg_walk_batch_run. Queries are packed at 64-source word boundaries into 4096-source
+ * panels per graph; every hop level of every panel is one set of kernel launches, and
+ * a query's words stop propagating after its own hop count, so per-query results are
+ * bit-identical to running the query alone with mg_walk_run.
+ * ------------------------------------------------------------------------- */
+typedef struct mg_walk_batch mg_walk_batch;
+typedef struct mg_walk_query {
+  uint32_t graph;          /* index into the graphs array of mg_walk_batch_create */
+  uint32_t n_sources;      /* 1..4096 distinct source vertices */
+  const uint32_t* sources; /* read during mg_walk_batch_create only */
+  uint32_t hops;           /* 1..64 */
+} mg_walk_query;
+enum {
+  MG_PACK_BY_HOPS = 0,      /* per graph, descending hop count, first fit into panels */
+  MG_PACK_ONE_PER_PANEL = 1 /* every query in its own panel (no sharing) */
+};
+typedef struct mg_batch_options {
+  uint32_t packing;     /* MG_PACK_* */
+  uint32_t chunk_tiles; /* scheduler chunk size in A^T tiles (0 = default 1024) */
+  uint32_t use_graph;   /* non-zero: capture once, replay as a HIP graph */
+  uint32_t reserved;
+} mg_batch_options;
+
+/* graphs_at[g] is graph g supplied transposed (A^T); they must outlive the batch.
+ * opt may be NULL (packing by hops, default chunks, HIP graph). */
+int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* const* graphs_at, uint32_t n_graphs,
+                         const mg_walk_query* queries, uint32_t n_queries, const mg_batch_options* opt,
+                         mg_walk_batch** out);
+/* Runs every query (asynchronous; the first call captures the graph). */
+int mg_walk_batch_run(mg_context* ctx, mg_walk_batch* b);
+int mg_walk_batch_info(const mg_walk_batch* b, uint32_t* n_panels, uint32_t* max_hops, uint64_t* n_chunks);
+/* Counters of the panel holding query q at hop (its tile products cover all queries of
+ * that panel). */
+int mg_walk_batch_query_stats(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint32_t hop, mg_hop_stats* out);
+/* Sums over all panels and hops (structural tile products, B_alg, visited A^T bits). */
+int mg_walk_batch_totals(mg_context* ctx, const mg_walk_batch* b, uint64_t* tile_products, uint64_t* alg_bytes,
+                         uint64_t* edges_visited);
+/* Query q's X_hops: digest equal to mg_walk_frontier_digest of the same query run alone;
+ * download gives n_vertices x ceil(n_sources/64) words (vertex-major); export gives the
+ * reference tile layout as mg_walk_export does. */
+int mg_walk_batch_query_digest(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint64_t* digest);
+int mg_walk_batch_query_download(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint64_t* out);
+int mg_walk_batch_query_export(mg_context* ctx, const mg_walk_batch* b, uint32_t q, mg_tiled_matrix** out);
+int mg_walk_batch_destroy(mg_walk_batch* b);
+
+/* ---------------------------------------------------------------------------
+ * Learned cost model: MLP n_in -> 64 -> 64 -> n_out (ReLU), BF16 weights rounded to
+ * nearest-even from FP32 masters, FP32 accumulation, hand-written forward / backward /
+ * Adam kernels with deterministic gradient reduction. Supported shapes: n_in in
+ * {4, 8, 16}, n_out in {1, 2}. Sample arrays (x: n x n_in, t and y: n x n_out, FP32,
+ * row-major) are DEVICE pointers. Parameter layout: W1[64][n_in], b1[64], W2[64][64],
+ * b2[64], W3[n_out][64], b3[n_out]. No counterpart in the reference (new subsystem).
+ * ------------------------------------------------------------------------- */
+typedef struct mg_mlp mg_mlp;
+int mg_mlp_create(mg_context* ctx, uint32_t n_in, uint32_t n_out, uint64_t seed, mg_mlp** out);
+int mg_mlp_num_params(const mg_mlp* p, uint32_t* n_params);
+/* master: FP32 masters (n_params); bf16: the BF16 bit patterns used by the kernels. */
+int mg_mlp_get_params(mg_context* ctx, const mg_mlp* p, float* master, uint16_t* bf16);
+int mg_mlp_set_params(mg_context* ctx, mg_mlp* p, const float* master);
+int mg_mlp_forward(mg_context* ctx, const mg_mlp* p, const float* x, uint32_t n, float* y);
+/* loss = mean over samples and outputs of (y - t)^2; grad (host, n_params) of that loss
+ * with respect to the BF16 weights. Either output may be NULL. Synchronises. */
+int mg_mlp_loss_grad(mg_context* ctx, mg_mlp* p, const float* x, const float* t, uint32_t n, float* loss, float* grad);
+/* Adam (lr, 0.9, 0.999, 1e-8) over mini-batches of `batch` samples in array order. */
+int mg_mlp_train(mg_context* ctx, mg_mlp* p, const float* x, const float* t, uint32_t n, uint32_t epochs,
+                 uint32_t batch, float lr, float* final_loss);
+int mg_mlp_destroy(mg_mlp* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -459,3 +459,137 @@ def read_grid(path):
     _check(lib().mg_io_read_grid_metadata(_dir(path), C.byref(nc), C.byref(nv), C.byref(ne)), "mg_io_read_grid_metadata")
     blocks = [read_block(f"{str(path).rstrip('/')}/block{g}/") for g in range(nc.value * nc.value)]
     return blocks, {"n_chunks": nc.value, "n_vertices": nv.value, "n_edges": ne.value}
+
+
+# ----------------------------------------------------------------------------
+# Batched walks
+# ----------------------------------------------------------------------------
+class WalkBatch:
+    """Many queries (workloads.QuerySpec: graph, sources, hops) over several graphs (A^T),
+    run as one captured HIP graph (mg_walk_batch_*)."""
+
+    def __init__(self, ctx: Context, graphs_at, queries, packing: int = 0, chunk_tiles: int = 0,
+                 use_graph: bool = True):
+        self.ctx = ctx
+        self.graphs = list(graphs_at)
+        self.queries = list(queries)
+        srcs = [np.ascontiguousarray(q.sources, np.uint32) for q in self.queries]
+        arr = (_capi.WalkQuery * len(self.queries))()
+        for i, (q, s) in enumerate(zip(self.queries, srcs)):
+            arr[i].graph = q.graph
+            arr[i].n_sources = len(s)
+            arr[i].sources = s.ctypes.data_as(C.POINTER(C.c_uint32))
+            arr[i].hops = q.hops
+        gp = (C.c_void_p * len(self.graphs))(*[g.h.value if hasattr(g.h, "value") else g.h for g in self.graphs])
+        opt = _capi.BatchOptions(packing, chunk_tiles, 1 if use_graph else 0, 0)
+        h = C.c_void_p()
+        _check(lib().mg_walk_batch_create(ctx.h, gp, len(self.graphs), arr, len(self.queries), C.byref(opt),
+                                          C.byref(h)), "mg_walk_batch_create")
+        self.h = h
+
+    def run(self):
+        _check(lib().mg_walk_batch_run(self.ctx.h, self.h), "mg_walk_batch_run")
+
+    def info(self):
+        p, hmax, ch = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().mg_walk_batch_info(self.h, C.byref(p), C.byref(hmax), C.byref(ch)), "mg_walk_batch_info")
+        return p.value, hmax.value, ch.value
+
+    def totals(self):
+        tp, by, ed = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().mg_walk_batch_totals(self.ctx.h, self.h, C.byref(tp), C.byref(by), C.byref(ed)),
+               "mg_walk_batch_totals")
+        return {"tile_products": tp.value, "alg_bytes": by.value, "edges_visited": ed.value}
+
+    def query_digest(self, q: int) -> int:
+        d = C.c_uint64()
+        _check(lib().mg_walk_batch_query_digest(self.ctx.h, self.h, q, C.byref(d)), "mg_walk_batch_query_digest")
+        return d.value
+
+    def query_frontier(self, q: int) -> np.ndarray:
+        g = self.graphs[self.queries[q].graph]
+        nw = (len(self.queries[q].sources) + 63) // 64
+        out = np.empty((g.n_strips * TILE, nw), np.uint64)
+        _check(lib().mg_walk_batch_query_download(self.ctx.h, self.h, q, _ptr(out)), "mg_walk_batch_query_download")
+        return out
+
+    def query_export(self, q: int) -> TiledMatrix:
+        h = C.c_void_p()
+        _check(lib().mg_walk_batch_query_export(self.ctx.h, self.h, q, C.byref(h)), "mg_walk_batch_query_export")
+        return TiledMatrix(self.ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().mg_walk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------
+# MLP cost model
+# ----------------------------------------------------------------------------
+class MLP:
+    """n_in -> 64 -> 64 -> n_out ReLU MLP, BF16 weights, FP32 accumulation (mg_mlp_*)."""
+
+    def __init__(self, ctx: Context, n_in: int, n_out: int, seed: int = 0):
+        self.ctx, self.n_in, self.n_out = ctx, n_in, n_out
+        h = C.c_void_p()
+        _check(lib().mg_mlp_create(ctx.h, n_in, n_out, seed, C.byref(h)), "mg_mlp_create")
+        self.h = h
+        n = C.c_uint32()
+        _check(lib().mg_mlp_num_params(h, C.byref(n)), "mg_mlp_num_params")
+        self.n_params = n.value
+
+    def params(self):
+        master = np.empty(self.n_params, np.float32)
+        bf16 = np.empty(self.n_params, np.uint16)
+        _check(lib().mg_mlp_get_params(self.ctx.h, self.h, _ptr(master), _ptr(bf16)), "mg_mlp_get_params")
+        return master, bf16
+
+    def set_params(self, master):
+        m = np.ascontiguousarray(master, np.float32)
+        _check(lib().mg_mlp_set_params(self.ctx.h, self.h, _ptr(m)), "mg_mlp_set_params")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        dx = DeviceBuffer.from_array(self.ctx, x)
+        dy = DeviceBuffer(self.ctx, x.shape[0] * self.n_out * 4)
+        _check(lib().mg_mlp_forward(self.ctx.h, self.h, C.c_void_p(dx.ptr), x.shape[0], C.c_void_p(dy.ptr)),
+               "mg_mlp_forward")
+        self.ctx.synchronize()
+        return dy.to_array(np.float32).reshape(x.shape[0], self.n_out)
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        dx, dt = DeviceBuffer.from_array(self.ctx, x), DeviceBuffer.from_array(self.ctx, t)
+        loss = C.c_float()
+        grad = np.empty(self.n_params, np.float32)
+        _check(lib().mg_mlp_loss_grad(self.ctx.h, self.h, C.c_void_p(dx.ptr), C.c_void_p(dt.ptr), x.shape[0],
+                                      C.byref(loss), _ptr(grad)), "mg_mlp_loss_grad")
+        return loss.value, grad
+
+    def train(self, x, t, epochs=10, batch=1024, lr=1e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        dx, dt = DeviceBuffer.from_array(self.ctx, x), DeviceBuffer.from_array(self.ctx, t)
+        loss = C.c_float()
+        _check(lib().mg_mlp_train(self.ctx.h, self.h, C.c_void_p(dx.ptr), C.c_void_p(dt.ptr), x.shape[0], epochs,
+                                  batch, lr, C.byref(loss)), "mg_mlp_train")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().mg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

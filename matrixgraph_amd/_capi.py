@@ -40,6 +40,14 @@ class HopStats(C.Structure):
     ]
 
 
+class WalkQuery(C.Structure):
+    _fields_ = [("graph", u32), ("n_sources", u32), ("sources", C.POINTER(u32)), ("hops", u32)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("packing", u32), ("chunk_tiles", u32), ("use_graph", u32), ("reserved", u32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mg_version": (C.c_char_p, []),
@@ -99,6 +107,25 @@ SIGNATURES = {
     "mg_walk_frontier_digest": (i32, [vp, vp, u32, P_u64]),
     "mg_walk_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
     "mg_walk_destroy": (i32, [vp]),
+    # batched walks
+    "mg_walk_batch_create": (i32, [vp, vp, u32, vp, u32, vp, C.POINTER(vp)]),
+    "mg_walk_batch_run": (i32, [vp, vp]),
+    "mg_walk_batch_info": (i32, [vp, P_u32, P_u32, P_u64]),
+    "mg_walk_batch_query_stats": (i32, [vp, vp, u32, u32, C.POINTER(HopStats)]),
+    "mg_walk_batch_totals": (i32, [vp, vp, P_u64, P_u64, P_u64]),
+    "mg_walk_batch_query_digest": (i32, [vp, vp, u32, P_u64]),
+    "mg_walk_batch_query_download": (i32, [vp, vp, u32, vp]),
+    "mg_walk_batch_query_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
+    "mg_walk_batch_destroy": (i32, [vp]),
+    # mlp
+    "mg_mlp_create": (i32, [vp, u32, u32, u64, C.POINTER(vp)]),
+    "mg_mlp_num_params": (i32, [vp, P_u32]),
+    "mg_mlp_get_params": (i32, [vp, vp, vp, vp]),
+    "mg_mlp_set_params": (i32, [vp, vp, vp]),
+    "mg_mlp_forward": (i32, [vp, vp, vp, u32, vp]),
+    "mg_mlp_loss_grad": (i32, [vp, vp, vp, vp, u32, C.POINTER(C.c_float), vp]),
+    "mg_mlp_train": (i32, [vp, vp, vp, vp, u32, u32, u32, C.c_float, C.POINTER(C.c_float)]),
+    "mg_mlp_destroy": (i32, [vp]),
 }
 
 _lib = None

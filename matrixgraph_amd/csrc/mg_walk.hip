@@ -8,11 +8,17 @@
 // tiles): vertex v's source set is W consecutive u64, so one set bit of A^T costs one
 // coalesced W*8-byte gather, OR-ed into the LDS accumulator of strip y.
 //
-// Per hop:
-//   symbolic   S_{k+1}[y] = OR_{t in A^T.row(y)} S_k[t]   (u64 per strip: tile rows x of X
-//              structurally present), with tile-product / tile counts for B_alg;
-//   numeric    the gather/OR kernel below, streaming A^T tiles in scheduler chunks;
-//   finalize   per-strip numeric masks for strips split across chunks.
+// Panels. A panel is up to 64*W sources of one graph walked together (one walk, or many
+// batched queries packed at word boundaries). Every launch covers one hop level of any
+// number of panels: chunks and items carry a panel index into a table of per-level
+// panel descriptors. Words of queries that have reached their hop count are masked out
+// of later gathers (alive masks), so each X_k keeps the finished queries' results.
+//
+// Per level k (X_k -> X_{k+1}):
+//   symbolic   S_{k+1}[y] = OR_{t in A^T.row(y)} (S_k[t] & alive) per strip, with
+//              tile-product / tile counts for B_alg;
+//   numeric    the gather/OR kernel, streaming A^T tiles in scheduler chunks;
+//   finalize   per-strip masks for strips split across chunks.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -27,59 +33,151 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
-constexpr uint32_t kChunkTiles = 1024;   // scheduler target tiles per chunk
-constexpr uint32_t kChunkStrips = 32;    // at most this many strips per chunk
-constexpr int kLoadsInFlight = 8;        // frontier gathers per wave flush
-constexpr int kTileBatch = 8;            // A^T tile loads in flight per wave
+constexpr uint32_t kDefaultChunkTiles = 1024;  // scheduler target tiles per chunk
+constexpr uint32_t kChunkStrips = 32;          // at most this many strips per chunk
+constexpr uint32_t kSymStrips = 16;            // strips per symbolic work item (4 per wave)
+constexpr int kLoadsInFlight = 8;              // frontier gathers per wave flush
+constexpr int kTileBatch = 8;                  // A^T tile loads in flight per wave
 
 enum Counter { kTP = 0, kLeft, kRightUsed, kOut, kOutNumeric, kPop, kEdges, kNumCounters };
 
-// Chunk of the A^T tile stream: strips [strip_begin, strip_end) restricted to tiles
-// [tile_begin, tile_end). A strip whose tiles exceed one chunk is split into pieces.
-struct Chunk {
-  uint32_t strip_begin, strip_end, tile_begin, tile_end;
+// One panel at one hop level: the launch's inputs (X_k side) and outputs (X_{k+1} side).
+struct PanelLevel {
+  const uint32_t* off;
+  const uint32_t* col;
+  const uint64_t* data;
+  const uint64_t* x_in;
+  const uint64_t* v_in;
+  const uint64_t* s_in;
+  uint64_t* x_out;
+  uint64_t* n_out;
+  uint64_t* v_out;
+  uint64_t* s_out;
+  unsigned long long* counters;
+  uint64_t alive_in;    // words of X_k that propagate into X_{k+1}
+  uint64_t alive_next;  // words of X_{k+1} that propagate further (they decide V_{k+1})
+  uint32_t n_strips;
+  uint32_t pad;
 };
 
-// --- X_0 --------------------------------------------------------------------
-__global__ void init_sources_kernel(const uint32_t* __restrict__ src, uint32_t n_sources, uint32_t W,
-                                    unsigned long long* __restrict__ x0, unsigned long long* __restrict__ s0,
-                                    unsigned long long* __restrict__ v0) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_sources) return;
-  const uint32_t v = src[s];
-  atomicOr(x0 + static_cast<uint64_t>(v) * W + s / 64, 1ull << (s % 64));
-  atomicOr(s0 + v / 64, 1ull << (s / 64));
-  atomicOr(v0 + v / 64, 1ull << (v % 64));
+// Chunk of one panel's A^T tile stream: strips [strip_begin, strip_end) restricted to
+// tiles [tile_begin, tile_end). A strip whose tiles exceed one chunk is split in pieces.
+struct Chunk {
+  uint32_t panel, strip_begin, strip_end, tile_begin, tile_end;
+};
+struct StripItem {
+  uint32_t panel, strip;
+};
+struct SymItem {
+  uint32_t panel, strip_begin, strip_end;
+};
+// Source s of a panel: bit `col` of the X_0 row of vertex `vertex`.
+struct SourceItem {
+  uint32_t panel, col, vertex;
+};
+struct PanelInit {
+  uint64_t* x0;
+  uint64_t* s0;
+  uint64_t* v0;
+  uint64_t* n0;
+};
+struct FillRange {
+  uint64_t* ptr;
+  uint64_t n;
+};
+
+// --- fills and X_0 ------------------------------------------------------------
+// Zero-fill of a list of ranges (a kernel, so it can sit inside a captured graph).
+__global__ void __launch_bounds__(kBlock) fill_zero_kernel(const FillRange* __restrict__ ranges, uint32_t n_ranges) {
+  for (uint32_t r = 0; r < n_ranges; ++r) {
+    uint64_t* p = ranges[r].ptr;
+    const uint64_t n = ranges[r].n;
+    const uint64_t n2 = n / 2;
+    ulonglong2* p2 = reinterpret_cast<ulonglong2*>(p);
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n2;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+      p2[i] = make_ulonglong2(0, 0);
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) p[n - 1] = 0;
+  }
+}
+
+// Before a run: zero the strips of X_0 that the previous run set (their numeric mask is
+// non-zero) and reset the X_0 masks. X_0 buffers start zeroed at allocation.
+template <int W>
+__global__ void __launch_bounds__(kBlock) clear_x0_kernel(const PanelInit* __restrict__ panels,
+                                                          const uint32_t* __restrict__ n_strips, uint32_t n_panels) {
+  constexpr int kWords = 64 * W;
+  uint32_t item = blockIdx.x;
+  for (uint32_t p = 0; p < n_panels; ++p) {
+    const uint32_t ns = n_strips[p];
+    for (uint32_t y = item; y < ns; y += gridDim.x) {
+      const PanelInit P = panels[p];
+      if (P.n0[y] != 0) {
+        uint64_t* dst = P.x0 + static_cast<uint64_t>(y) * kWords;
+        for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = 0;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        P.n0[y] = 0;
+        P.s0[y] = 0;
+        P.v0[y] = 0;
+      }
+    }
+  }
+}
+
+__global__ void init_sources_kernel(const SourceItem* __restrict__ items, uint32_t n_items,
+                                    const PanelInit* __restrict__ panels, uint32_t W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  const SourceItem it = items[i];
+  const PanelInit p = panels[it.panel];
+  const uint32_t v = it.vertex;
+  atomicOr(reinterpret_cast<unsigned long long*>(p.x0) + static_cast<uint64_t>(v) * W + it.col / 64,
+           1ull << (it.col % 64));
+  atomicOr(reinterpret_cast<unsigned long long*>(p.s0) + v / 64, 1ull << (it.col / 64));
+  atomicOr(reinterpret_cast<unsigned long long*>(p.n0) + v / 64, 1ull << (it.col / 64));
+  atomicOr(reinterpret_cast<unsigned long long*>(p.v0) + v / 64, 1ull << (v % 64));
 }
 
 // --- symbolic ---------------------------------------------------------------
-// One wave per strip y. Also accumulates T_L (structural tiles of X_k, counted once
-// per strip), tile products and T_R,used.
-__global__ void __launch_bounds__(kBlock) walk_symbolic_kernel(uint32_t n_strips, const uint32_t* __restrict__ off,
-                                                               const uint32_t* __restrict__ col,
-                                                               const uint64_t* __restrict__ s_in,
-                                                               uint64_t* __restrict__ s_out,
-                                                               unsigned long long* __restrict__ counters) {
+// One wave per strip. Accumulates T_L (structural tiles of X_k, counted once per strip),
+// tile products and T_R,used of the panel, restricted to alive tile rows. Each lane keeps
+// kSymUnroll column-index loads in flight before the dependent mask loads.
+constexpr int kSymUnroll = 4;
+
+__global__ void __launch_bounds__(kBlock) walk_symbolic_kernel(const SymItem* __restrict__ items,
+                                                               const PanelLevel* __restrict__ lv) {
+  const SymItem it = items[blockIdx.x];
+  const PanelLevel& L = lv[it.panel];
+  const uint64_t* s_in = L.s_in;  // structural masks, or numeric ones when pruning
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint64_t alive = L.alive_in;
   __shared__ unsigned long long s_acc[4];
   if (threadIdx.x < 4) s_acc[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t lane = lane_id();
-  const uint32_t n_waves = gridDim.x * kWaves;
   uint64_t tp = 0, used = 0, left = 0, out = 0;
-  for (uint32_t y = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); y < n_strips; y += n_waves) {
-    const uint32_t p0 = off[y], p1 = off[y + 1];
+  for (uint32_t y = it.strip_begin + wave; y < it.strip_end; y += kWaves) {
+    const uint32_t p0 = L.off[y], p1 = L.off[y + 1];
     uint64_t s = 0;
-    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
-      const uint64_t m = s_in[col[p]];
-      s |= m;
-      tp += __popcll(m);
-      used += (m != 0);
+    for (uint32_t p = p0 + lane; p < p1; p += kWave * kSymUnroll) {
+      uint32_t c[kSymUnroll];
+#pragma unroll
+      for (int j = 0; j < kSymUnroll; ++j) c[j] = (p + j * kWave < p1) ? L.col[p + j * kWave] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int j = 0; j < kSymUnroll; ++j) {
+        const uint64_t m = (c[j] != 0xFFFFFFFFu) ? (s_in[c[j]] & alive) : 0;
+        s |= m;
+        tp += __popcll(m);
+        used += (m != 0);
+      }
     }
     s = wave_or(s);
     if (lane == 0) {
-      s_out[y] = s;
+      L.s_out[y] = s;
       out += __popcll(s);
-      left += __popcll(s_in[y]);
+      left += __popcll(s_in[y] & alive);
     }
   }
   tp = wave_sum(tp);
@@ -87,17 +185,15 @@ __global__ void __launch_bounds__(kBlock) walk_symbolic_kernel(uint32_t n_strips
   if (lane == 0) {
     atomicAdd(&s_acc[0], static_cast<unsigned long long>(tp));
     atomicAdd(&s_acc[1], static_cast<unsigned long long>(used));
-  }
-  if (lane == 0) {
     atomicAdd(&s_acc[2], static_cast<unsigned long long>(left));
     atomicAdd(&s_acc[3], static_cast<unsigned long long>(out));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(counters + kTP, s_acc[0]);
-    atomicAdd(counters + kRightUsed, s_acc[1]);
-    atomicAdd(counters + kLeft, s_acc[2]);
-    atomicAdd(counters + kOut, s_acc[3]);
+    if (s_acc[0]) atomicAdd(L.counters + kTP, s_acc[0]);
+    if (s_acc[1]) atomicAdd(L.counters + kRightUsed, s_acc[1]);
+    if (s_acc[2]) atomicAdd(L.counters + kLeft, s_acc[2]);
+    if (s_acc[3]) atomicAdd(L.counters + kOut, s_acc[3]);
   }
 }
 
@@ -119,9 +215,10 @@ struct QueueShape {
   static constexpr int kLoads = (kCapacity + kJobsPerLoad - 1) / kJobsPerLoad;
 };
 
+// keep = alive mask of this lane's word (all ones or zero).
 template <int W>
 __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ x_in,
-                                   unsigned long long* __restrict__ acc, uint32_t lane) {
+                                   unsigned long long* __restrict__ acc, uint32_t lane, uint64_t keep) {
   using S = QueueShape<W>;
   uint64_t v[S::kLoads];
   if constexpr (W == 64) {
@@ -131,7 +228,7 @@ __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ 
       v[j] = 0;
       if (static_cast<uint32_t>(j) < q.len) {
         const uint32_t u = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(q.u), j));
-        v[j] = x_in[static_cast<uint64_t>(u) * W + lane];
+        v[j] = x_in[static_cast<uint64_t>(u) * W + lane] & keep;
       }
     }
 #pragma unroll
@@ -154,7 +251,7 @@ __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ 
     for (int j = 0; j < S::kLoads; ++j) {
       const uint32_t job = j * S::kJobsPerLoad + slot;
       const uint32_t u = static_cast<uint32_t>(__shfl(static_cast<int>(q.u), static_cast<int>(job & 63), kWave));
-      v[j] = (job < q.len) ? x_in[static_cast<uint64_t>(u) * W + w] : 0;
+      v[j] = (job < q.len) ? (x_in[static_cast<uint64_t>(u) * W + w] & keep) : 0;
     }
 #pragma unroll
     for (int j = 0; j < S::kLoads; ++j) {
@@ -177,12 +274,13 @@ __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ 
 
 template <int W>
 __device__ inline void push_job(GatherQueue& q, uint32_t u, uint64_t rows, uint32_t lane,
-                                const uint64_t* __restrict__ x_in, unsigned long long* __restrict__ acc) {
+                                const uint64_t* __restrict__ x_in, unsigned long long* __restrict__ acc,
+                                uint64_t keep) {
   if (lane == q.len) {
     q.u = u;
     q.rows = rows;
   }
-  if (++q.len == QueueShape<W>::kCapacity) flush_queue<W>(q, x_in, acc, lane);
+  if (++q.len == QueueShape<W>::kCapacity) flush_queue<W>(q, x_in, acc, lane, keep);
 }
 
 // Extract the gather jobs of one A^T tile: for each source column t' with a set bit
@@ -190,7 +288,7 @@ __device__ inline void push_job(GatherQueue& q, uint32_t u, uint64_t rows, uint3
 template <int W>
 __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueue& q, uint32_t lane,
                                  const uint64_t* __restrict__ x_in, unsigned long long* __restrict__ acc,
-                                 uint64_t& edges) {
+                                 uint64_t keep, uint64_t& edges) {
   b &= vm;
   const uint64_t nz = ballot(b != 0);
   if (nz == 0) return;
@@ -211,23 +309,29 @@ __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueu
     const int tp = __ffsll(static_cast<unsigned long long>(cols)) - 1;
     cols &= cols - 1;
     const uint64_t rows = ballot((b >> tp) & 1);
-    push_job<W>(q, t * 64u + static_cast<uint32_t>(tp), rows, lane, x_in, acc);
+    push_job<W>(q, t * 64u + static_cast<uint32_t>(tp), rows, lane, x_in, acc, keep);
   }
 }
 
 // One workgroup per scheduler chunk; per strip a 64 x W-word LDS accumulator.
 template <int W>
-__global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
-    const Chunk* __restrict__ chunks, const uint32_t* __restrict__ off, const uint32_t* __restrict__ col,
-    const uint64_t* __restrict__ data, const uint64_t* __restrict__ x_in, const uint64_t* __restrict__ v_in,
-    uint64_t* __restrict__ x_out, uint64_t* __restrict__ n_out, uint64_t* __restrict__ v_out,
-    unsigned long long* __restrict__ counters) {
+__global__ void __launch_bounds__(kBlock) walk_numeric_kernel(const Chunk* __restrict__ chunks,
+                                                              const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;  // accumulator words per strip
   __shared__ unsigned long long acc[kWords];
   __shared__ unsigned long long s_n, s_v, s_pop;
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   const Chunk ch = chunks[blockIdx.x];
+  const PanelLevel& L = lv[ch.panel];
+  const uint32_t* __restrict__ off = L.off;
+  const uint32_t* __restrict__ col = L.col;
+  const uint64_t* __restrict__ data = L.data;
+  const uint64_t* __restrict__ x_in = L.x_in;
+  const uint64_t* __restrict__ v_in = L.v_in;
+  uint64_t* __restrict__ x_out = L.x_out;
+  const uint64_t keep = ((L.alive_in >> ((W == 64) ? lane : (lane % W))) & 1) ? ~0ull : 0ull;
+  const uint64_t alive_next = L.alive_next;
   uint64_t edges = 0, pop_total = 0, numeric_tiles = 0;
 
   for (uint32_t y = ch.strip_begin; y < ch.strip_end; ++y) {
@@ -274,24 +378,24 @@ __global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
           if (j < cnt) {
             const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), ks[j]));
             const uint64_t vm = read_lane(vm_l, ks[j]);
-            tile_jobs<W>(t, b[j], vm, q, lane, x_in, acc, edges);
+            tile_jobs<W>(t, b[j], vm, q, lane, x_in, acc, keep, edges);
           }
         }
       }
     }
-    if (q.len) flush_queue<W>(q, x_in, acc, lane);
+    if (q.len) flush_queue<W>(q, x_in, acc, lane, keep);
     __syncthreads();
 
-    // Write the strip out. Unsplit strips also reduce their numeric masks here.
+    // Write the strip out. Unsplit strips also reduce their masks here; a strip that is
+    // all zero and already zero in memory (its previous numeric mask is 0) is not stored.
     uint64_t* dst = x_out + static_cast<uint64_t>(y) * kWords;
     if (!split) {
       uint64_t nmask = 0, vmask = 0, pc = 0;
       for (int i = threadIdx.x; i < kWords; i += kBlock) {
         const uint64_t val = acc[i];
-        dst[i] = val;
         if (val) {
           nmask |= 1ull << (i % W);
-          vmask |= 1ull << (i / W);
+          if ((alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
           pc += __popcll(val);
         }
       }
@@ -304,9 +408,13 @@ __global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
         if (pc) atomicAdd(&s_pop, static_cast<unsigned long long>(pc));
       }
       __syncthreads();
+      const bool store = (s_n != 0) || (L.n_out[y] != 0);
+      if (store)
+        for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = acc[i];
+      __syncthreads();  // every thread has read n_out[y] before it changes
       if (threadIdx.x == 0) {
-        n_out[y] = s_n;
-        v_out[y] = s_v;
+        L.n_out[y] = s_n;
+        L.v_out[y] = s_v;
         pop_total += s_pop;
         numeric_tiles += __popcll(s_n);
       }
@@ -319,40 +427,40 @@ __global__ void __launch_bounds__(kBlock) walk_numeric_kernel(
     __syncthreads();
   }
   edges = wave_sum(edges);
-  if (lane == 0 && edges) atomicAdd(counters + kEdges, static_cast<unsigned long long>(edges));
+  if (lane == 0 && edges) atomicAdd(L.counters + kEdges, static_cast<unsigned long long>(edges));
   if (threadIdx.x == 0) {
-    if (pop_total) atomicAdd(counters + kPop, static_cast<unsigned long long>(pop_total));
-    if (numeric_tiles) atomicAdd(counters + kOutNumeric, static_cast<unsigned long long>(numeric_tiles));
+    if (pop_total) atomicAdd(L.counters + kPop, static_cast<unsigned long long>(pop_total));
+    if (numeric_tiles) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(numeric_tiles));
   }
 }
 
 // Zero the strips that are written by several chunks (atomic OR targets).
 template <int W>
-__global__ void __launch_bounds__(kBlock) zero_strips_kernel(const uint32_t* __restrict__ strips, uint64_t* __restrict__ x) {
+__global__ void __launch_bounds__(kBlock) zero_strips_kernel(const StripItem* __restrict__ items,
+                                                             const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;
-  uint64_t* dst = x + static_cast<uint64_t>(strips[blockIdx.x]) * kWords;
+  const StripItem it = items[blockIdx.x];
+  uint64_t* dst = lv[it.panel].x_out + static_cast<uint64_t>(it.strip) * kWords;
   for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = 0;
 }
 
 // Masks and popcount of the split strips once every piece has been OR-ed in.
 template <int W>
-__global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const uint32_t* __restrict__ strips,
-                                                                 const uint64_t* __restrict__ x,
-                                                                 uint64_t* __restrict__ n_out,
-                                                                 uint64_t* __restrict__ v_out,
-                                                                 unsigned long long* __restrict__ counters) {
+__global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem* __restrict__ items,
+                                                                 const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;
   __shared__ unsigned long long s_n, s_v, s_pop;
   if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_pop = 0; }
   __syncthreads();
-  const uint32_t y = strips[blockIdx.x];
-  const uint64_t* src = x + static_cast<uint64_t>(y) * kWords;
+  const StripItem it = items[blockIdx.x];
+  const PanelLevel& L = lv[it.panel];
+  const uint64_t* src = L.x_out + static_cast<uint64_t>(it.strip) * kWords;
   uint64_t nmask = 0, vmask = 0, pc = 0;
   for (int i = threadIdx.x; i < kWords; i += kBlock) {
     const uint64_t val = src[i];
     if (val) {
       nmask |= 1ull << (i % W);
-      vmask |= 1ull << (i / W);
+      if ((L.alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
       pc += __popcll(val);
     }
   }
@@ -366,40 +474,51 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const uint32_t*
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    n_out[y] = s_n;
-    v_out[y] = s_v;
-    if (s_pop) atomicAdd(counters + kPop, s_pop);
-    if (s_n) atomicAdd(counters + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
+    L.n_out[it.strip] = s_n;
+    L.v_out[it.strip] = s_v;
+    if (s_pop) atomicAdd(L.counters + kPop, s_pop);
+    if (s_n) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
   }
 }
 
-// Digest of a vertex-major frontier: non-zero words of the first n_words source words.
+// Digest of words [w0, w0+n_words) of a vertex-major frontier (word index relative to w0).
 __global__ void __launch_bounds__(kBlock) frontier_digest_kernel(const uint64_t* __restrict__ x, uint64_t n_vertices,
-                                                                 uint32_t W, uint32_t n_words,
+                                                                 uint32_t W, uint32_t w0, uint32_t n_words,
                                                                  unsigned long long* __restrict__ out) {
   uint64_t acc = 0;
-  const uint64_t total = n_vertices * W;
+  const uint64_t total = n_vertices * n_words;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const uint32_t w = static_cast<uint32_t>(i % W);
-    const uint64_t val = x[i];
-    if (val && w < n_words) acc += digest_term((i / W) * 64 + w, val);
+    const uint64_t v = i / n_words;
+    const uint32_t w = static_cast<uint32_t>(i % n_words);
+    const uint64_t val = x[v * W + w0 + w];
+    if (val) acc += digest_term(v * 64 + w, val);
   }
   acc = wave_sum(acc);
   if (lane_id() == 0 && acc) atomicAdd(out, static_cast<unsigned long long>(acc));
 }
 
-// Export X_k tile (x, y) in the reference orientation: X tile word c (source x*64+c)
-// bit r (vertex y*64+r) = bit c of vertex-major word x of vertex y*64+r.
+// Copy words [w0, w0+n_words) of every vertex into a compact (n_vertices x n_words) array.
+__global__ void __launch_bounds__(kBlock) frontier_slice_kernel(const uint64_t* __restrict__ x, uint64_t n_vertices,
+                                                                uint32_t W, uint32_t w0, uint32_t n_words,
+                                                                uint64_t* __restrict__ out) {
+  const uint64_t total = n_vertices * n_words;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride)
+    out[i] = x[(i / n_words) * W + w0 + (i % n_words)];
+}
+
+// Export tile (xr, y) of X_k in the reference orientation: X tile word c (source xr*64+c)
+// bit r (vertex y*64+r) = bit c of vertex-major word w0+xr of vertex y*64+r.
 __global__ void __launch_bounds__(kBlock) export_tiles_kernel(const uint32_t* __restrict__ row_idx,
                                                               const uint32_t* __restrict__ col_idx, uint32_t n_tiles,
-                                                              const uint64_t* __restrict__ x, uint32_t W,
+                                                              const uint64_t* __restrict__ x, uint32_t W, uint32_t w0,
                                                               uint64_t* __restrict__ out) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * kWaves;
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
     const uint32_t xr = row_idx[i], y = col_idx[i];
-    const uint64_t w = x[(static_cast<uint64_t>(y) * 64 + lane) * W + xr];
+    const uint64_t w = x[(static_cast<uint64_t>(y) * 64 + lane) * W + w0 + xr];
     uint64_t mine = 0;
     for (int c = 0; c < 64; ++c) {
       const uint64_t m = ballot((w >> c) & 1);
@@ -409,35 +528,287 @@ __global__ void __launch_bounds__(kBlock) export_tiles_kernel(const uint32_t* __
   }
 }
 
-std::vector<Chunk> build_chunks(const std::vector<uint32_t>& off, uint32_t n_strips, std::vector<uint32_t>* split) {
+// Scheduler: chunks over one graph's A^T tile stream (panel index filled in later).
+std::vector<Chunk> build_chunks(const std::vector<uint32_t>& off, uint32_t n_strips, uint32_t chunk_tiles,
+                                std::vector<uint32_t>* split) {
   std::vector<Chunk> chunks;
   uint32_t cur_begin = 0, cur_tiles = 0, cur_strips = 0;
   auto close = [&](uint32_t end_strip) {
-    if (end_strip > cur_begin) chunks.push_back({cur_begin, end_strip, off[cur_begin], off[end_strip]});
+    if (end_strip > cur_begin) chunks.push_back({0, cur_begin, end_strip, off[cur_begin], off[end_strip]});
     cur_begin = end_strip;
     cur_tiles = 0;
     cur_strips = 0;
   };
   for (uint32_t y = 0; y < n_strips; ++y) {
     const uint32_t sz = off[y + 1] - off[y];
-    if (sz > kChunkTiles) {
+    if (sz > chunk_tiles) {
       close(y);
       split->push_back(y);
-      for (uint32_t b = off[y]; b < off[y + 1]; b += kChunkTiles)
-        chunks.push_back({y, y + 1, b, std::min(off[y + 1], b + kChunkTiles)});
+      for (uint32_t b = off[y]; b < off[y + 1]; b += chunk_tiles)
+        chunks.push_back({0, y, y + 1, b, std::min(off[y + 1], b + chunk_tiles)});
       cur_begin = y + 1;
       continue;
     }
-    if (cur_tiles + sz > kChunkTiles || cur_strips == kChunkStrips) close(y);
+    if (cur_tiles + sz > chunk_tiles || cur_strips == kChunkStrips) close(y);
     cur_tiles += sz;
     ++cur_strips;
   }
   close(n_strips);
-  // Heaviest chunks first so the tail of the launch is made of light ones.
-  std::stable_sort(chunks.begin(), chunks.end(), [](const Chunk& a, const Chunk& b) {
-    return (a.tile_end - a.tile_begin) > (b.tile_end - b.tile_begin);
-  });
   return chunks;
+}
+
+// Per-graph scheduler data, built once from the A^T row pointer.
+struct GraphPlan {
+  const mg_tiled_matrix* at = nullptr;
+  uint32_t n_strips = 0;
+  std::vector<Chunk> chunks;
+  std::vector<uint32_t> split;
+};
+
+GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
+  GraphPlan g;
+  g.at = at;
+  g.n_strips = at->n_strips;
+  std::vector<uint32_t> off(at->n_strips + 1ull);
+  MG_HIP(hipMemcpy(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
+  g.chunks = build_chunks(off, at->n_strips, chunk_tiles, &g.split);
+  return g;
+}
+
+}  // namespace
+
+// One panel's device state: frontiers and masks for levels 0..hops.
+struct Panel {
+  uint32_t graph = 0;
+  uint32_t hops = 0;
+  uint32_t n_strips = 0;
+  uint64_t words_used = 0;  // words of X holding sources
+  std::vector<uint64_t> alive;  // alive[k]: words that propagate from X_k to X_{k+1}
+  std::vector<DevBuf<uint64_t>> x, s, n, v;
+};
+
+// Device launch plan for a set of panels over levels 0..max_hops-1.
+struct Engine {
+  uint32_t W = 64;
+  std::vector<GraphPlan> graphs;
+  std::vector<Panel> panels;
+  uint32_t max_hops = 0;
+  DevBuf<unsigned long long> counters;  // panels * (max_hops+1) * kNumCounters
+  // per level
+  std::vector<DevBuf<PanelLevel>> lv;
+  std::vector<DevBuf<Chunk>> chunks;
+  std::vector<uint32_t> n_chunks;
+  std::vector<DevBuf<StripItem>> split;
+  std::vector<uint32_t> n_split;
+  std::vector<DevBuf<SymItem>> sym;
+  std::vector<uint32_t> n_sym;
+  // X_0 setup
+  DevBuf<PanelInit> inits;
+  DevBuf<uint32_t> panel_strips;
+  DevBuf<FillRange> fills;
+  uint32_t n_fills = 0;
+  DevBuf<SourceItem> sources;
+  uint32_t n_sources = 0;
+  int prune = 0;
+
+  unsigned long long* ctr(uint32_t p, uint32_t level) {
+    return counters.get() + (static_cast<size_t>(p) * (max_hops + 1) + level) * kNumCounters;
+  }
+};
+
+namespace {
+
+// Allocate frontiers, build the per-level descriptor tables, chunk lists and items.
+void build_engine(Engine* e) {
+  const uint32_t P = static_cast<uint32_t>(e->panels.size());
+  e->max_hops = 0;
+  for (auto& p : e->panels) e->max_hops = std::max(e->max_hops, p.hops);
+  for (auto& p : e->panels) {
+    const uint64_t nv = static_cast<uint64_t>(p.n_strips) * 64;
+    p.x.resize(p.hops + 1);
+    p.s.resize(p.hops + 1);
+    p.n.resize(p.hops + 1);
+    p.v.resize(p.hops + 1);
+    for (uint32_t k = 0; k <= p.hops; ++k) {
+      p.x[k].alloc(nv * e->W);
+      p.s[k].alloc(p.n_strips);
+      p.n[k].alloc(p.n_strips);
+      p.v[k].alloc(p.n_strips);
+      // All strips start zero and known to be zero (numeric masks 0); later runs rely on
+      // that to skip stores of strips that stay zero.
+      MG_HIP(hipMemset(p.x[k].get(), 0, p.x[k].bytes()));
+      MG_HIP(hipMemset(p.n[k].get(), 0, p.n[k].bytes()));
+      MG_HIP(hipMemset(p.s[k].get(), 0, p.s[k].bytes()));
+      MG_HIP(hipMemset(p.v[k].get(), 0, p.v[k].bytes()));
+    }
+  }
+  e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kNumCounters);
+  e->lv.resize(e->max_hops);
+  e->chunks.resize(e->max_hops);
+  e->n_chunks.assign(e->max_hops, 0);
+  e->split.resize(e->max_hops);
+  e->n_split.assign(e->max_hops, 0);
+  e->sym.resize(e->max_hops);
+  e->n_sym.assign(e->max_hops, 0);
+  for (uint32_t k = 0; k < e->max_hops; ++k) {
+    std::vector<PanelLevel> lv(P);
+    std::vector<Chunk> ch;
+    std::vector<StripItem> sp;
+    std::vector<SymItem> sy;
+    for (uint32_t pi = 0; pi < P; ++pi) {
+      Panel& p = e->panels[pi];
+      if (k >= p.hops) continue;
+      const GraphPlan& g = e->graphs[p.graph];
+      PanelLevel& L = lv[pi];
+      L.off = g.at->offset_row.get();
+      L.col = g.at->col_idx.get();
+      L.data = g.at->data.get();
+      L.x_in = p.x[k].get();
+      L.v_in = p.v[k].get();
+      L.s_in = e->prune ? p.n[k].get() : p.s[k].get();
+      L.x_out = p.x[k + 1].get();
+      L.n_out = p.n[k + 1].get();
+      L.v_out = p.v[k + 1].get();
+      L.s_out = p.s[k + 1].get();
+      L.counters = e->ctr(pi, k + 1);
+      L.alive_in = p.alive[k];
+      L.alive_next = (k + 1 < p.hops) ? p.alive[k + 1] : 0;
+      L.n_strips = p.n_strips;
+      for (Chunk c : g.chunks) {
+        c.panel = pi;
+        ch.push_back(c);
+      }
+      for (uint32_t y : g.split) sp.push_back({pi, y});
+      for (uint32_t y0 = 0; y0 < p.n_strips; y0 += kSymStrips) sy.push_back({pi, y0, std::min(p.n_strips, y0 + kSymStrips)});
+    }
+    // Heaviest chunks first so the tail of the launch is made of light ones. Among equal
+    // sizes, the same chunk of every panel of one graph is adjacent, so the panels' reads of
+    // those A^T tiles follow each other closely and the repeats are served from cache.
+    std::sort(ch.begin(), ch.end(), [&](const Chunk& a, const Chunk& b) {
+      const uint32_t ta = a.tile_end - a.tile_begin, tb = b.tile_end - b.tile_begin;
+      if (ta != tb) return ta > tb;
+      const uint32_t ga = e->panels[a.panel].graph, gb = e->panels[b.panel].graph;
+      if (ga != gb) return ga < gb;
+      if (a.strip_begin != b.strip_begin) return a.strip_begin < b.strip_begin;
+      if (a.tile_begin != b.tile_begin) return a.tile_begin < b.tile_begin;
+      return a.panel < b.panel;
+    });
+    e->lv[k].alloc(P);
+    MG_HIP(hipMemcpy(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice));
+    e->n_chunks[k] = static_cast<uint32_t>(ch.size());
+    e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
+    if (!ch.empty()) MG_HIP(hipMemcpy(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    e->n_split[k] = static_cast<uint32_t>(sp.size());
+    e->split[k].alloc(std::max<size_t>(1, sp.size()));
+    if (!sp.empty()) MG_HIP(hipMemcpy(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice));
+    e->n_sym[k] = static_cast<uint32_t>(sy.size());
+    e->sym[k].alloc(std::max<size_t>(1, sy.size()));
+    if (!sy.empty()) MG_HIP(hipMemcpy(e->sym[k].get(), sy.data(), sizeof(SymItem) * sy.size(), hipMemcpyHostToDevice));
+  }
+  // X_0 fills: frontier and masks of every panel, plus all counters.
+  std::vector<PanelInit> inits(P);
+  std::vector<uint32_t> strips(P);
+  std::vector<FillRange> fills;
+  for (uint32_t pi = 0; pi < P; ++pi) {
+    Panel& p = e->panels[pi];
+    inits[pi] = {p.x[0].get(), p.s[0].get(), p.v[0].get(), p.n[0].get()};
+    strips[pi] = p.n_strips;
+  }
+  fills.push_back({reinterpret_cast<uint64_t*>(e->counters.get()), e->counters.size()});
+  e->inits.alloc(P);
+  MG_HIP(hipMemcpy(e->inits.get(), inits.data(), sizeof(PanelInit) * P, hipMemcpyHostToDevice));
+  e->panel_strips.alloc(P);
+  MG_HIP(hipMemcpy(e->panel_strips.get(), strips.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice));
+  e->n_fills = static_cast<uint32_t>(fills.size());
+  e->fills.alloc(fills.size());
+  MG_HIP(hipMemcpy(e->fills.get(), fills.data(), sizeof(FillRange) * fills.size(), hipMemcpyHostToDevice));
+}
+
+template <int W>
+void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
+  if (ev) MG_HIP(hipEventRecord(ev[0], st));
+  if (e->n_sym[k]) {
+    walk_symbolic_kernel<<<e->n_sym[k], kBlock, 0, st>>>(e->sym[k].get(), e->lv[k].get());
+    MG_LAUNCH_CHECK();
+  }
+  if (ev) MG_HIP(hipEventRecord(ev[1], st));
+  if (e->n_split[k]) {
+    zero_strips_kernel<W><<<e->n_split[k], kBlock, 0, st>>>(e->split[k].get(), e->lv[k].get());
+    MG_LAUNCH_CHECK();
+  }
+  if (ev) MG_HIP(hipEventRecord(ev[2], st));
+  if (e->n_chunks[k]) {
+    walk_numeric_kernel<W><<<e->n_chunks[k], kBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+    MG_LAUNCH_CHECK();
+  }
+  if (ev) MG_HIP(hipEventRecord(ev[3], st));
+  if (e->n_split[k]) {
+    finalize_strips_kernel<W><<<e->n_split[k], kBlock, 0, st>>>(e->split[k].get(), e->lv[k].get());
+    MG_LAUNCH_CHECK();
+  }
+}
+
+void launch_level(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
+  switch (e->W) {
+    case 1: launch_level_w<1>(st, e, k, ev); break;
+    case 2: launch_level_w<2>(st, e, k, ev); break;
+    case 4: launch_level_w<4>(st, e, k, ev); break;
+    case 8: launch_level_w<8>(st, e, k, ev); break;
+    case 16: launch_level_w<16>(st, e, k, ev); break;
+    case 32: launch_level_w<32>(st, e, k, ev); break;
+    case 64: launch_level_w<64>(st, e, k, ev); break;
+    default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex " + std::to_string(e->W));
+  }
+}
+
+// X_0 fills + source bits (sources already on the device).
+template <int W>
+void launch_clear_x0(hipStream_t st, Engine* e) {
+  clear_x0_kernel<W><<<2048, kBlock, 0, st>>>(e->inits.get(), e->panel_strips.get(),
+                                              static_cast<uint32_t>(e->panels.size()));
+  MG_LAUNCH_CHECK();
+}
+
+void launch_init(hipStream_t st, Engine* e, uint32_t n_sources) {
+  fill_zero_kernel<<<64, kBlock, 0, st>>>(e->fills.get(), e->n_fills);
+  MG_LAUNCH_CHECK();
+  switch (e->W) {
+    case 1: launch_clear_x0<1>(st, e); break;
+    case 2: launch_clear_x0<2>(st, e); break;
+    case 4: launch_clear_x0<4>(st, e); break;
+    case 8: launch_clear_x0<8>(st, e); break;
+    case 16: launch_clear_x0<16>(st, e); break;
+    case 32: launch_clear_x0<32>(st, e); break;
+    case 64: launch_clear_x0<64>(st, e); break;
+    default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex");
+  }
+  if (n_sources) {
+    init_sources_kernel<<<grid_for(n_sources, 256), 256, 0, st>>>(e->sources.get(), n_sources, e->inits.get(), e->W);
+    MG_LAUNCH_CHECK();
+  }
+}
+
+uint32_t words_for(uint32_t n_sources) {
+  uint32_t W = 1;
+  while (W * 64 < n_sources) W <<= 1;
+  return W;
+}
+
+void hop_stats_from(const unsigned long long* c, uint32_t rows, mg_hop_stats* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->tile_products = c[kTP];
+  out->left_tiles = c[kLeft];
+  out->right_tiles_used = c[kRightUsed];
+  out->out_tiles = c[kOut];
+  out->out_tiles_numeric = c[kOutNumeric];
+  out->popcount = c[kPop];
+  out->edges_visited = c[kEdges];
+  out->alg_bytes = 520ull * (c[kLeft] + c[kRightUsed] + c[kOut]) + 4ull * (2ull * rows + 2);
+}
+
+uint64_t words_mask(uint32_t w0, uint32_t n) {
+  if (n >= 64) return ~0ull;
+  return ((1ull << n) - 1) << w0;
 }
 
 }  // namespace
@@ -445,91 +816,43 @@ std::vector<Chunk> build_chunks(const std::vector<uint32_t>& off, uint32_t n_str
 
 using namespace mgamd;
 
+// ----------------------------------------------------------------------------
+// Single walks: one panel.
+// ----------------------------------------------------------------------------
 struct mg_walk {
   int device = 0;
-  const mg_tiled_matrix* at = nullptr;
-  uint32_t n_strips = 0;
-  uint32_t W = 64;
+  Engine e;
   uint32_t max_sources = 0;
-  uint32_t max_hops = 0;
   uint32_t flags = 0;
   uint32_t n_sources = 0;  // of the last run
   uint32_t hops = 0;       // of the last run
   bool timing = false;
-  std::vector<DevBuf<uint64_t>> x;    // hops+1 frontiers, vertex-major
-  std::vector<DevBuf<uint64_t>> s;    // structural tile-row masks per strip
-  std::vector<DevBuf<uint64_t>> nmask;  // numeric tile-row masks per strip
-  std::vector<DevBuf<uint64_t>> vmask;  // non-empty vertex masks per strip
-  DevBuf<Chunk> chunks;
-  uint32_t n_chunks = 0;
-  DevBuf<uint32_t> split_strips;
-  uint32_t n_split = 0;
-  DevBuf<unsigned long long> counters;  // (max_hops+1) * kNumCounters
-  DevBuf<uint32_t> sources;
-  PinnedBuf<uint32_t> staging;
+  PinnedBuf<SourceItem> staging;
   // Timing log: per timed run, 4 events per hop (symbolic start/end, numeric kernel
   // start/end), kept for every run since timing was enabled.
   std::vector<std::vector<hipEvent_t>> run_events;
   std::vector<hipEvent_t> event_pool;
   ~mg_walk() {
     for (auto& r : run_events)
-      for (auto e : r) (void)hipEventDestroy(e);
-    for (auto e : event_pool) (void)hipEventDestroy(e);
+      for (auto ev : r) (void)hipEventDestroy(ev);
+    for (auto ev : event_pool) (void)hipEventDestroy(ev);
   }
   hipEvent_t take_event() {
     if (event_pool.empty()) {
-      hipEvent_t e;
-      MG_HIP(hipEventCreate(&e));
-      return e;
+      hipEvent_t ev;
+      MG_HIP(hipEventCreate(&ev));
+      return ev;
     }
-    hipEvent_t e = event_pool.back();
+    hipEvent_t ev = event_pool.back();
     event_pool.pop_back();
-    return e;
+    return ev;
   }
   void reset_log() {
     for (auto& r : run_events)
-      for (auto e : r) event_pool.push_back(e);
+      for (auto ev : r) event_pool.push_back(ev);
     run_events.clear();
   }
 };
-
-namespace {
-
-template <int W>
-void launch_numeric_w(hipStream_t st, mg_walk* w, uint32_t k, hipEvent_t ev_begin, hipEvent_t ev_end) {
-  const mg_tiled_matrix* at = w->at;
-  unsigned long long* ctr = w->counters.get() + (k + 1) * kNumCounters;
-  if (w->n_split) {
-    zero_strips_kernel<W><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get());
-    MG_LAUNCH_CHECK();
-  }
-  if (ev_begin) MG_HIP(hipEventRecord(ev_begin, st));
-  walk_numeric_kernel<W><<<w->n_chunks, kBlock, 0, st>>>(
-      w->chunks.get(), at->offset_row.get(), at->col_idx.get(), at->data.get(), w->x[k].get(), w->vmask[k].get(),
-      w->x[k + 1].get(), w->nmask[k + 1].get(), w->vmask[k + 1].get(), ctr);
-  MG_LAUNCH_CHECK();
-  if (ev_end) MG_HIP(hipEventRecord(ev_end, st));
-  if (w->n_split) {
-    finalize_strips_kernel<W><<<w->n_split, kBlock, 0, st>>>(w->split_strips.get(), w->x[k + 1].get(),
-                                                            w->nmask[k + 1].get(), w->vmask[k + 1].get(), ctr);
-    MG_LAUNCH_CHECK();
-  }
-}
-
-void launch_numeric(mg_context* ctx, mg_walk* w, uint32_t k, hipEvent_t ev_begin, hipEvent_t ev_end) {
-  switch (w->W) {
-    case 1: launch_numeric_w<1>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 2: launch_numeric_w<2>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 4: launch_numeric_w<4>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 8: launch_numeric_w<8>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 16: launch_numeric_w<16>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 32: launch_numeric_w<32>(ctx->stream, w, k, ev_begin, ev_end); break;
-    case 64: launch_numeric_w<64>(ctx->stream, w, k, ev_begin, ev_end); break;
-    default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex " + std::to_string(w->W));
-  }
-}
-
-}  // namespace
 
 extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sources, uint32_t max_hops,
                               uint32_t flags, mg_walk** out) {
@@ -542,40 +865,21 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     DeviceGuard g(ctx->device);
     auto w = std::make_unique<mg_walk>();
     w->device = ctx->device;
-    w->at = at;
-    w->n_strips = at->n_strips;
-    // Words per vertex: the smallest power of two covering max_sources (64 sources per word).
-    w->W = 1;
-    while (w->W * 64 < max_sources) w->W <<= 1;
     w->max_sources = max_sources;
-    w->max_hops = max_hops;
     w->flags = flags;
-    const uint64_t n_vertices = static_cast<uint64_t>(at->n_strips) * 64;
-    w->x.resize(max_hops + 1);
-    w->s.resize(max_hops + 1);
-    w->nmask.resize(max_hops + 1);
-    w->vmask.resize(max_hops + 1);
-    for (uint32_t k = 0; k <= max_hops; ++k) {
-      w->x[k].alloc(n_vertices * w->W);
-      w->s[k].alloc(at->n_strips);
-      w->nmask[k].alloc(at->n_strips);
-      w->vmask[k].alloc(at->n_strips);
-    }
-    // Scheduler: chunks over the A^T tile stream, built once per graph.
-    std::vector<uint32_t> off(at->n_strips + 1ull);
-    MG_HIP(hipMemcpy(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
-    std::vector<uint32_t> split;
-    std::vector<Chunk> chunks = build_chunks(off, at->n_strips, &split);
-    w->n_chunks = static_cast<uint32_t>(chunks.size());
-    w->chunks.alloc(chunks.size());
-    MG_HIP(hipMemcpy(w->chunks.get(), chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice));
-    w->n_split = static_cast<uint32_t>(split.size());
-    if (!split.empty()) {
-      w->split_strips.alloc(split.size());
-      MG_HIP(hipMemcpy(w->split_strips.get(), split.data(), sizeof(uint32_t) * split.size(), hipMemcpyHostToDevice));
-    }
-    w->counters.alloc(static_cast<size_t>(max_hops + 1) * kNumCounters);
-    w->sources.alloc(max_sources);
+    Engine& e = w->e;
+    e.W = words_for(max_sources);
+    e.prune = (flags & MG_WALK_PRUNE) ? 1 : 0;
+    e.graphs.push_back(plan_graph(at, kDefaultChunkTiles));
+    Panel p;
+    p.graph = 0;
+    p.hops = max_hops;
+    p.n_strips = at->n_strips;
+    p.words_used = e.W;
+    p.alive.assign(max_hops, words_mask(0, e.W));
+    e.panels.push_back(std::move(p));
+    build_engine(&e);
+    e.sources.alloc(max_sources);
     w->staging.alloc(max_sources);
     *out = w.release();
   });
@@ -584,8 +888,9 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
 extern "C" int mg_walk_set_timing(mg_walk* w, int enable) {
   return guarded([&] {
     MG_REQUIRE(w != nullptr, "mg_walk_set_timing: NULL walk");
-    w->timing = enable != 0;
-    w->reset_log();
+    const bool on = enable != 0;
+    if (on && !w->timing) w->reset_log();
+    w->timing = on;
   });
 }
 
@@ -594,9 +899,9 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     MG_REQUIRE(ctx && w, "mg_walk_run: NULL argument");
     MG_REQUIRE(ctx->device == w->device, "mg_walk_run: walk belongs to another device");
     MG_REQUIRE(n_sources >= 1 && n_sources <= w->max_sources, "mg_walk_run: n_sources out of range");
-    MG_REQUIRE(hops >= 1 && hops <= w->max_hops, "mg_walk_run: hops out of range");
+    MG_REQUIRE(hops >= 1 && hops <= w->e.max_hops, "mg_walk_run: hops out of range");
     MG_REQUIRE(sources != nullptr, "mg_walk_run: sources is NULL");
-    const uint64_t n_vertices = static_cast<uint64_t>(w->n_strips) * 64;
+    const uint64_t n_vertices = static_cast<uint64_t>(w->e.panels[0].n_strips) * 64;
     {
       std::vector<uint32_t> sorted(sources, sources + n_sources);
       std::sort(sorted.begin(), sorted.end());
@@ -608,34 +913,16 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     hipStream_t st = ctx->stream;
     w->n_sources = n_sources;
     w->hops = hops;
-    std::memcpy(w->staging.get(), sources, sizeof(uint32_t) * n_sources);
-    MG_HIP(hipMemcpyAsync(w->sources.get(), w->staging.get(), sizeof(uint32_t) * n_sources, hipMemcpyHostToDevice, st));
-    const uint32_t ns = w->n_strips;
-    MG_HIP(hipMemsetAsync(w->x[0].get(), 0, w->x[0].bytes(), st));
-    MG_HIP(hipMemsetAsync(w->s[0].get(), 0, sizeof(uint64_t) * ns, st));
-    MG_HIP(hipMemsetAsync(w->vmask[0].get(), 0, sizeof(uint64_t) * ns, st));
-    MG_HIP(hipMemsetAsync(w->counters.get(), 0, w->counters.bytes(), st));
-    init_sources_kernel<<<grid_for(n_sources, 256), 256, 0, st>>>(
-        w->sources.get(), n_sources, w->W, reinterpret_cast<unsigned long long*>(w->x[0].get()),
-        reinterpret_cast<unsigned long long*>(w->s[0].get()), reinterpret_cast<unsigned long long*>(w->vmask[0].get()));
-    MG_LAUNCH_CHECK();
-    MG_HIP(hipMemcpyAsync(w->nmask[0].get(), w->s[0].get(), sizeof(uint64_t) * ns, hipMemcpyDeviceToDevice, st));
-    const bool prune = (w->flags & MG_WALK_PRUNE) != 0;
+    SourceItem* stg = w->staging.get();
+    for (uint32_t i = 0; i < n_sources; ++i) stg[i] = {0, i, sources[i]};
+    MG_HIP(hipMemcpyAsync(w->e.sources.get(), stg, sizeof(SourceItem) * n_sources, hipMemcpyHostToDevice, st));
+    launch_init(st, &w->e, n_sources);
     std::vector<hipEvent_t> ev;
     if (w->timing) {
       ev.resize(4 * static_cast<size_t>(hops));
-      for (auto& e : ev) e = w->take_event();
+      for (auto& x : ev) x = w->take_event();
     }
-    for (uint32_t k = 0; k < hops; ++k) {
-      if (w->timing) MG_HIP(hipEventRecord(ev[4 * k], st));
-      const uint64_t* s_in = prune ? w->nmask[k].get() : w->s[k].get();
-      walk_symbolic_kernel<<<grid_for(ns, kWaves, 16384), kBlock, 0, st>>>(
-          ns, w->at->offset_row.get(), w->at->col_idx.get(), s_in, w->s[k + 1].get(),
-          w->counters.get() + (k + 1) * kNumCounters);
-      MG_LAUNCH_CHECK();
-      if (w->timing) MG_HIP(hipEventRecord(ev[4 * k + 1], st));
-      launch_numeric(ctx, w, k, w->timing ? ev[4 * k + 2] : nullptr, w->timing ? ev[4 * k + 3] : nullptr);
-    }
+    for (uint32_t k = 0; k < hops; ++k) launch_level(st, &w->e, k, w->timing ? &ev[4 * k] : nullptr);
     if (w->timing) w->run_events.push_back(std::move(ev));
   });
 }
@@ -646,18 +933,9 @@ extern "C" int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_h
     MG_REQUIRE(hop >= 1 && hop <= w->hops, "mg_walk_hop_stats: hop out of range");
     DeviceGuard g(ctx->device);
     unsigned long long c[kNumCounters];
-    MG_HIP(hipMemcpyAsync(c, w->counters.get() + hop * kNumCounters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    MG_HIP(hipMemcpyAsync(c, w->e.ctr(0, hop), sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
-    std::memset(out, 0, sizeof(*out));
-    out->tile_products = c[kTP];
-    out->left_tiles = c[kLeft];
-    out->right_tiles_used = c[kRightUsed];
-    out->out_tiles = c[kOut];
-    out->out_tiles_numeric = c[kOutNumeric];
-    out->popcount = c[kPop];
-    out->edges_visited = c[kEdges];
-    const uint64_t rows = (w->n_sources + 63) / 64;
-    out->alg_bytes = 520ull * (c[kLeft] + c[kRightUsed] + c[kOut]) + 4ull * (2 * rows + 2);
+    hop_stats_from(c, (w->n_sources + 63) / 64, out);
     if (!w->run_events.empty() && w->run_events.back().size() >= 4ull * hop) {
       const auto& ev = w->run_events.back();
       float a = 0, b = 0;
@@ -696,7 +974,7 @@ extern "C" int mg_walk_kernel_times(mg_context* ctx, mg_walk* w, uint32_t hop, d
 extern "C" int mg_walk_words_per_vertex(const mg_walk* w, uint32_t* words) {
   return guarded([&] {
     MG_REQUIRE(w && words, "mg_walk_words_per_vertex: NULL argument");
-    *words = w->W;
+    *words = w->e.W;
   });
 }
 
@@ -705,26 +983,69 @@ extern "C" int mg_walk_frontier_download(mg_context* ctx, const mg_walk* w, uint
     MG_REQUIRE(ctx && w && xt, "mg_walk_frontier_download: NULL argument");
     MG_REQUIRE(hop <= w->hops, "mg_walk_frontier_download: hop out of range");
     DeviceGuard g(ctx->device);
-    MG_HIP(hipMemcpyAsync(xt, w->x[hop].get(), w->x[hop].bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    const auto& x = w->e.panels[0].x[hop];
+    MG_HIP(hipMemcpyAsync(xt, x.get(), x.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
+
+namespace {
+uint64_t frontier_digest(mg_context* ctx, const uint64_t* x, uint64_t nv, uint32_t W, uint32_t w0, uint32_t n_words) {
+  unsigned long long* d = ctx->counters.get();
+  MG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
+  if (n_words) {
+    frontier_digest_kernel<<<grid_for(nv * n_words, kBlock * 16, 16384), kBlock, 0, ctx->stream>>>(x, nv, W, w0,
+                                                                                                   n_words, d);
+    MG_LAUNCH_CHECK();
+  }
+  unsigned long long h = 0;
+  MG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  MG_HIP(hipStreamSynchronize(ctx->stream));
+  return h;
+}
+
+mg_tiled_matrix* export_frontier(mg_context* ctx, const uint64_t* x, const uint64_t* s_dev, uint32_t ns, uint32_t W,
+                                 uint32_t w0, uint32_t rows) {
+  std::vector<uint64_t> s(ns);
+  MG_HIP(hipMemcpyAsync(s.data(), s_dev, sizeof(uint64_t) * ns, hipMemcpyDeviceToHost, ctx->stream));
+  MG_HIP(hipStreamSynchronize(ctx->stream));
+  // Tile index in row-major (x, y) order over the padded-square grid.
+  std::vector<uint32_t> off(ns + 1ull, 0), ri, ci;
+  for (uint32_t xr = 0; xr < rows; ++xr) {
+    for (uint32_t y = 0; y < ns; ++y)
+      if ((s[y] >> (w0 + xr)) & 1) {
+        ri.push_back(xr);
+        ci.push_back(y);
+      }
+    off[xr + 1] = static_cast<uint32_t>(ri.size());
+  }
+  for (uint32_t xr = rows; xr < ns; ++xr) off[xr + 1] = off[xr];
+  auto m = std::make_unique<mg_tiled_matrix>();
+  m->device = ctx->device;
+  const uint32_t nt = static_cast<uint32_t>(ri.size());
+  alloc_tiled_matrix(m.get(), ns, nt);
+  MG_HIP(hipMemcpyAsync(m->offset_row.get(), off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  if (nt) {
+    MG_HIP(hipMemcpyAsync(m->row_idx.get(), ri.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
+    MG_HIP(hipMemcpyAsync(m->col_idx.get(), ci.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
+    export_tiles_kernel<<<grid_for(nt, kWaves, 65536), kBlock, 0, ctx->stream>>>(m->row_idx.get(), m->col_idx.get(),
+                                                                                 nt, x, W, w0, m->data.get());
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipStreamSynchronize(ctx->stream));
+  return m.release();
+}
+}  // namespace
 
 extern "C" int mg_walk_frontier_digest(mg_context* ctx, const mg_walk* w, uint32_t hop, uint64_t* digest) {
   return guarded([&] {
     MG_REQUIRE(ctx && w && digest, "mg_walk_frontier_digest: NULL argument");
     MG_REQUIRE(hop <= w->hops, "mg_walk_frontier_digest: hop out of range");
     DeviceGuard g(ctx->device);
-    unsigned long long* d = ctx->counters.get();
-    MG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
-    const uint64_t nv = static_cast<uint64_t>(w->n_strips) * 64;
-    frontier_digest_kernel<<<grid_for(nv * w->W, kBlock * 16, 16384), kBlock, 0, ctx->stream>>>(
-        w->x[hop].get(), nv, w->W, (w->n_sources + 63) / 64, d);
-    MG_LAUNCH_CHECK();
-    unsigned long long h = 0;
-    MG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    MG_HIP(hipStreamSynchronize(ctx->stream));
-    *digest = h;
+    const Panel& p = w->e.panels[0];
+    *digest = frontier_digest(ctx, p.x[hop].get(), static_cast<uint64_t>(p.n_strips) * 64, w->e.W, 0,
+                              (w->n_sources + 63) / 64);
   });
 }
 
@@ -734,37 +1055,8 @@ extern "C" int mg_walk_export(mg_context* ctx, const mg_walk* w, uint32_t hop, m
     *out = nullptr;
     MG_REQUIRE(hop <= w->hops, "mg_walk_export: hop out of range");
     DeviceGuard g(ctx->device);
-    const uint32_t ns = w->n_strips;
-    const uint32_t rows = (w->n_sources + 63) / 64;
-    std::vector<uint64_t> s(ns);
-    MG_HIP(hipMemcpyAsync(s.data(), w->s[hop].get(), sizeof(uint64_t) * ns, hipMemcpyDeviceToHost, ctx->stream));
-    MG_HIP(hipStreamSynchronize(ctx->stream));
-    // Tile index in row-major (x, y) order over the padded-square grid.
-    std::vector<uint32_t> off(ns + 1ull, 0), ri, ci;
-    for (uint32_t xr = 0; xr < rows; ++xr) {
-      for (uint32_t y = 0; y < ns; ++y)
-        if ((s[y] >> xr) & 1) {
-          ri.push_back(xr);
-          ci.push_back(y);
-        }
-      off[xr + 1] = static_cast<uint32_t>(ri.size());
-    }
-    for (uint32_t xr = rows; xr < ns; ++xr) off[xr + 1] = off[xr];
-    auto m = std::make_unique<mg_tiled_matrix>();
-    m->device = ctx->device;
-    const uint32_t nt = static_cast<uint32_t>(ri.size());
-    alloc_tiled_matrix(m.get(), ns, nt);
-    MG_HIP(hipMemcpyAsync(m->offset_row.get(), off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-    if (nt) {
-      MG_HIP(hipMemcpyAsync(m->row_idx.get(), ri.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
-      MG_HIP(hipMemcpyAsync(m->col_idx.get(), ci.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
-      export_tiles_kernel<<<grid_for(nt, kWaves, 65536), kBlock, 0, ctx->stream>>>(
-          m->row_idx.get(), m->col_idx.get(), nt, w->x[hop].get(), w->W, m->data.get());
-      MG_LAUNCH_CHECK();
-    }
-    MG_HIP(hipStreamSynchronize(ctx->stream));
-    *out = m.release();
+    const Panel& p = w->e.panels[0];
+    *out = export_frontier(ctx, p.x[hop].get(), p.s[hop].get(), p.n_strips, w->e.W, 0, (w->n_sources + 63) / 64);
   });
 }
 
@@ -773,5 +1065,266 @@ extern "C" int mg_walk_destroy(mg_walk* w) {
     if (!w) return;
     DeviceGuard g(w->device);
     delete w;
+  });
+}
+
+// ----------------------------------------------------------------------------
+// Batched walks: many queries over several graphs, packed into 4096-source panels,
+// all hop levels of all panels issued as one captured HIP graph.
+// ----------------------------------------------------------------------------
+struct QueryPlace {
+  uint32_t panel = 0, w0 = 0, n_words = 0, n_sources = 0, hops = 0, graph = 0;
+};
+
+struct mg_walk_batch {
+  int device = 0;
+  Engine e;
+  std::vector<QueryPlace> queries;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int captured_on = -1;
+  ~mg_walk_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+void record_batch(hipStream_t st, mg_walk_batch* b) {
+  launch_init(st, &b->e, b->e.n_sources);
+  for (uint32_t k = 0; k < b->e.max_hops; ++k) launch_level(st, &b->e, k, nullptr);
+}
+
+}  // namespace
+
+extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* const* graphs_at, uint32_t n_graphs,
+                                    const mg_walk_query* queries, uint32_t n_queries, const mg_batch_options* opt,
+                                    mg_walk_batch** out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && graphs_at && queries && out, "mg_walk_batch_create: NULL argument");
+    *out = nullptr;
+    MG_REQUIRE(n_graphs >= 1 && n_queries >= 1, "mg_walk_batch_create: empty batch");
+    const uint32_t chunk_tiles = (opt && opt->chunk_tiles) ? opt->chunk_tiles : kDefaultChunkTiles;
+    const uint32_t packing = opt ? opt->packing : MG_PACK_BY_HOPS;
+    MG_REQUIRE(packing <= MG_PACK_ONE_PER_PANEL, "mg_walk_batch_create: unknown packing");
+    DeviceGuard g(ctx->device);
+    auto b = std::make_unique<mg_walk_batch>();
+    b->device = ctx->device;
+    b->use_graph = !opt || opt->use_graph;
+    Engine& e = b->e;
+    e.W = 64;
+    for (uint32_t gi = 0; gi < n_graphs; ++gi) {
+      MG_REQUIRE(graphs_at[gi] != nullptr, "mg_walk_batch_create: NULL graph");
+      e.graphs.push_back(plan_graph(graphs_at[gi], chunk_tiles));
+    }
+    // Validate queries.
+    b->queries.resize(n_queries);
+    for (uint32_t q = 0; q < n_queries; ++q) {
+      const mg_walk_query& Q = queries[q];
+      MG_REQUIRE(Q.graph < n_graphs, "mg_walk_batch_create: query graph out of range");
+      MG_REQUIRE(Q.n_sources >= 1 && Q.n_sources <= 4096, "mg_walk_batch_create: query size must be in [1, 4096]");
+      MG_REQUIRE(Q.hops >= 1 && Q.hops <= 64, "mg_walk_batch_create: query hops must be in [1, 64]");
+      MG_REQUIRE(Q.sources != nullptr, "mg_walk_batch_create: query sources NULL");
+      const uint64_t nv = static_cast<uint64_t>(e.graphs[Q.graph].n_strips) * 64;
+      std::vector<uint32_t> sorted(Q.sources, Q.sources + Q.n_sources);
+      std::sort(sorted.begin(), sorted.end());
+      MG_REQUIRE(sorted.back() < nv, "mg_walk_batch_create: source vertex out of range");
+      MG_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(),
+                 "mg_walk_batch_create: sources of a query must be distinct");
+      b->queries[q].graph = Q.graph;
+      b->queries[q].n_sources = Q.n_sources;
+      b->queries[q].n_words = (Q.n_sources + 63) / 64;
+      b->queries[q].hops = Q.hops;
+    }
+    // Packing: per graph, queries in descending hop order (ties: larger first), first fit
+    // into 64-word panels; MG_PACK_ONE_PER_PANEL gives every query its own panel.
+    std::vector<uint32_t> order(n_queries);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+      const auto& A = b->queries[a];
+      const auto& B = b->queries[c];
+      if (A.graph != B.graph) return A.graph < B.graph;
+      if (A.hops != B.hops) return A.hops > B.hops;
+      return A.n_words > B.n_words;
+    });
+    std::vector<uint32_t> panel_fill;  // words used per panel
+    uint32_t first_panel_of_graph = 0;
+    uint32_t cur_graph = ~0u;
+    for (uint32_t q : order) {
+      QueryPlace& P = b->queries[q];
+      if (P.graph != cur_graph) {
+        cur_graph = P.graph;
+        first_panel_of_graph = static_cast<uint32_t>(e.panels.size());
+      }
+      uint32_t chosen = ~0u;
+      if (packing == MG_PACK_BY_HOPS) {
+        for (uint32_t pi = first_panel_of_graph; pi < e.panels.size(); ++pi)
+          if (panel_fill[pi] + P.n_words <= 64) {
+            chosen = pi;
+            break;
+          }
+      }
+      if (chosen == ~0u) {
+        Panel p;
+        p.graph = P.graph;
+        p.hops = 0;
+        p.n_strips = e.graphs[P.graph].n_strips;
+        e.panels.push_back(std::move(p));
+        panel_fill.push_back(0);
+        chosen = static_cast<uint32_t>(e.panels.size() - 1);
+      }
+      P.panel = chosen;
+      P.w0 = panel_fill[chosen];
+      panel_fill[chosen] += P.n_words;
+      Panel& pn = e.panels[chosen];
+      pn.hops = std::max(pn.hops, P.hops);
+    }
+    for (uint32_t pi = 0; pi < e.panels.size(); ++pi) {
+      e.panels[pi].words_used = panel_fill[pi];
+      e.panels[pi].alive.assign(e.panels[pi].hops, 0);
+    }
+    for (const QueryPlace& P : b->queries)
+      for (uint32_t k = 0; k < P.hops; ++k) e.panels[P.panel].alive[k] |= words_mask(P.w0, P.n_words);
+    build_engine(&e);
+    // Sources: all queries' bits, uploaded once.
+    std::vector<SourceItem> items;
+    for (uint32_t q = 0; q < n_queries; ++q) {
+      const QueryPlace& P = b->queries[q];
+      for (uint32_t i = 0; i < P.n_sources; ++i) items.push_back({P.panel, P.w0 * 64 + i, queries[q].sources[i]});
+    }
+    e.n_sources = static_cast<uint32_t>(items.size());
+    e.sources.alloc(items.size());
+    MG_HIP(hipMemcpy(e.sources.get(), items.data(), sizeof(SourceItem) * items.size(), hipMemcpyHostToDevice));
+    *out = b.release();
+  });
+}
+
+extern "C" int mg_walk_batch_run(mg_context* ctx, mg_walk_batch* b) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b, "mg_walk_batch_run: NULL argument");
+    MG_REQUIRE(ctx->device == b->device, "mg_walk_batch_run: batch belongs to another device");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (!b->use_graph) {
+      record_batch(st, b);
+      return;
+    }
+    if (!b->exec) {
+      MG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        record_batch(st, b);
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+      }
+      MG_HIP(hipStreamEndCapture(st, &b->graph));
+      MG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    MG_HIP(hipGraphLaunch(b->exec, st));
+  });
+}
+
+extern "C" int mg_walk_batch_info(const mg_walk_batch* b, uint32_t* n_panels, uint32_t* max_hops, uint64_t* n_chunks) {
+  return guarded([&] {
+    MG_REQUIRE(b != nullptr, "mg_walk_batch_info: NULL batch");
+    if (n_panels) *n_panels = static_cast<uint32_t>(b->e.panels.size());
+    if (max_hops) *max_hops = b->e.max_hops;
+    if (n_chunks) {
+      uint64_t c = 0;
+      for (uint32_t k = 0; k < b->e.max_hops; ++k) c += b->e.n_chunks[k];
+      *n_chunks = c;
+    }
+  });
+}
+
+extern "C" int mg_walk_batch_query_stats(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint32_t hop,
+                                         mg_hop_stats* out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b && out, "mg_walk_batch_query_stats: NULL argument");
+    MG_REQUIRE(q < b->queries.size(), "mg_walk_batch_query_stats: query out of range");
+    const QueryPlace& P = b->queries[q];
+    MG_REQUIRE(hop >= 1 && hop <= b->e.panels[P.panel].hops, "mg_walk_batch_query_stats: hop out of range");
+    DeviceGuard g(ctx->device);
+    unsigned long long c[kNumCounters];
+    MG_HIP(hipMemcpyAsync(c, const_cast<Engine&>(b->e).ctr(P.panel, hop), sizeof(c), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+    hop_stats_from(c, b->e.panels[P.panel].words_used, out);
+  });
+}
+
+extern "C" int mg_walk_batch_totals(mg_context* ctx, const mg_walk_batch* b, uint64_t* tile_products,
+                                    uint64_t* alg_bytes, uint64_t* edges_visited) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b, "mg_walk_batch_totals: NULL argument");
+    DeviceGuard g(ctx->device);
+    std::vector<unsigned long long> c(b->e.counters.size());
+    MG_HIP(hipMemcpyAsync(c.data(), b->e.counters.get(), b->e.counters.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t tp = 0, by = 0, ed = 0;
+    for (uint32_t pi = 0; pi < b->e.panels.size(); ++pi)
+      for (uint32_t k = 1; k <= b->e.panels[pi].hops; ++k) {
+        const unsigned long long* x = c.data() + (static_cast<size_t>(pi) * (b->e.max_hops + 1) + k) * kNumCounters;
+        mg_hop_stats s;
+        hop_stats_from(x, b->e.panels[pi].words_used, &s);
+        tp += s.tile_products;
+        by += s.alg_bytes;
+        ed += s.edges_visited;
+      }
+    if (tile_products) *tile_products = tp;
+    if (alg_bytes) *alg_bytes = by;
+    if (edges_visited) *edges_visited = ed;
+  });
+}
+
+extern "C" int mg_walk_batch_query_digest(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint64_t* digest) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b && digest, "mg_walk_batch_query_digest: NULL argument");
+    MG_REQUIRE(q < b->queries.size(), "mg_walk_batch_query_digest: query out of range");
+    DeviceGuard g(ctx->device);
+    const QueryPlace& P = b->queries[q];
+    const Panel& p = b->e.panels[P.panel];
+    *digest = frontier_digest(ctx, p.x[P.hops].get(), static_cast<uint64_t>(p.n_strips) * 64, b->e.W, P.w0, P.n_words);
+  });
+}
+
+extern "C" int mg_walk_batch_query_download(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint64_t* out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b && out, "mg_walk_batch_query_download: NULL argument");
+    MG_REQUIRE(q < b->queries.size(), "mg_walk_batch_query_download: query out of range");
+    DeviceGuard g(ctx->device);
+    const QueryPlace& P = b->queries[q];
+    const Panel& p = b->e.panels[P.panel];
+    const uint64_t nv = static_cast<uint64_t>(p.n_strips) * 64;
+    DevBuf<uint64_t> tmp(nv * P.n_words);
+    frontier_slice_kernel<<<grid_for(nv * P.n_words, kBlock * 8, 16384), kBlock, 0, ctx->stream>>>(
+        p.x[P.hops].get(), nv, b->e.W, P.w0, P.n_words, tmp.get());
+    MG_LAUNCH_CHECK();
+    MG_HIP(hipMemcpyAsync(out, tmp.get(), tmp.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int mg_walk_batch_query_export(mg_context* ctx, const mg_walk_batch* b, uint32_t q, mg_tiled_matrix** out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b && out, "mg_walk_batch_query_export: NULL argument");
+    *out = nullptr;
+    MG_REQUIRE(q < b->queries.size(), "mg_walk_batch_query_export: query out of range");
+    DeviceGuard g(ctx->device);
+    const QueryPlace& P = b->queries[q];
+    const Panel& p = b->e.panels[P.panel];
+    *out = export_frontier(ctx, p.x[P.hops].get(), p.s[P.hops].get(), p.n_strips, b->e.W, P.w0, P.n_words);
+  });
+}
+
+extern "C" int mg_walk_batch_destroy(mg_walk_batch* b) {
+  return guarded([&] {
+    if (!b) return;
+    DeviceGuard g(b->device);
+    delete b;
   });
 }

@@ -42,6 +42,9 @@ _SIG = {
     "or_walk_tile_hop": (u64, [u32, vp, vp, vp, u32, vp, vp, u32, u32, u32, i32]),
     "or_walk_symbolic": (u64, [u32, vp, vp, vp, vp]),
     "or_max_threads": (i32, []),
+    "or_mlp_loss_grad": (None, [u32, u32, vp, vp, vp, u32, vp, vp]),
+    "or_f32_to_bf16": (C.c_uint16, [C.c_float]),
+    "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
 }
 
 
@@ -254,3 +257,27 @@ def walk_symbolic(n_strips, at_off, at_col, s_in):
 
 def max_threads() -> int:
     return lib().or_max_threads()
+
+
+# --- MLP ------------------------------------------------------------------------
+def mlp_n_params(n_in, n_out):
+    return 64 * n_in + 64 + 64 * 64 + 64 + n_out * 64 + n_out
+
+
+def mlp_loss_grad(n_in, n_out, w, x, t):
+    """FP64 loss and gradient (oracle) for weights w (float64 array)."""
+    w = _c(w, np.float64)
+    x = _c(x, np.float32)
+    t = _c(t, np.float32)
+    grad = np.empty(mlp_n_params(n_in, n_out), np.float64)
+    loss = C.c_double()
+    lib().or_mlp_loss_grad(n_in, n_out, _p(w), _p(x), _p(t), x.shape[0], C.byref(loss), _p(grad))
+    return loss.value, grad
+
+
+def f32_to_bf16(f: float) -> int:
+    return lib().or_f32_to_bf16(f)
+
+
+def bf16_to_f32(h: int) -> float:
+    return lib().or_bf16_to_f32(h)

@@ -528,3 +528,78 @@ uint64_t or_walk_symbolic(uint32_t n_strips, const uint32_t* at_off, const uint3
   }
   return tp;
 }
+
+/* ------------------------------------------------------------------------- */
+/* MLP oracle: FP64 forward / backward of n_in -> 64 -> 64 -> n_out (ReLU) with  */
+/* the same parameter layout as the GPU model, on given (BF16-valued) weights.  */
+/* ------------------------------------------------------------------------- */
+
+uint16_t or_f32_to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
+  /* round to nearest, ties to even: add 0x7FFF plus the lowest kept bit */
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+float or_bf16_to_f32(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+void or_mlp_loss_grad(uint32_t n_in, uint32_t n_out, const double* w, const float* x, const float* t, uint32_t n,
+                      double* loss, double* grad) {
+  const uint32_t H = 64;
+  const uint32_t o_w1 = 0, o_b1 = H * n_in, o_w2 = o_b1 + H, o_b2 = o_w2 + H * H, o_w3 = o_b2 + H,
+                 o_b3 = o_w3 + n_out * H, np = o_b3 + n_out;
+  memset(grad, 0, sizeof(double) * np);
+  double L = 0;
+  const double scale = 1.0 / ((double)n * n_out);
+  double h1[64], h2[64], dh1[64], dh2[64], y[4], dy[4];
+  for (uint32_t s = 0; s < n; ++s) {
+    const float* xs = x + (size_t)s * n_in;
+    for (uint32_t j = 0; j < H; ++j) {
+      double a = w[o_b1 + j];
+      for (uint32_t i = 0; i < n_in; ++i) a += w[o_w1 + j * n_in + i] * xs[i];
+      h1[j] = a > 0 ? a : 0;
+    }
+    for (uint32_t j = 0; j < H; ++j) {
+      double a = w[o_b2 + j];
+      for (uint32_t i = 0; i < H; ++i) a += w[o_w2 + j * H + i] * h1[i];
+      h2[j] = a > 0 ? a : 0;
+    }
+    for (uint32_t o = 0; o < n_out; ++o) {
+      double a = w[o_b3 + o];
+      for (uint32_t i = 0; i < H; ++i) a += w[o_w3 + o * H + i] * h2[i];
+      y[o] = a;
+      const double e = a - t[(size_t)s * n_out + o];
+      L += e * e;
+      dy[o] = 2 * e * scale;
+    }
+    for (uint32_t i = 0; i < H; ++i) {
+      double a = 0;
+      for (uint32_t o = 0; o < n_out; ++o) a += w[o_w3 + o * H + i] * dy[o];
+      dh2[i] = h2[i] > 0 ? a : 0;
+    }
+    for (uint32_t i = 0; i < H; ++i) {
+      double a = 0;
+      for (uint32_t j = 0; j < H; ++j) a += w[o_w2 + j * H + i] * dh2[j];
+      dh1[i] = h1[i] > 0 ? a : 0;
+    }
+    for (uint32_t o = 0; o < n_out; ++o) {
+      grad[o_b3 + o] += dy[o];
+      for (uint32_t i = 0; i < H; ++i) grad[o_w3 + o * H + i] += dy[o] * h2[i];
+    }
+    for (uint32_t j = 0; j < H; ++j) {
+      grad[o_b2 + j] += dh2[j];
+      for (uint32_t i = 0; i < H; ++i) grad[o_w2 + j * H + i] += dh2[j] * h1[i];
+      grad[o_b1 + j] += dh1[j];
+      for (uint32_t i = 0; i < n_in; ++i) grad[o_w1 + j * n_in + i] += dh1[j] * xs[i];
+    }
+  }
+  (void)y;
+  *loss = L * scale;
+}

@@ -94,6 +94,15 @@ uint64_t or_walk_tile_hop(uint32_t n_strips, const uint32_t* at_off, const uint3
 uint64_t or_walk_symbolic(uint32_t n_strips, const uint32_t* at_off, const uint32_t* at_col, const uint64_t* s_in,
                           uint64_t* s_out);
 
+/* MLP oracle (n_in -> 64 -> 64 -> n_out, ReLU): FP64 mean-squared-error loss and its
+ * gradient for the given weights (layout W1[64][n_in], b1, W2[64][64], b2, W3[n_out][64],
+ * b3), and the BF16 round-to-nearest-even conversion defined from the IEEE bit layout.
+ * Pinned in tests by central finite differences and by bit-pattern cases. */
+void or_mlp_loss_grad(uint32_t n_in, uint32_t n_out, const double* w, const float* x, const float* t, uint32_t n,
+                      double* loss, double* grad);
+uint16_t or_f32_to_bf16(float f);
+float or_bf16_to_f32(uint16_t h);
+
 int or_max_threads(void);
 
 #ifdef __cplusplus
