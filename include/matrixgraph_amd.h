@@ -261,21 +261,22 @@ typedef struct mg_hop_stats {
   uint64_t popcount;          /* set bits of X_{k+1} */
   uint64_t alg_bytes;         /* 520*(T_L+T_R,used+T_C) + 4*(2*rows+2), rows = tile rows of X */
   uint64_t edges_visited;     /* set bits of A read by the numeric pass */
-  double symbolic_ms;         /* device time of the last logged run (timing enabled), else 0 */
-  double numeric_ms;          /* numeric kernel only (walk_numeric), same run */
+  double symbolic_ms;         /* 0: the structural pass runs inside the numeric kernel */
+  double numeric_ms;          /* hop kernel (structural + numeric) of the last logged run, else 0 */
 } mg_hop_stats;
 
 /* Allocates X_0..X_max_hops (W = ceil(max_sources/64) rounded up to a power of two,
- * max_sources <= 4096) plus the strip scheduler for at. at must outlive w. */
+ * max_sources <= 4096) plus the strip scheduler and the per-tile row-occupancy masks of
+ * at. at must outlive w and keep its contents (the masks are taken here). */
 int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sources,
                    uint32_t max_hops, uint32_t flags, mg_walk** out);
 /* Runs hops 1..hops from the given distinct sources (async; stats after sync). */
 int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, const uint32_t* sources, uint32_t hops);
-/* enable != 0 starts (and clears) a log of HIP events around the symbolic pass and the
- * numeric kernel of every hop of every later run; enable == 0 stops logging. */
+/* enable != 0 starts (and clears) a log of HIP events around the hop kernel of every hop
+ * of every later run; enable == 0 stops logging (the log stays readable). */
 int mg_walk_set_timing(mg_walk* w, int enable);
-/* Sums over the logged runs that reached `hop`: symbolic-pass and numeric-kernel device
- * milliseconds, and the number of such runs. Synchronises. */
+/* Sums over the logged runs that reached `hop`: symbolic-pass (0, fused) and hop-kernel
+ * device milliseconds, and the number of such runs. Synchronises. */
 int mg_walk_kernel_times(mg_context* ctx, mg_walk* w, uint32_t hop, double* symbolic_ms, double* numeric_ms,
                          uint32_t* n_runs);
 /* hop in [1, hops]; synchronises. */
@@ -316,7 +317,8 @@ typedef struct mg_batch_options {
   uint32_t reserved;
 } mg_batch_options;
 
-/* graphs_at[g] is graph g supplied transposed (A^T); they must outlive the batch.
+/* graphs_at[g] is graph g supplied transposed (A^T); they must outlive the batch and keep
+ * their contents (row-occupancy masks are taken here).
  * opt may be NULL (packing by hops, default chunks, HIP graph). */
 int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* const* graphs_at, uint32_t n_graphs,
                          const mg_walk_query* queries, uint32_t n_queries, const mg_batch_options* opt,

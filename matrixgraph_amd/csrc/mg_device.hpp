@@ -35,6 +35,17 @@ __host__ __device__ inline uint64_t digest_term(uint64_t index, uint64_t value) 
   return sm64_mix(index * 0xD1B54A32D192ED03ull ^ sm64_mix(value + 0x8CB92BA72F3D8DD7ull));
 }
 
+// Global-memory (address space 1) views of device pointers. A pointer read from memory (a
+// descriptor table, say) is generic to the compiler, which then emits flat instructions;
+// flat loads also count on the LDS counter, so every wait for a gather would wait for the
+// wave's LDS operations too. Loads and stores through these views are global_* ones.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ inline gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }

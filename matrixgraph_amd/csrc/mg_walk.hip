@@ -14,11 +14,13 @@
 // panel descriptors. Words of queries that have reached their hop count are masked out
 // of later gathers (alive masks), so each X_k keeps the finished queries' results.
 //
-// Per level k (X_k -> X_{k+1}):
-//   symbolic   S_{k+1}[y] = OR_{t in A^T.row(y)} (S_k[t] & alive) per strip, with
-//              tile-product / tile counts for B_alg;
-//   numeric    the gather/OR kernel, streaming A^T tiles in scheduler chunks;
-//   finalize   per-strip masks for strips split across chunks.
+// Per level k (X_k -> X_{k+1}) one kernel does both passes of the hop, tile by tile:
+//   structural S_{k+1}[y] = OR_{t in A^T.row(y)} (S_k[t] & alive) per strip, with the
+//              tile-product / tile counts of B_alg;
+//   numeric    the gather/OR of the set bits, streaming A^T tiles in scheduler chunks.
+// Only the non-zero rows of an A^T tile are read: a per-tile row-occupancy mask, built
+// once per graph, predicates the 64 row loads of the tile.
+// A finalize kernel completes the masks of strips split across chunks.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -33,19 +35,37 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
+// Numeric hop kernel: waves per workgroup (all share one strip accumulator in LDS) and the
+// minimum waves per SIMD the register allocation must allow.
+#ifndef MG_NUMERIC_WAVES
+#define MG_NUMERIC_WAVES 4
+#endif
+#ifndef MG_NUMERIC_MIN_WAVES_PER_SIMD
+#define MG_NUMERIC_MIN_WAVES_PER_SIMD 4
+#endif
+constexpr int kNumWaves = MG_NUMERIC_WAVES;
+constexpr int kNumBlock = kNumWaves * kWave;
 constexpr uint32_t kDefaultChunkTiles = 1024;  // scheduler target tiles per chunk
 constexpr uint32_t kChunkStrips = 32;          // at most this many strips per chunk
-constexpr uint32_t kSymStrips = 16;            // strips per symbolic work item (4 per wave)
-constexpr int kLoadsInFlight = 8;              // frontier gathers per wave flush
-constexpr int kTileBatch = 8;                  // A^T tile loads in flight per wave
+#ifndef MG_LOADS_IN_FLIGHT
+#define MG_LOADS_IN_FLIGHT 8
+#endif
+#ifndef MG_TILE_BATCH
+#define MG_TILE_BATCH 8
+#endif
+constexpr int kLoadsInFlight = MG_LOADS_IN_FLIGHT;  // frontier gathers per wave flush
+constexpr int kTileBatch = MG_TILE_BATCH;           // A^T tile loads in flight per wave
 
 enum Counter { kTP = 0, kLeft, kRightUsed, kOut, kOutNumeric, kPop, kEdges, kNumCounters };
+// Block-level partial counters of the numeric kernel, reduced in LDS before one atomic each.
+enum BlockCounter { bTP = 0, bUsed, bLeft, bOut, bEdges, bPop, bNumeric, kBlockCounters };
 
 // One panel at one hop level: the launch's inputs (X_k side) and outputs (X_{k+1} side).
 struct PanelLevel {
   const uint32_t* off;
   const uint32_t* col;
   const uint64_t* data;
+  const uint64_t* rmask;  // per A^T tile: bit r = row r non-zero
   const uint64_t* x_in;
   const uint64_t* v_in;
   const uint64_t* s_in;
@@ -67,9 +87,6 @@ struct Chunk {
 };
 struct StripItem {
   uint32_t panel, strip;
-};
-struct SymItem {
-  uint32_t panel, strip_begin, strip_end;
 };
 // Source s of a panel: bit `col` of the X_0 row of vertex `vertex`.
 struct SourceItem {
@@ -140,60 +157,14 @@ __global__ void init_sources_kernel(const SourceItem* __restrict__ items, uint32
   atomicOr(reinterpret_cast<unsigned long long*>(p.v0) + v / 64, 1ull << (v % 64));
 }
 
-// --- symbolic ---------------------------------------------------------------
-// One wave per strip. Accumulates T_L (structural tiles of X_k, counted once per strip),
-// tile products and T_R,used of the panel, restricted to alive tile rows. Each lane keeps
-// kSymUnroll column-index loads in flight before the dependent mask loads.
-constexpr int kSymUnroll = 4;
-
-__global__ void __launch_bounds__(kBlock) walk_symbolic_kernel(const SymItem* __restrict__ items,
-                                                               const PanelLevel* __restrict__ lv) {
-  const SymItem it = items[blockIdx.x];
-  const PanelLevel& L = lv[it.panel];
-  const uint64_t* s_in = L.s_in;  // structural masks, or numeric ones when pruning
+// Row-occupancy mask of every A^T tile: bit r set when word r (target row r) is non-zero.
+__global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                          uint64_t* __restrict__ mask) {
   const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x / kWave;
-  const uint64_t alive = L.alive_in;
-  __shared__ unsigned long long s_acc[4];
-  if (threadIdx.x < 4) s_acc[threadIdx.x] = 0;
-  __syncthreads();
-  uint64_t tp = 0, used = 0, left = 0, out = 0;
-  for (uint32_t y = it.strip_begin + wave; y < it.strip_end; y += kWaves) {
-    const uint32_t p0 = L.off[y], p1 = L.off[y + 1];
-    uint64_t s = 0;
-    for (uint32_t p = p0 + lane; p < p1; p += kWave * kSymUnroll) {
-      uint32_t c[kSymUnroll];
-#pragma unroll
-      for (int j = 0; j < kSymUnroll; ++j) c[j] = (p + j * kWave < p1) ? L.col[p + j * kWave] : 0xFFFFFFFFu;
-#pragma unroll
-      for (int j = 0; j < kSymUnroll; ++j) {
-        const uint64_t m = (c[j] != 0xFFFFFFFFu) ? (s_in[c[j]] & alive) : 0;
-        s |= m;
-        tp += __popcll(m);
-        used += (m != 0);
-      }
-    }
-    s = wave_or(s);
-    if (lane == 0) {
-      L.s_out[y] = s;
-      out += __popcll(s);
-      left += __popcll(s_in[y] & alive);
-    }
-  }
-  tp = wave_sum(tp);
-  used = wave_sum(used);
-  if (lane == 0) {
-    atomicAdd(&s_acc[0], static_cast<unsigned long long>(tp));
-    atomicAdd(&s_acc[1], static_cast<unsigned long long>(used));
-    atomicAdd(&s_acc[2], static_cast<unsigned long long>(left));
-    atomicAdd(&s_acc[3], static_cast<unsigned long long>(out));
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_acc[0]) atomicAdd(L.counters + kTP, s_acc[0]);
-    if (s_acc[1]) atomicAdd(L.counters + kRightUsed, s_acc[1]);
-    if (s_acc[2]) atomicAdd(L.counters + kLeft, s_acc[2]);
-    if (s_acc[3]) atomicAdd(L.counters + kOut, s_acc[3]);
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
+    const uint64_t m = ballot(__builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane) != 0);
+    if (lane == 0) mask[i] = m;
   }
 }
 
@@ -217,7 +188,7 @@ struct QueueShape {
 
 // keep = alive mask of this lane's word (all ones or zero).
 template <int W>
-__device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ x_in,
+__device__ inline void flush_queue(GatherQueue& q, gptr<const uint64_t> __restrict__ x_in,
                                    unsigned long long* __restrict__ acc, uint32_t lane, uint64_t keep) {
   using S = QueueShape<W>;
   uint64_t v[S::kLoads];
@@ -274,7 +245,7 @@ __device__ inline void flush_queue(GatherQueue& q, const uint64_t* __restrict__ 
 
 template <int W>
 __device__ inline void push_job(GatherQueue& q, uint32_t u, uint64_t rows, uint32_t lane,
-                                const uint64_t* __restrict__ x_in, unsigned long long* __restrict__ acc,
+                                gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
                                 uint64_t keep) {
   if (lane == q.len) {
     q.u = u;
@@ -287,8 +258,8 @@ __device__ inline void push_job(GatherQueue& q, uint32_t u, uint64_t rows, uint3
 // (restricted to non-empty sources vm), job (t*64+t', rows with bit t').
 template <int W>
 __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueue& q, uint32_t lane,
-                                 const uint64_t* __restrict__ x_in, unsigned long long* __restrict__ acc,
-                                 uint64_t keep, uint64_t& edges) {
+                                 gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
+                                 uint64_t keep, uint32_t& edges) {
   b &= vm;
   const uint64_t nz = ballot(b != 0);
   if (nz == 0) return;
@@ -313,128 +284,347 @@ __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueu
   }
 }
 
-// One workgroup per scheduler chunk; per strip a 64 x W-word LDS accumulator.
+// Block-wide sum of per-thread counters into LDS (one atomic per wave), then one global
+// atomic per non-zero counter.
+__device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters], unsigned long long* s_cnt,
+                                            gptr<unsigned long long> __restrict__ counters) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) {
+    const uint64_t v = wave_sum(c[i]);
+    if (lane == 0 && v) atomicAdd(&s_cnt[i], static_cast<unsigned long long>(v));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    constexpr int map[kBlockCounters] = {kTP, kRightUsed, kLeft, kOut, kEdges, kPop, kOutNumeric};
+#pragma unroll
+    for (int i = 0; i < kBlockCounters; ++i)
+      if (s_cnt[i]) atomicAdd((unsigned long long*)(counters + map[i]), s_cnt[i]);
+  }
+}
+
+// Strip write-out shared by the hop kernels. An unsplit strip reduces its N and V masks and
+// popcount, is stored (unless it is all zero and already zero in memory: its previous
+// numeric mask is 0) and records S, N, V and its tile counts; a piece of a split strip ORs
+// into memory that was zeroed before the launch.
+template <int W, int BLOCK>
+__device__ inline void write_strip(const unsigned long long* __restrict__ acc, unsigned long long& s_n,
+                                   unsigned long long& s_v, unsigned long long sym, uint32_t y, bool split,
+                                   gptr<uint64_t> x_out, gptr<uint64_t> n_out, gptr<uint64_t> v_out,
+                                   gptr<uint64_t> s_out, gptr<const uint64_t> s_in, uint64_t alive_in,
+                                   uint64_t alive_next, uint32_t (&cnt)[kBlockCounters]) {
+  constexpr int kWords = 64 * W;
+  const uint32_t lane = lane_id();
+  const gptr<uint64_t> dst = x_out + static_cast<uint64_t>(y) * kWords;
+  if (!split) {
+    uint64_t nmask = 0, vmask = 0, pc = 0;
+    for (int i = threadIdx.x; i < kWords; i += BLOCK) {
+      const uint64_t val = acc[i];
+      if (val) {
+        nmask |= 1ull << (i % W);
+        if ((alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
+        pc += __popcll(val);
+      }
+    }
+    nmask = wave_or(nmask);
+    vmask = wave_or(vmask);
+    cnt[bPop] += static_cast<uint32_t>(pc);
+    if (lane == 0) {
+      if (nmask) atomicOr(&s_n, static_cast<unsigned long long>(nmask));
+      if (vmask) atomicOr(&s_v, static_cast<unsigned long long>(vmask));
+    }
+    __syncthreads();
+    const bool store = (s_n != 0) || (n_out[y] != 0);
+    if (store)
+      for (int i = threadIdx.x; i < kWords; i += BLOCK) dst[i] = acc[i];
+    __syncthreads();  // every thread has read n_out[y] before it changes
+    if (threadIdx.x == 0) {
+      n_out[y] = s_n;
+      v_out[y] = s_v;
+      s_out[y] = sym;
+      cnt[bNumeric] += __popcll(s_n);
+      cnt[bOut] += __popcll(sym);
+      cnt[bLeft] += __popcll(s_in[y] & alive_in);
+    }
+  } else {
+    for (int i = threadIdx.x; i < kWords; i += BLOCK) {
+      const uint64_t val = acc[i];
+      if (val) atomicOr((unsigned long long*)(dst + i), static_cast<unsigned long long>(val));
+    }
+    if (threadIdx.x == 0 && sym) atomicOr((unsigned long long*)(s_out + y), sym);
+  }
+}
+
+// One workgroup per scheduler chunk; per strip a 64 x W-word LDS accumulator. The same
+// sweep over the strip's tiles computes the structural output mask (OR of the source
+// strips' structural masks) and the tile-product counts.
 template <int W>
-__global__ void __launch_bounds__(kBlock) walk_numeric_kernel(const Chunk* __restrict__ chunks,
+__global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk_numeric_kernel(const Chunk* __restrict__ chunks,
                                                               const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;  // accumulator words per strip
   __shared__ unsigned long long acc[kWords];
-  __shared__ unsigned long long s_n, s_v, s_pop;
+  __shared__ unsigned long long s_n, s_v, s_sym;
+  __shared__ unsigned long long s_cnt[kBlockCounters];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   const Chunk ch = chunks[blockIdx.x];
   const PanelLevel& L = lv[ch.panel];
-  const uint32_t* __restrict__ off = L.off;
-  const uint32_t* __restrict__ col = L.col;
-  const uint64_t* __restrict__ data = L.data;
-  const uint64_t* __restrict__ x_in = L.x_in;
-  const uint64_t* __restrict__ v_in = L.v_in;
-  uint64_t* __restrict__ x_out = L.x_out;
-  const uint64_t keep = ((L.alive_in >> ((W == 64) ? lane : (lane % W))) & 1) ? ~0ull : 0ull;
+  const gptr<const uint32_t> __restrict__ off = as_global(L.off);
+  const gptr<const uint32_t> __restrict__ col = as_global(L.col);
+  const gptr<const uint64_t> __restrict__ data = as_global(L.data);
+  const gptr<const uint64_t> __restrict__ rmask = as_global(L.rmask);
+  const gptr<const uint64_t> __restrict__ x_in = as_global(L.x_in);
+  const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
+  const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
+  const gptr<uint64_t> __restrict__ x_out = as_global(L.x_out);
+  const gptr<uint64_t> __restrict__ n_out = as_global(L.n_out);
+  const gptr<uint64_t> __restrict__ v_out = as_global(L.v_out);
+  const gptr<uint64_t> __restrict__ s_out = as_global(L.s_out);
+  const uint64_t alive_in = L.alive_in;
+  const uint64_t keep = ((alive_in >> ((W == 64) ? lane : (lane % W))) & 1) ? ~0ull : 0ull;
   const uint64_t alive_next = L.alive_next;
-  uint64_t edges = 0, pop_total = 0, numeric_tiles = 0;
+  uint32_t cnt[kBlockCounters];  // per-thread partial counts (each fits 32 bits per chunk)
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
 
   for (uint32_t y = ch.strip_begin; y < ch.strip_end; ++y) {
     const uint32_t r0 = off[y], r1 = off[y + 1];
     const uint32_t seg0 = r0 > ch.tile_begin ? r0 : ch.tile_begin;
     const uint32_t seg1 = r1 < ch.tile_end ? r1 : ch.tile_end;
     const bool split = (seg0 != r0) || (seg1 != r1);
-    for (int i = threadIdx.x; i < kWords; i += kBlock) acc[i] = 0;
-    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_pop = 0; }
+    for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
+    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_sym = 0; }
     __syncthreads();
 
     GatherQueue q{0, 0, 0};
-    // Wave `wave` scans tiles p = seg0 + wave + kWaves*(64*g + lane): one coalesced load of
-    // 64 tile columns and one gather of their source-strip masks per 64 tiles; the active
-    // tiles then go in groups of kTileBatch whose 512-B loads are in flight together.
-    for (uint32_t g0 = seg0 + wave; g0 < seg1; g0 += kWaves * 64) {
-      const uint32_t p = g0 + kWaves * lane;
+    uint64_t sym = 0;
+    // Wave `wave` scans tiles p = seg0 + wave + kNumWaves*(64*g + lane): one coalesced load of
+    // 64 tile columns and row masks and one gather of their source-strip masks per 64
+    // tiles; the active tiles then go in groups of kTileBatch whose row loads are in
+    // flight together.
+    for (uint32_t g0 = seg0 + wave; g0 < seg1; g0 += kNumWaves * 64) {
+      const uint32_t p = g0 + kNumWaves * lane;
       const bool valid = p < seg1;
       const uint32_t t_l = valid ? col[p] : 0;
+      const uint64_t rm_l = valid ? rmask[p] : 0;
       const uint64_t vm_l = valid ? v_in[t_l] : 0;
-      uint64_t act = ballot(vm_l != 0);
+      const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0;
+      sym |= sm_l;
+      cnt[bTP] += __popcll(sm_l);
+      cnt[bUsed] += (sm_l != 0);
+      uint64_t act = ballot(vm_l != 0 && rm_l != 0);
       while (act) {
         int ks[kTileBatch];
-        int cnt = 0;
+        int cnt_b = 0;
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
           ks[j] = 0;
           if (act) {
             ks[j] = __ffsll(static_cast<unsigned long long>(act)) - 1;
             act &= act - 1;
-            cnt = j + 1;
+            cnt_b = j + 1;
           }
         }
         // A^T tiles are streamed once per hop: non-temporal loads keep them from
-        // evicting the frontier words that the gathers re-read.
+        // evicting the frontier words that the gathers re-read. Only rows set in the
+        // tile's row mask are read.
         uint64_t b[kTileBatch];
 #pragma unroll
-        for (int j = 0; j < kTileBatch; ++j)
-          b[j] = (j < cnt) ? __builtin_nontemporal_load(
-                                 data + static_cast<uint64_t>(g0 + kWaves * static_cast<uint32_t>(ks[j])) * kTile + lane)
-                           : 0;
+        for (int j = 0; j < kTileBatch; ++j) {
+          b[j] = 0;
+          if (j < cnt_b) {
+            const uint64_t rm = read_lane(rm_l, ks[j]);
+            if ((rm >> lane) & 1)
+              b[j] = __builtin_nontemporal_load(data + static_cast<uint64_t>(g0 + kNumWaves * static_cast<uint32_t>(ks[j])) *
+                                                           kTile + lane);
+          }
+        }
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
-          if (j < cnt) {
+          if (j < cnt_b) {
             const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), ks[j]));
             const uint64_t vm = read_lane(vm_l, ks[j]);
-            tile_jobs<W>(t, b[j], vm, q, lane, x_in, acc, keep, edges);
+            tile_jobs<W>(t, b[j], vm, q, lane, x_in, acc, keep, cnt[bEdges]);
           }
         }
       }
     }
     if (q.len) flush_queue<W>(q, x_in, acc, lane, keep);
+    sym = wave_or(sym);
+    if (lane == 0 && sym) atomicOr(&s_sym, static_cast<unsigned long long>(sym));
     __syncthreads();
 
-    // Write the strip out. Unsplit strips also reduce their masks here; a strip that is
-    // all zero and already zero in memory (its previous numeric mask is 0) is not stored.
-    uint64_t* dst = x_out + static_cast<uint64_t>(y) * kWords;
-    if (!split) {
-      uint64_t nmask = 0, vmask = 0, pc = 0;
-      for (int i = threadIdx.x; i < kWords; i += kBlock) {
-        const uint64_t val = acc[i];
-        if (val) {
-          nmask |= 1ull << (i % W);
-          if ((alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
-          pc += __popcll(val);
-        }
-      }
-      nmask = wave_or(nmask);
-      vmask = wave_or(vmask);
-      pc = wave_sum(pc);
-      if (lane == 0) {
-        if (nmask) atomicOr(&s_n, static_cast<unsigned long long>(nmask));
-        if (vmask) atomicOr(&s_v, static_cast<unsigned long long>(vmask));
-        if (pc) atomicAdd(&s_pop, static_cast<unsigned long long>(pc));
-      }
-      __syncthreads();
-      const bool store = (s_n != 0) || (L.n_out[y] != 0);
-      if (store)
-        for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = acc[i];
-      __syncthreads();  // every thread has read n_out[y] before it changes
-      if (threadIdx.x == 0) {
-        L.n_out[y] = s_n;
-        L.v_out[y] = s_v;
-        pop_total += s_pop;
-        numeric_tiles += __popcll(s_n);
-      }
-    } else {
-      for (int i = threadIdx.x; i < kWords; i += kBlock) {
-        const uint64_t val = acc[i];
-        if (val) atomicOr(reinterpret_cast<unsigned long long*>(dst + i), static_cast<unsigned long long>(val));
-      }
-    }
+    write_strip<W, kNumBlock>(acc, s_n, s_v, s_sym, y, split, x_out, n_out, v_out, s_out, s_in, alive_in,
+                              alive_next, cnt);
     __syncthreads();
   }
-  edges = wave_sum(edges);
-  if (lane == 0 && edges) atomicAdd(L.counters + kEdges, static_cast<unsigned long long>(edges));
-  if (threadIdx.x == 0) {
-    if (pop_total) atomicAdd(L.counters + kPop, static_cast<unsigned long long>(pop_total));
-    if (numeric_tiles) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(numeric_tiles));
+  flush_block_counters(cnt, s_cnt, as_global(L.counters));
+}
+
+// --- numeric, 64 words per vertex ---------------------------------------------
+// For the full 4096-source panel the hop kernel splits tile decoding from gathering. Lane i
+// of a wave decodes tile i of a group of 64: it loads only the non-zero rows of its tile
+// (at most kSparseRows of them; denser tiles take the wave-wide path above) and appends one
+// job (source vertex, target row) per set bit to the wave's job list in LDS. Draining the
+// list, each half-wave gathers one 512-B source row with 16-B loads and ORs it into the
+// target row, kDrainUnroll job pairs in flight per wave.
+constexpr int kJobCap = 240;     // jobs per wave list
+constexpr int kSparseRows = 4;   // row loads per lane when decoding
+constexpr int kDrainUnroll = 8;  // job pairs per drain step
+
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Job: source vertex in the low 32 bits, target row in bits 32..37.
+__device__ inline void drain_jobs(const unsigned long long* __restrict__ jobs, uint32_t n,
+                                  gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
+                                  uint32_t lane, uint64_t keep0, uint64_t keep1) {
+  const uint32_t half = lane >> 5;
+  const uint32_t w2 = (lane & 31) * 2;  // this lane's two words of a 64-word row
+  for (uint32_t j = 0; j < n; j += 2 * kDrainUnroll) {
+    unsigned long long e[kDrainUnroll];
+#pragma unroll
+    for (int k = 0; k < kDrainUnroll; ++k) {
+      const uint32_t idx = j + 2 * k + half;
+      e[k] = idx < n ? jobs[idx] : ~0ull;
+    }
+    u64x2_t v[kDrainUnroll];
+#pragma unroll
+    for (int k = 0; k < kDrainUnroll; ++k) {
+      v[k] = u64x2_t{0, 0};
+      if (e[k] != ~0ull) {
+        const uint32_t u = static_cast<uint32_t>(e[k]);
+        v[k] = *reinterpret_cast<gptr<const u64x2_t>>(x_in + static_cast<uint64_t>(u) * 64 + w2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kDrainUnroll; ++k) {
+      if (e[k] != ~0ull) {
+        unsigned long long* a = acc + static_cast<uint32_t>(e[k] >> 32) * 64 + w2;
+        atomicOr(a, v[k].x & keep0);
+        atomicOr(a + 1, v[k].y & keep1);
+      }
+    }
   }
 }
 
-// Zero the strips that are written by several chunks (atomic OR targets).
+__global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
+    walk_numeric64_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv) {
+  constexpr int W = 64;
+  constexpr int kWords = 64 * W;
+  __shared__ unsigned long long acc[kWords];
+  __shared__ unsigned long long jobs_all[kNumWaves][kJobCap];
+  __shared__ unsigned long long s_n, s_v, s_sym;
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  unsigned long long* const jobs = jobs_all[wave];
+  const Chunk ch = chunks[blockIdx.x];
+  const PanelLevel& L = lv[ch.panel];
+  const gptr<const uint32_t> __restrict__ off = as_global(L.off);
+  const gptr<const uint32_t> __restrict__ col = as_global(L.col);
+  const gptr<const uint64_t> __restrict__ data = as_global(L.data);
+  const gptr<const uint64_t> __restrict__ rmask = as_global(L.rmask);
+  const gptr<const uint64_t> __restrict__ x_in = as_global(L.x_in);
+  const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
+  const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
+  const uint64_t alive_in = L.alive_in;
+  const uint64_t alive_next = L.alive_next;
+  const uint64_t keep = ((alive_in >> lane) & 1) ? ~0ull : 0ull;
+  const uint64_t keep0 = ((alive_in >> ((lane & 31) * 2)) & 1) ? ~0ull : 0ull;
+  const uint64_t keep1 = ((alive_in >> ((lane & 31) * 2 + 1)) & 1) ? ~0ull : 0ull;
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+
+  for (uint32_t y = ch.strip_begin; y < ch.strip_end; ++y) {
+    const uint32_t r0 = off[y], r1 = off[y + 1];
+    const uint32_t seg0 = r0 > ch.tile_begin ? r0 : ch.tile_begin;
+    const uint32_t seg1 = r1 < ch.tile_end ? r1 : ch.tile_end;
+    const bool split = (seg0 != r0) || (seg1 != r1);
+    for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
+    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_sym = 0; }
+    __syncthreads();
+
+    GatherQueue q{0, 0, 0};
+    uint32_t len = 0;  // jobs in this wave's list (wave-uniform)
+    uint64_t sym = 0;
+    for (uint32_t g0 = seg0 + 64 * wave; g0 < seg1; g0 += 64 * kNumWaves) {
+      const uint32_t p = g0 + lane;
+      const bool valid = p < seg1;
+      const uint32_t t_l = valid ? col[p] : 0;
+      const uint64_t rm_l = valid ? rmask[p] : 0;
+      const uint64_t vm_l = valid ? v_in[t_l] : 0;
+      const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0;
+      sym |= sm_l;
+      cnt[bTP] += __popcll(sm_l);
+      cnt[bUsed] += (sm_l != 0);
+      const bool act = vm_l != 0 && rm_l != 0;
+      const bool dense = act && __popcll(rm_l) > kSparseRows;
+      // Sparse tiles: this lane loads the non-zero rows of its own tile.
+      uint64_t m = (act && !dense) ? rm_l : 0;
+      uint64_t w[kSparseRows];
+      uint32_t rr[kSparseRows];
+#pragma unroll
+      for (int i = 0; i < kSparseRows; ++i) {
+        rr[i] = m ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) : 0u;
+        w[i] = m ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(p) * kTile + rr[i]) & vm_l) : 0ull;
+        m &= m - 1;
+      }
+#pragma unroll
+      for (int i = 0; i < kSparseRows; ++i) {
+        uint64_t wi = w[i];
+        cnt[bEdges] += __popcll(wi);
+        const uint64_t tag = static_cast<uint64_t>(t_l) * 64 | (static_cast<uint64_t>(rr[i]) << 32);
+        for (uint64_t has = ballot(wi != 0); has; has = ballot(wi != 0)) {
+          if (len + 64 > kJobCap) {
+            wave_lds_sync();
+            drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
+            wave_lds_sync();
+            len = 0;
+          }
+          if (wi) {
+            jobs[len + lanes_below(has)] = tag + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wi)) - 1);
+            wi &= wi - 1;
+          }
+          len += __popcll(has);
+        }
+      }
+      // Dense tiles: the whole wave loads a tile and gathers each source column once.
+      for (uint64_t dm = ballot(dense); dm; dm &= dm - 1) {
+        const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+        const uint32_t pk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), k));
+        const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
+        const uint64_t rm = read_lane(rm_l, k);
+        const uint64_t b = ((rm >> lane) & 1) ? __builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) : 0ull;
+        tile_jobs<W>(t, b, read_lane(vm_l, k), q, lane, x_in, acc, keep, cnt[bEdges]);
+      }
+    }
+    if (q.len) flush_queue<W>(q, x_in, acc, lane, keep);
+    if (len) {
+      wave_lds_sync();
+      drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
+    }
+    sym = wave_or(sym);
+    if (lane == 0 && sym) atomicOr(&s_sym, static_cast<unsigned long long>(sym));
+    __syncthreads();
+    write_strip<W, kNumBlock>(acc, s_n, s_v, s_sym, y, split, as_global(L.x_out), as_global(L.n_out),
+                              as_global(L.v_out), as_global(L.s_out), s_in, alive_in, alive_next, cnt);
+    __syncthreads();
+  }
+  flush_block_counters(cnt, s_cnt, as_global(L.counters));
+}
+
+// Zero the strips that are written by several chunks (atomic OR targets) and their
+// structural masks.
 template <int W>
 __global__ void __launch_bounds__(kBlock) zero_strips_kernel(const StripItem* __restrict__ items,
                                                              const PanelLevel* __restrict__ lv) {
@@ -442,9 +632,10 @@ __global__ void __launch_bounds__(kBlock) zero_strips_kernel(const StripItem* __
   const StripItem it = items[blockIdx.x];
   uint64_t* dst = lv[it.panel].x_out + static_cast<uint64_t>(it.strip) * kWords;
   for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = 0;
+  if (threadIdx.x == 0) lv[it.panel].s_out[it.strip] = 0;
 }
 
-// Masks and popcount of the split strips once every piece has been OR-ed in.
+// Masks, popcount and tile counts of the split strips once every piece has been OR-ed in.
 template <int W>
 __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem* __restrict__ items,
                                                                  const PanelLevel* __restrict__ lv) {
@@ -478,6 +669,10 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
     L.v_out[it.strip] = s_v;
     if (s_pop) atomicAdd(L.counters + kPop, s_pop);
     if (s_n) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
+    const uint64_t so = L.s_out[it.strip];
+    if (so) atomicAdd(L.counters + kOut, static_cast<unsigned long long>(__popcll(so)));
+    const uint64_t left = L.s_in[it.strip] & L.alive_in;
+    if (left) atomicAdd(L.counters + kLeft, static_cast<unsigned long long>(__popcll(left)));
   }
 }
 
@@ -563,6 +758,7 @@ struct GraphPlan {
   uint32_t n_strips = 0;
   std::vector<Chunk> chunks;
   std::vector<uint32_t> split;
+  DevBuf<uint64_t> rmask;  // row-occupancy mask per A^T tile, taken when the plan is made
 };
 
 GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
@@ -572,6 +768,12 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
   std::vector<uint32_t> off(at->n_strips + 1ull);
   MG_HIP(hipMemcpy(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
   g.chunks = build_chunks(off, at->n_strips, chunk_tiles, &g.split);
+  g.rmask.alloc(std::max<uint32_t>(1, at->n_nz_tile));
+  if (at->n_nz_tile) {
+    row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile, g.rmask.get());
+    MG_LAUNCH_CHECK();
+    MG_HIP(hipDeviceSynchronize());
+  }
   return g;
 }
 
@@ -600,8 +802,6 @@ struct Engine {
   std::vector<uint32_t> n_chunks;
   std::vector<DevBuf<StripItem>> split;
   std::vector<uint32_t> n_split;
-  std::vector<DevBuf<SymItem>> sym;
-  std::vector<uint32_t> n_sym;
   // X_0 setup
   DevBuf<PanelInit> inits;
   DevBuf<uint32_t> panel_strips;
@@ -648,13 +848,10 @@ void build_engine(Engine* e) {
   e->n_chunks.assign(e->max_hops, 0);
   e->split.resize(e->max_hops);
   e->n_split.assign(e->max_hops, 0);
-  e->sym.resize(e->max_hops);
-  e->n_sym.assign(e->max_hops, 0);
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
     std::vector<Chunk> ch;
     std::vector<StripItem> sp;
-    std::vector<SymItem> sy;
     for (uint32_t pi = 0; pi < P; ++pi) {
       Panel& p = e->panels[pi];
       if (k >= p.hops) continue;
@@ -663,6 +860,7 @@ void build_engine(Engine* e) {
       L.off = g.at->offset_row.get();
       L.col = g.at->col_idx.get();
       L.data = g.at->data.get();
+      L.rmask = g.rmask.get();
       L.x_in = p.x[k].get();
       L.v_in = p.v[k].get();
       L.s_in = e->prune ? p.n[k].get() : p.s[k].get();
@@ -679,7 +877,6 @@ void build_engine(Engine* e) {
         ch.push_back(c);
       }
       for (uint32_t y : g.split) sp.push_back({pi, y});
-      for (uint32_t y0 = 0; y0 < p.n_strips; y0 += kSymStrips) sy.push_back({pi, y0, std::min(p.n_strips, y0 + kSymStrips)});
     }
     // Heaviest chunks first so the tail of the launch is made of light ones. Among equal
     // sizes, the same chunk of every panel of one graph is adjacent, so the panels' reads of
@@ -701,9 +898,6 @@ void build_engine(Engine* e) {
     e->n_split[k] = static_cast<uint32_t>(sp.size());
     e->split[k].alloc(std::max<size_t>(1, sp.size()));
     if (!sp.empty()) MG_HIP(hipMemcpy(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice));
-    e->n_sym[k] = static_cast<uint32_t>(sy.size());
-    e->sym[k].alloc(std::max<size_t>(1, sy.size()));
-    if (!sy.empty()) MG_HIP(hipMemcpy(e->sym[k].get(), sy.data(), sizeof(SymItem) * sy.size(), hipMemcpyHostToDevice));
   }
   // X_0 fills: frontier and masks of every panel, plus all counters.
   std::vector<PanelInit> inits(P);
@@ -726,11 +920,8 @@ void build_engine(Engine* e) {
 
 template <int W>
 void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
+  // The structural pass runs inside the numeric kernel: its event pair brackets nothing.
   if (ev) MG_HIP(hipEventRecord(ev[0], st));
-  if (e->n_sym[k]) {
-    walk_symbolic_kernel<<<e->n_sym[k], kBlock, 0, st>>>(e->sym[k].get(), e->lv[k].get());
-    MG_LAUNCH_CHECK();
-  }
   if (ev) MG_HIP(hipEventRecord(ev[1], st));
   if (e->n_split[k]) {
     zero_strips_kernel<W><<<e->n_split[k], kBlock, 0, st>>>(e->split[k].get(), e->lv[k].get());
@@ -738,7 +929,10 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   }
   if (ev) MG_HIP(hipEventRecord(ev[2], st));
   if (e->n_chunks[k]) {
-    walk_numeric_kernel<W><<<e->n_chunks[k], kBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+    if constexpr (W == 64)
+      walk_numeric64_kernel<<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+    else
+      walk_numeric_kernel<W><<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
