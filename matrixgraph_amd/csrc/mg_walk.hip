@@ -65,13 +65,15 @@ struct PanelLevel {
   const uint32_t* off;
   const uint32_t* col;
   const uint64_t* data;
-  const uint64_t* rmask;  // per A^T tile: bit r = row r non-zero
+  const uint64_t* rmask;    // per A^T tile: bit r = row r non-zero
+  const uint64_t* payload;  // per A^T tile: compact bit list (see tile_payload_kernel)
   const uint64_t* x_in;
   const uint64_t* v_in;
   const uint64_t* s_in;
   uint64_t* x_out;
   uint64_t* n_out;
   uint64_t* v_out;
+  uint64_t* z_out;  // per strip: rows of X_{k+1} that may be non-zero in memory
   uint64_t* s_out;
   unsigned long long* counters;
   uint64_t alive_in;    // words of X_k that propagate into X_{k+1}
@@ -165,6 +167,37 @@ __global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __rest
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
     const uint64_t m = ballot(__builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane) != 0);
     if (lane == 0) mask[i] = m;
+  }
+}
+
+// Compact payload of an A^T tile with at most kPayloadMax set bits: entry i = (row << 6) |
+// column in bits 12i..12i+11 and the entry count in bits 48..50. A tile with more bits has
+// kPayloadDense set and is read row by row through its row-occupancy mask.
+constexpr uint32_t kPayloadMax = 4;
+constexpr uint64_t kPayloadDense = 1ull << 63;
+
+__global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                              uint64_t* __restrict__ payload) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
+    const uint64_t word = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
+    const uint64_t total = wave_sum(__popcll(word));
+    uint64_t pl = kPayloadDense;
+    if (total <= kPayloadMax) {
+      pl = 0;
+      uint32_t n = 0;
+      for (uint64_t m = ballot(word != 0); m; m &= m - 1) {
+        const int r = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        for (uint64_t wr = read_lane(word, r); wr; wr &= wr - 1) {
+          const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wr)) - 1);
+          pl |= ((static_cast<uint64_t>(r) << 6) | c) << (12 * n);
+          ++n;
+        }
+      }
+      pl |= static_cast<uint64_t>(n) << 48;
+    }
+    if (lane == 0) payload[i] = pl;
   }
 }
 
@@ -303,44 +336,55 @@ __device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters],
   }
 }
 
-// Strip write-out shared by the hop kernels. An unsplit strip reduces its N and V masks and
-// popcount, is stored (unless it is all zero and already zero in memory: its previous
-// numeric mask is 0) and records S, N, V and its tile counts; a piece of a split strip ORs
-// into memory that was zeroed before the launch.
+// Strip write-out shared by the hop kernels. An unsplit strip reduces its masks (N: non-zero
+// words, V: non-empty vertices that propagate, Z: non-zero rows) and popcount, records S and
+// its tile counts, and is stored with non-temporal stores. Memory can only be non-zero at
+// the rows x words of the previous Z and N masks, so a zero word outside them is not
+// written. A piece of a split strip ORs into memory that was zeroed before the launch.
 template <int W, int BLOCK>
 __device__ inline void write_strip(const unsigned long long* __restrict__ acc, unsigned long long& s_n,
-                                   unsigned long long& s_v, unsigned long long sym, uint32_t y, bool split,
-                                   gptr<uint64_t> x_out, gptr<uint64_t> n_out, gptr<uint64_t> v_out,
-                                   gptr<uint64_t> s_out, gptr<const uint64_t> s_in, uint64_t alive_in,
-                                   uint64_t alive_next, uint32_t (&cnt)[kBlockCounters]) {
+                                   unsigned long long& s_v, unsigned long long& s_z, unsigned long long sym,
+                                   uint32_t y, bool split, gptr<uint64_t> x_out, gptr<uint64_t> n_out,
+                                   gptr<uint64_t> v_out, gptr<uint64_t> z_out, gptr<uint64_t> s_out,
+                                   gptr<const uint64_t> s_in, uint64_t alive_in, uint64_t alive_next,
+                                   uint32_t (&cnt)[kBlockCounters]) {
   constexpr int kWords = 64 * W;
   const uint32_t lane = lane_id();
   const gptr<uint64_t> dst = x_out + static_cast<uint64_t>(y) * kWords;
   if (!split) {
-    uint64_t nmask = 0, vmask = 0, pc = 0;
+    uint64_t nmask = 0, vmask = 0, zmask = 0, pc = 0;
     for (int i = threadIdx.x; i < kWords; i += BLOCK) {
       const uint64_t val = acc[i];
       if (val) {
         nmask |= 1ull << (i % W);
+        zmask |= 1ull << (i / W);
         if ((alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
         pc += __popcll(val);
       }
     }
     nmask = wave_or(nmask);
     vmask = wave_or(vmask);
+    zmask = wave_or(zmask);
     cnt[bPop] += static_cast<uint32_t>(pc);
     if (lane == 0) {
       if (nmask) atomicOr(&s_n, static_cast<unsigned long long>(nmask));
       if (vmask) atomicOr(&s_v, static_cast<unsigned long long>(vmask));
+      if (zmask) atomicOr(&s_z, static_cast<unsigned long long>(zmask));
     }
     __syncthreads();
-    const bool store = (s_n != 0) || (n_out[y] != 0);
-    if (store)
-      for (int i = threadIdx.x; i < kWords; i += BLOCK) dst[i] = acc[i];
-    __syncthreads();  // every thread has read n_out[y] before it changes
+    const uint64_t z_prev = z_out[y], n_prev = n_out[y];
+    if (s_z != 0 || z_prev != 0) {
+      for (int i = threadIdx.x; i < kWords; i += BLOCK) {
+        const uint64_t val = acc[i];
+        if (val != 0 || (((z_prev >> (i / W)) & 1) && ((n_prev >> (i % W)) & 1)))
+          __builtin_nontemporal_store(val, dst + i);
+      }
+    }
+    __syncthreads();  // every thread has read z_out[y] and n_out[y] before they change
     if (threadIdx.x == 0) {
       n_out[y] = s_n;
       v_out[y] = s_v;
+      z_out[y] = s_z;
       s_out[y] = sym;
       cnt[bNumeric] += __popcll(s_n);
       cnt[bOut] += __popcll(sym);
@@ -363,7 +407,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
                                                               const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;  // accumulator words per strip
   __shared__ unsigned long long acc[kWords];
-  __shared__ unsigned long long s_n, s_v, s_sym;
+  __shared__ unsigned long long s_n, s_v, s_z, s_sym;
   __shared__ unsigned long long s_cnt[kBlockCounters];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
@@ -395,7 +439,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
     const uint32_t seg1 = r1 < ch.tile_end ? r1 : ch.tile_end;
     const bool split = (seg0 != r0) || (seg1 != r1);
     for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
-    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_sym = 0; }
+    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
     __syncthreads();
 
     GatherQueue q{0, 0, 0};
@@ -456,8 +500,8 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
     if (lane == 0 && sym) atomicOr(&s_sym, static_cast<unsigned long long>(sym));
     __syncthreads();
 
-    write_strip<W, kNumBlock>(acc, s_n, s_v, s_sym, y, split, x_out, n_out, v_out, s_out, s_in, alive_in,
-                              alive_next, cnt);
+    write_strip<W, kNumBlock>(acc, s_n, s_v, s_z, s_sym, y, split, x_out, n_out, v_out, as_global(L.z_out), s_out,
+                              s_in, alive_in, alive_next, cnt);
     __syncthreads();
   }
   flush_block_counters(cnt, s_cnt, as_global(L.counters));
@@ -465,13 +509,13 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
 
 // --- numeric, 64 words per vertex ---------------------------------------------
 // For the full 4096-source panel the hop kernel splits tile decoding from gathering. Lane i
-// of a wave decodes tile i of a group of 64: it loads only the non-zero rows of its tile
-// (at most kSparseRows of them; denser tiles take the wave-wide path above) and appends one
-// job (source vertex, target row) per set bit to the wave's job list in LDS. Draining the
-// list, each half-wave gathers one 512-B source row with 16-B loads and ORs it into the
-// target row, kDrainUnroll job pairs in flight per wave.
+// of a wave decodes tile i of a group of 64 from its compact payload (tiles with at most
+// kPayloadMax bits) and appends one job (source vertex, target row) per set bit to the
+// wave's job list in LDS; a denser tile is loaded by the whole wave (lane = row, only the
+// rows in its row mask) and emits its jobs the same way. Draining the list, each half-wave
+// gathers one 512-B source row with 16-B loads and ORs it into the target row, kDrainUnroll
+// job pairs in flight per wave.
 constexpr int kJobCap = 240;     // jobs per wave list
-constexpr int kSparseRows = 4;   // row loads per lane when decoding
 constexpr int kDrainUnroll = 8;  // job pairs per drain step
 
 typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
@@ -514,13 +558,33 @@ __device__ inline void drain_jobs(const unsigned long long* __restrict__ jobs, u
   }
 }
 
+// Appends the job (tag + lowest set bit of `bits`) of every lane whose `bits` is non-zero,
+// clearing that bit, until no lane has bits left; drains the list whenever it could overflow.
+__device__ inline void emit_jobs(uint64_t bits, uint64_t tag, unsigned long long* __restrict__ jobs, uint32_t& len,
+                                 gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
+                                 uint32_t lane, uint64_t keep0, uint64_t keep1) {
+  for (uint64_t has = ballot(bits != 0); has; has = ballot(bits != 0)) {
+    if (len + 64 > kJobCap) {
+      wave_lds_sync();
+      drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
+      wave_lds_sync();
+      len = 0;
+    }
+    if (bits) {
+      jobs[len + lanes_below(has)] = tag + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1);
+      bits &= bits - 1;
+    }
+    len += __popcll(has);
+  }
+}
+
 __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
     walk_numeric64_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv) {
   constexpr int W = 64;
   constexpr int kWords = 64 * W;
   __shared__ unsigned long long acc[kWords];
   __shared__ unsigned long long jobs_all[kNumWaves][kJobCap];
-  __shared__ unsigned long long s_n, s_v, s_sym;
+  __shared__ unsigned long long s_n, s_v, s_z, s_sym;
   __shared__ unsigned long long s_cnt[kBlockCounters];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
@@ -531,12 +595,12 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
   const gptr<const uint32_t> __restrict__ col = as_global(L.col);
   const gptr<const uint64_t> __restrict__ data = as_global(L.data);
   const gptr<const uint64_t> __restrict__ rmask = as_global(L.rmask);
+  const gptr<const uint64_t> __restrict__ payload = as_global(L.payload);
   const gptr<const uint64_t> __restrict__ x_in = as_global(L.x_in);
   const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
   const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
   const uint64_t alive_in = L.alive_in;
   const uint64_t alive_next = L.alive_next;
-  const uint64_t keep = ((alive_in >> lane) & 1) ? ~0ull : 0ull;
   const uint64_t keep0 = ((alive_in >> ((lane & 31) * 2)) & 1) ? ~0ull : 0ull;
   const uint64_t keep1 = ((alive_in >> ((lane & 31) * 2 + 1)) & 1) ? ~0ull : 0ull;
   uint32_t cnt[kBlockCounters];
@@ -551,64 +615,59 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
     const uint32_t seg1 = r1 < ch.tile_end ? r1 : ch.tile_end;
     const bool split = (seg0 != r0) || (seg1 != r1);
     for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
-    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_sym = 0; }
+    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
     __syncthreads();
 
-    GatherQueue q{0, 0, 0};
     uint32_t len = 0;  // jobs in this wave's list (wave-uniform)
     uint64_t sym = 0;
     for (uint32_t g0 = seg0 + 64 * wave; g0 < seg1; g0 += 64 * kNumWaves) {
       const uint32_t p = g0 + lane;
       const bool valid = p < seg1;
-      const uint32_t t_l = valid ? col[p] : 0;
-      const uint64_t rm_l = valid ? rmask[p] : 0;
-      const uint64_t vm_l = valid ? v_in[t_l] : 0;
-      const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0;
+      // The tile index and payload streams are read once per hop.
+      const uint32_t t_l = valid ? __builtin_nontemporal_load(col + p) : 0u;
+      const uint64_t pl = valid ? __builtin_nontemporal_load(payload + p) : 0ull;
+      const uint64_t vm_l = valid ? v_in[t_l] : 0ull;
+      const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0ull;
       sym |= sm_l;
       cnt[bTP] += __popcll(sm_l);
       cnt[bUsed] += (sm_l != 0);
-      const bool act = vm_l != 0 && rm_l != 0;
-      const bool dense = act && __popcll(rm_l) > kSparseRows;
-      // Sparse tiles: this lane loads the non-zero rows of its own tile.
-      uint64_t m = (act && !dense) ? rm_l : 0;
-      uint64_t w[kSparseRows];
-      uint32_t rr[kSparseRows];
+      const bool act = vm_l != 0 && pl != 0;
+      const bool dense = act && (pl & kPayloadDense) != 0;
+      // Sparse tiles: entries of this lane's payload whose source vertex is non-empty.
+      const uint64_t ent = (act && !dense) ? pl : 0ull;
+      const uint32_t n_ent = static_cast<uint32_t>(ent >> 48) & 7u;
 #pragma unroll
-      for (int i = 0; i < kSparseRows; ++i) {
-        rr[i] = m ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) : 0u;
-        w[i] = m ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(p) * kTile + rr[i]) & vm_l) : 0ull;
-        m &= m - 1;
-      }
-#pragma unroll
-      for (int i = 0; i < kSparseRows; ++i) {
-        uint64_t wi = w[i];
-        cnt[bEdges] += __popcll(wi);
-        const uint64_t tag = static_cast<uint64_t>(t_l) * 64 | (static_cast<uint64_t>(rr[i]) << 32);
-        for (uint64_t has = ballot(wi != 0); has; has = ballot(wi != 0)) {
+      for (uint32_t i = 0; i < kPayloadMax; ++i) {
+        const uint32_t e = static_cast<uint32_t>(ent >> (12 * i)) & 0xFFFu;
+        const bool ok = i < n_ent && ((vm_l >> (e & 63u)) & 1);
+        cnt[bEdges] += ok;
+        const uint64_t tag = (static_cast<uint64_t>(e >> 6) << 32) | (static_cast<uint64_t>(t_l) * 64 + (e & 63u));
+        const uint64_t has = ballot(ok);
+        if (has) {
           if (len + 64 > kJobCap) {
             wave_lds_sync();
             drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
             wave_lds_sync();
             len = 0;
           }
-          if (wi) {
-            jobs[len + lanes_below(has)] = tag + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wi)) - 1);
-            wi &= wi - 1;
-          }
+          if (ok) jobs[len + lanes_below(has)] = tag;
           len += __popcll(has);
         }
       }
-      // Dense tiles: the whole wave loads a tile and gathers each source column once.
+      // Dense tiles: the whole wave loads one (lane = row) and emits its jobs.
       for (uint64_t dm = ballot(dense); dm; dm &= dm - 1) {
         const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
         const uint32_t pk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), k));
         const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
-        const uint64_t rm = read_lane(rm_l, k);
-        const uint64_t b = ((rm >> lane) & 1) ? __builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) : 0ull;
-        tile_jobs<W>(t, b, read_lane(vm_l, k), q, lane, x_in, acc, keep, cnt[bEdges]);
+        const uint64_t vm = read_lane(vm_l, k);
+        const uint64_t rm = rmask[pk];
+        const uint64_t bits =
+            ((rm >> lane) & 1) ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) & vm) : 0ull;
+        cnt[bEdges] += __popcll(bits);
+        emit_jobs(bits, (static_cast<uint64_t>(lane) << 32) | (static_cast<uint64_t>(t) * 64), jobs, len, x_in, acc,
+                  lane, keep0, keep1);
       }
     }
-    if (q.len) flush_queue<W>(q, x_in, acc, lane, keep);
     if (len) {
       wave_lds_sync();
       drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
@@ -616,8 +675,9 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
     sym = wave_or(sym);
     if (lane == 0 && sym) atomicOr(&s_sym, static_cast<unsigned long long>(sym));
     __syncthreads();
-    write_strip<W, kNumBlock>(acc, s_n, s_v, s_sym, y, split, as_global(L.x_out), as_global(L.n_out),
-                              as_global(L.v_out), as_global(L.s_out), s_in, alive_in, alive_next, cnt);
+    write_strip<W, kNumBlock>(acc, s_n, s_v, s_z, s_sym, y, split, as_global(L.x_out), as_global(L.n_out),
+                              as_global(L.v_out), as_global(L.z_out), as_global(L.s_out), s_in, alive_in, alive_next,
+                              cnt);
     __syncthreads();
   }
   flush_block_counters(cnt, s_cnt, as_global(L.counters));
@@ -640,33 +700,37 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem* __restrict__ items,
                                                                  const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * W;
-  __shared__ unsigned long long s_n, s_v, s_pop;
-  if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_pop = 0; }
+  __shared__ unsigned long long s_n, s_v, s_z, s_pop;
+  if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_pop = 0; }
   __syncthreads();
   const StripItem it = items[blockIdx.x];
   const PanelLevel& L = lv[it.panel];
   const uint64_t* src = L.x_out + static_cast<uint64_t>(it.strip) * kWords;
-  uint64_t nmask = 0, vmask = 0, pc = 0;
+  uint64_t nmask = 0, vmask = 0, zmask = 0, pc = 0;
   for (int i = threadIdx.x; i < kWords; i += kBlock) {
     const uint64_t val = src[i];
     if (val) {
       nmask |= 1ull << (i % W);
+      zmask |= 1ull << (i / W);
       if ((L.alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
       pc += __popcll(val);
     }
   }
   nmask = wave_or(nmask);
   vmask = wave_or(vmask);
+  zmask = wave_or(zmask);
   pc = wave_sum(pc);
   if (lane_id() == 0) {
     if (nmask) atomicOr(&s_n, static_cast<unsigned long long>(nmask));
     if (vmask) atomicOr(&s_v, static_cast<unsigned long long>(vmask));
+    if (zmask) atomicOr(&s_z, static_cast<unsigned long long>(zmask));
     if (pc) atomicAdd(&s_pop, static_cast<unsigned long long>(pc));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     L.n_out[it.strip] = s_n;
     L.v_out[it.strip] = s_v;
+    L.z_out[it.strip] = s_z;
     if (s_pop) atomicAdd(L.counters + kPop, s_pop);
     if (s_n) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
     const uint64_t so = L.s_out[it.strip];
@@ -758,7 +822,8 @@ struct GraphPlan {
   uint32_t n_strips = 0;
   std::vector<Chunk> chunks;
   std::vector<uint32_t> split;
-  DevBuf<uint64_t> rmask;  // row-occupancy mask per A^T tile, taken when the plan is made
+  DevBuf<uint64_t> rmask;    // row-occupancy mask per A^T tile, taken when the plan is made
+  DevBuf<uint64_t> payload;  // compact bit list per A^T tile (tile_payload_kernel)
 };
 
 GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
@@ -769,8 +834,12 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
   MG_HIP(hipMemcpy(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
   g.chunks = build_chunks(off, at->n_strips, chunk_tiles, &g.split);
   g.rmask.alloc(std::max<uint32_t>(1, at->n_nz_tile));
+  g.payload.alloc(std::max<uint32_t>(1, at->n_nz_tile));
   if (at->n_nz_tile) {
     row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile, g.rmask.get());
+    MG_LAUNCH_CHECK();
+    tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile,
+                                                                            g.payload.get());
     MG_LAUNCH_CHECK();
     MG_HIP(hipDeviceSynchronize());
   }
@@ -786,7 +855,7 @@ struct Panel {
   uint32_t n_strips = 0;
   uint64_t words_used = 0;  // words of X holding sources
   std::vector<uint64_t> alive;  // alive[k]: words that propagate from X_k to X_{k+1}
-  std::vector<DevBuf<uint64_t>> x, s, n, v;
+  std::vector<DevBuf<uint64_t>> x, s, n, v, z;
 };
 
 // Device launch plan for a set of panels over levels 0..max_hops-1.
@@ -829,17 +898,20 @@ void build_engine(Engine* e) {
     p.s.resize(p.hops + 1);
     p.n.resize(p.hops + 1);
     p.v.resize(p.hops + 1);
+    p.z.resize(p.hops + 1);
     for (uint32_t k = 0; k <= p.hops; ++k) {
       p.x[k].alloc(nv * e->W);
       p.s[k].alloc(p.n_strips);
       p.n[k].alloc(p.n_strips);
       p.v[k].alloc(p.n_strips);
-      // All strips start zero and known to be zero (numeric masks 0); later runs rely on
-      // that to skip stores of strips that stay zero.
+      p.z[k].alloc(p.n_strips);
+      // All strips start zero and known to be zero (Z and N masks 0); later runs rely on
+      // that to skip stores of words that stay zero.
       MG_HIP(hipMemset(p.x[k].get(), 0, p.x[k].bytes()));
       MG_HIP(hipMemset(p.n[k].get(), 0, p.n[k].bytes()));
       MG_HIP(hipMemset(p.s[k].get(), 0, p.s[k].bytes()));
       MG_HIP(hipMemset(p.v[k].get(), 0, p.v[k].bytes()));
+      MG_HIP(hipMemset(p.z[k].get(), 0, p.z[k].bytes()));
     }
   }
   e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kNumCounters);
@@ -861,12 +933,14 @@ void build_engine(Engine* e) {
       L.col = g.at->col_idx.get();
       L.data = g.at->data.get();
       L.rmask = g.rmask.get();
+      L.payload = g.payload.get();
       L.x_in = p.x[k].get();
       L.v_in = p.v[k].get();
       L.s_in = e->prune ? p.n[k].get() : p.s[k].get();
       L.x_out = p.x[k + 1].get();
       L.n_out = p.n[k + 1].get();
       L.v_out = p.v[k + 1].get();
+      L.z_out = p.z[k + 1].get();
       L.s_out = p.s[k + 1].get();
       L.counters = e->ctr(pi, k + 1);
       L.alive_in = p.alive[k];
