@@ -312,7 +312,7 @@ enum {
 };
 typedef struct mg_batch_options {
   uint32_t packing;     /* MG_PACK_* */
-  uint32_t chunk_tiles; /* scheduler chunk size in A^T tiles (0 = default 1024) */
+  uint32_t chunk_tiles; /* scheduler chunk size in A^T tiles (0 = default 2048) */
   uint32_t use_graph;   /* non-zero: capture once, replay as a HIP graph */
   uint32_t reserved;
 } mg_batch_options;

@@ -45,8 +45,15 @@ constexpr int kWaves = kBlock / kWave;
 #endif
 constexpr int kNumWaves = MG_NUMERIC_WAVES;
 constexpr int kNumBlock = kNumWaves * kWave;
-constexpr uint32_t kDefaultChunkTiles = 1024;  // scheduler target tiles per chunk
-constexpr uint32_t kChunkStrips = 32;          // at most this many strips per chunk
+#ifndef MG_DEFAULT_CHUNK_TILES
+#define MG_DEFAULT_CHUNK_TILES 2048
+#endif
+constexpr uint32_t kDefaultChunkTiles = MG_DEFAULT_CHUNK_TILES;  // scheduler target tiles per chunk
+#ifndef MG_CHUNK_STRIPS
+#define MG_CHUNK_STRIPS 32
+#endif
+constexpr uint32_t kChunkStrips = MG_CHUNK_STRIPS;  // at most this many strips per chunk (< 64)
+static_assert(kChunkStrips < 64, "one lane per strip of a chunk, plus the end pointer");
 #ifndef MG_LOADS_IN_FLIGHT
 #define MG_LOADS_IN_FLIGHT 8
 #endif
@@ -57,6 +64,11 @@ constexpr int kLoadsInFlight = MG_LOADS_IN_FLIGHT;  // frontier gathers per wave
 constexpr int kTileBatch = MG_TILE_BATCH;           // A^T tile loads in flight per wave
 
 enum Counter { kTP = 0, kLeft, kRightUsed, kOut, kOutNumeric, kPop, kEdges, kNumCounters };
+// Each (panel, level) keeps kCounterSlots copies of its counters; workgroup b adds into slot
+// b % kCounterSlots, so the atomics of a launch do not all queue on the same addresses.
+// Readers sum the slots.
+constexpr uint32_t kCounterSlots = 64;
+constexpr uint32_t kCounterStride = kCounterSlots * kNumCounters;  // per (panel, level)
 // Block-level partial counters of the numeric kernel, reduced in LDS before one atomic each.
 enum BlockCounter { bTP = 0, bUsed, bLeft, bOut, bEdges, bPop, bNumeric, kBlockCounters };
 
@@ -330,9 +342,10 @@ __device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters],
   __syncthreads();
   if (threadIdx.x == 0) {
     constexpr int map[kBlockCounters] = {kTP, kRightUsed, kLeft, kOut, kEdges, kPop, kOutNumeric};
+    const gptr<unsigned long long> slot = counters + (blockIdx.x % kCounterSlots) * kNumCounters;
 #pragma unroll
     for (int i = 0; i < kBlockCounters; ++i)
-      if (s_cnt[i]) atomicAdd((unsigned long long*)(counters + map[i]), s_cnt[i]);
+      if (s_cnt[i]) atomicAdd((unsigned long long*)(slot + map[i]), s_cnt[i]);
   }
 }
 
@@ -341,12 +354,13 @@ __device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters],
 // its tile counts, and is stored with non-temporal stores. Memory can only be non-zero at
 // the rows x words of the previous Z and N masks, so a zero word outside them is not
 // written. A piece of a split strip ORs into memory that was zeroed before the launch.
+// z_prev, n_prev: the strip's Z and N masks before this launch; left: S_k[y] & alive.
 template <int W, int BLOCK>
 __device__ inline void write_strip(const unsigned long long* __restrict__ acc, unsigned long long& s_n,
                                    unsigned long long& s_v, unsigned long long& s_z, unsigned long long sym,
                                    uint32_t y, bool split, gptr<uint64_t> x_out, gptr<uint64_t> n_out,
                                    gptr<uint64_t> v_out, gptr<uint64_t> z_out, gptr<uint64_t> s_out,
-                                   gptr<const uint64_t> s_in, uint64_t alive_in, uint64_t alive_next,
+                                   uint64_t z_prev, uint64_t n_prev, uint64_t left, uint64_t alive_next,
                                    uint32_t (&cnt)[kBlockCounters]) {
   constexpr int kWords = 64 * W;
   const uint32_t lane = lane_id();
@@ -372,7 +386,6 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
       if (zmask) atomicOr(&s_z, static_cast<unsigned long long>(zmask));
     }
     __syncthreads();
-    const uint64_t z_prev = z_out[y], n_prev = n_out[y];
     if (s_z != 0 || z_prev != 0) {
       for (int i = threadIdx.x; i < kWords; i += BLOCK) {
         const uint64_t val = acc[i];
@@ -380,7 +393,7 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
           __builtin_nontemporal_store(val, dst + i);
       }
     }
-    __syncthreads();  // every thread has read z_out[y] and n_out[y] before they change
+    __syncthreads();  // every thread has read s_n and s_z
     if (threadIdx.x == 0) {
       n_out[y] = s_n;
       v_out[y] = s_v;
@@ -388,7 +401,7 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
       s_out[y] = sym;
       cnt[bNumeric] += __popcll(s_n);
       cnt[bOut] += __popcll(sym);
-      cnt[bLeft] += __popcll(s_in[y] & alive_in);
+      cnt[bLeft] += __popcll(left);
     }
   } else {
     for (int i = threadIdx.x; i < kWords; i += BLOCK) {
@@ -501,7 +514,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
     __syncthreads();
 
     write_strip<W, kNumBlock>(acc, s_n, s_v, s_z, s_sym, y, split, x_out, n_out, v_out, as_global(L.z_out), s_out,
-                              s_in, alive_in, alive_next, cnt);
+                              as_global(L.z_out)[y], n_out[y], s_in[y] & alive_in, alive_next, cnt);
     __syncthreads();
   }
   flush_block_counters(cnt, s_cnt, as_global(L.counters));
@@ -513,68 +526,168 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
 // kPayloadMax bits) and appends one job (source vertex, target row) per set bit to the
 // wave's job list in LDS; a denser tile is loaded by the whole wave (lane = row, only the
 // rows in its row mask) and emits its jobs the same way. Draining the list, each half-wave
-// gathers one 512-B source row with 16-B loads and ORs it into the target row, kDrainUnroll
-// job pairs in flight per wave.
-constexpr int kJobCap = 240;     // jobs per wave list
-constexpr int kDrainUnroll = 8;  // job pairs per drain step
-
-typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+// gathers one 512-B source row (two 8-B loads per lane) and ORs it into the target row.
+//
+// A job is stored pre-scaled: source vertex * 512 (the byte offset of its frontier row) plus
+// target row * 8, so the drain needs no index arithmetic beyond two masks. The list is
+// padded to whole drain steps with jobs that read the frontier's extra all-zero row (vertex
+// n_vertices, allocated past the last strip and never written), so no slot is tested.
+constexpr int kJobCap = 232;     // jobs per wave list (a multiple of 2 * kDrainUnroll)
+constexpr int kDrainUnroll = 4;  // job pairs per drain step (two steps in flight)
 
 __device__ inline uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Job: source vertex in the low 32 bits, target row in bits 32..37.
-__device__ inline void drain_jobs(const unsigned long long* __restrict__ jobs, uint32_t n,
-                                  gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
-                                  uint32_t lane, uint64_t keep0, uint64_t keep1) {
-  const uint32_t half = lane >> 5;
-  const uint32_t w2 = (lane & 31) * 2;  // this lane's two words of a 64-word row
-  for (uint32_t j = 0; j < n; j += 2 * kDrainUnroll) {
-    unsigned long long e[kDrainUnroll];
+__device__ inline unsigned long long make_job(uint64_t vertex, uint32_t row) {
+  return (vertex << 9) | (static_cast<uint64_t>(row) << 3);
+}
+
+struct DrainStep {
+  uint64_t v0[kDrainUnroll], v1[kDrainUnroll];  // words c and c + 32 of the source row
+  uint32_t lds[kDrainUnroll];                    // LDS byte offset of the target words
+};
+
+// xl: frontier base + 8 * c (byte pointer); jobs[j .. j + 2 * kDrainUnroll) are valid slots.
+__device__ inline void drain_issue(DrainStep& d, const unsigned long long* __restrict__ jobs_h, uint32_t j,
+                                   gptr<const char> __restrict__ xl, uint32_t c8) {
 #pragma unroll
-    for (int k = 0; k < kDrainUnroll; ++k) {
-      const uint32_t idx = j + 2 * k + half;
-      e[k] = idx < n ? jobs[idx] : ~0ull;
-    }
-    u64x2_t v[kDrainUnroll];
-#pragma unroll
-    for (int k = 0; k < kDrainUnroll; ++k) {
-      v[k] = u64x2_t{0, 0};
-      if (e[k] != ~0ull) {
-        const uint32_t u = static_cast<uint32_t>(e[k]);
-        v[k] = *reinterpret_cast<gptr<const u64x2_t>>(x_in + static_cast<uint64_t>(u) * 64 + w2);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kDrainUnroll; ++k) {
-      if (e[k] != ~0ull) {
-        unsigned long long* a = acc + static_cast<uint32_t>(e[k] >> 32) * 64 + w2;
-        atomicOr(a, v[k].x & keep0);
-        atomicOr(a + 1, v[k].y & keep1);
-      }
-    }
+  for (int k = 0; k < kDrainUnroll; ++k) {
+    const unsigned long long e = jobs_h[j + 2 * k];
+    const gptr<const char> src = xl + (e & ~0x1FFull);
+    d.v0[k] = *reinterpret_cast<gptr<const uint64_t>>(src);
+    d.v1[k] = *reinterpret_cast<gptr<const uint64_t>>(src + 256);
+    d.lds[k] = ((static_cast<uint32_t>(e) & 0x1F8u) << 6) | c8;
   }
 }
 
-// Appends the job (tag + lowest set bit of `bits`) of every lane whose `bits` is non-zero,
-// clearing that bit, until no lane has bits left; drains the list whenever it could overflow.
-__device__ inline void emit_jobs(uint64_t bits, uint64_t tag, unsigned long long* __restrict__ jobs, uint32_t& len,
+template <bool kMasked>
+__device__ inline void drain_commit(const DrainStep& d, char* __restrict__ acc_b, uint64_t keep0, uint64_t keep1) {
+#pragma unroll
+  for (int k = 0; k < kDrainUnroll; ++k) {
+    unsigned long long* a = reinterpret_cast<unsigned long long*>(acc_b + d.lds[k]);
+    atomicOr(a, kMasked ? (d.v0[k] & keep0) : d.v0[k]);
+    atomicOr(a + 32, kMasked ? (d.v1[k] & keep1) : d.v1[k]);
+  }
+}
+
+template <bool kMasked>
+__device__ inline void drain_loop(const unsigned long long* __restrict__ jobs_h, uint32_t n, gptr<const char> xl,
+                                  char* __restrict__ acc_b, uint32_t c8, uint64_t keep0, uint64_t keep1) {
+  constexpr uint32_t kStep = 2 * kDrainUnroll;
+  // Ping-pong between two fixed register sets (no copies, so no wait before the next issue).
+  DrainStep a, b;
+  drain_issue(a, jobs_h, 0, xl, c8);
+  for (uint32_t j = kStep;; j += 2 * kStep) {
+    if (j >= n) {
+      drain_commit<kMasked>(a, acc_b, keep0, keep1);
+      break;
+    }
+    drain_issue(b, jobs_h, j, xl, c8);
+    drain_commit<kMasked>(a, acc_b, keep0, keep1);
+    if (j + kStep >= n) {
+      drain_commit<kMasked>(b, acc_b, keep0, keep1);
+      break;
+    }
+    drain_issue(a, jobs_h, j + kStep, xl, c8);
+    drain_commit<kMasked>(b, acc_b, keep0, keep1);
+  }
+}
+
+// Drains the first n jobs of this wave's list (n > 0). Lane c of a half-wave handles words
+// c and c + 32 of a 64-word row, so each LDS OR covers 32 consecutive words of two rows.
+// Words of finished queries (keep masks) are masked only in panels that have any.
+__device__ inline void drain_jobs(unsigned long long* __restrict__ jobs, uint32_t n, gptr<const uint64_t> x_in,
+                                  unsigned long long* __restrict__ acc, uint32_t lane, uint64_t keep0, uint64_t keep1,
+                                  bool all_alive, unsigned long long pad_job) {
+  constexpr uint32_t kStep = 2 * kDrainUnroll;
+  const uint32_t padded = (n + kStep - 1) / kStep * kStep;
+  if (lane < padded - n) jobs[n + lane] = pad_job;
+  wave_lds_sync();
+  const uint32_t c = lane & 31;
+  const gptr<const char> xl = reinterpret_cast<gptr<const char>>(x_in) + 8 * c;
+  const unsigned long long* jobs_h = jobs + (lane >> 5);
+  char* const acc_b = reinterpret_cast<char*>(acc);
+  if (all_alive)
+    drain_loop<false>(jobs_h, padded, xl, acc_b, 8 * c, keep0, keep1);
+  else
+    drain_loop<true>(jobs_h, padded, xl, acc_b, 8 * c, keep0, keep1);
+}
+
+// Appends make_job(tag_vertex + lowest set bit of `bits`, row) for every lane whose `bits`
+// is non-zero, clearing that bit, until no lane has bits left; drains the list whenever it
+// could overflow.
+__device__ inline void emit_jobs(uint64_t bits, uint64_t base_vertex, uint32_t row,
+                                 unsigned long long* __restrict__ jobs, uint32_t& len,
                                  gptr<const uint64_t> __restrict__ x_in, unsigned long long* __restrict__ acc,
-                                 uint32_t lane, uint64_t keep0, uint64_t keep1) {
+                                 uint32_t lane, uint64_t keep0, uint64_t keep1, bool all_alive,
+                                 unsigned long long pad_job) {
   for (uint64_t has = ballot(bits != 0); has; has = ballot(bits != 0)) {
     if (len + 64 > kJobCap) {
-      wave_lds_sync();
-      drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
+      drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1, all_alive, pad_job);
       wave_lds_sync();
       len = 0;
     }
     if (bits) {
-      jobs[len + lanes_below(has)] = tag + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1);
+      jobs[len + lanes_below(has)] =
+          make_job(base_vertex + static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(bits)) - 1), row);
       bits &= bits - 1;
     }
     len += __popcll(has);
+  }
+}
+
+// State a wave carries through the tiles of one strip.
+struct WaveTileState {
+  unsigned long long* jobs;
+  uint32_t len;  // jobs in the list (wave-uniform)
+  uint64_t sym;  // structural OR of the source strips (per lane)
+};
+
+// Decodes one group of 64 tiles (lane = tile) into jobs.
+__device__ inline void decode_group(WaveTileState& st, uint32_t p, bool valid, uint32_t t_l, uint64_t pl,
+                                    uint64_t vm_l, uint64_t sm_l, gptr<const uint64_t> __restrict__ data,
+                                    gptr<const uint64_t> __restrict__ rmask, gptr<const uint64_t> __restrict__ x_in,
+                                    unsigned long long* __restrict__ acc, uint32_t lane, uint64_t keep0,
+                                    uint64_t keep1, bool all_alive, unsigned long long pad_job,
+                                    uint32_t (&cnt)[kBlockCounters]) {
+  st.sym |= sm_l;
+  cnt[bTP] += __popcll(sm_l);
+  cnt[bUsed] += (sm_l != 0);
+  const bool act = valid && vm_l != 0 && pl != 0;
+  const bool dense = act && (pl & kPayloadDense) != 0;
+  // Sparse tiles: entries of this lane's payload whose source vertex is non-empty.
+  const uint64_t ent = (act && !dense) ? pl : 0ull;
+  const uint32_t n_ent = static_cast<uint32_t>(ent >> 48) & 7u;
+#pragma unroll
+  for (uint32_t i = 0; i < kPayloadMax; ++i) {
+    const uint32_t e = static_cast<uint32_t>(ent >> (12 * i)) & 0xFFFu;
+    const bool ok = i < n_ent && ((vm_l >> (e & 63u)) & 1);
+    cnt[bEdges] += ok;
+    const uint64_t has = ballot(ok);
+    if (has) {
+      if (st.len + 64 > kJobCap) {
+        drain_jobs(st.jobs, st.len, x_in, acc, lane, keep0, keep1, all_alive, pad_job);
+        wave_lds_sync();
+        st.len = 0;
+      }
+      if (ok) st.jobs[st.len + lanes_below(has)] = make_job(static_cast<uint64_t>(t_l) * 64 + (e & 63u), e >> 6);
+      st.len += __popcll(has);
+    }
+  }
+  // Dense tiles: the whole wave loads one (lane = row) and emits its jobs.
+  for (uint64_t dm = ballot(dense); dm; dm &= dm - 1) {
+    const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+    const uint32_t pk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), k));
+    const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
+    const uint64_t vm = read_lane(vm_l, k);
+    const uint64_t rm = rmask[pk];
+    const uint64_t bits =
+        ((rm >> lane) & 1) ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) & vm) : 0ull;
+    cnt[bEdges] += __popcll(bits);
+    emit_jobs(bits, static_cast<uint64_t>(t) * 64, lane, st.jobs, st.len, x_in, acc, lane, keep0, keep1, all_alive,
+              pad_job);
   }
 }
 
@@ -588,7 +701,6 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
   __shared__ unsigned long long s_cnt[kBlockCounters];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
-  unsigned long long* const jobs = jobs_all[wave];
   const Chunk ch = chunks[blockIdx.x];
   const PanelLevel& L = lv[ch.panel];
   const gptr<const uint32_t> __restrict__ off = as_global(L.off);
@@ -601,83 +713,64 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
   const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
   const uint64_t alive_in = L.alive_in;
   const uint64_t alive_next = L.alive_next;
-  const uint64_t keep0 = ((alive_in >> ((lane & 31) * 2)) & 1) ? ~0ull : 0ull;
-  const uint64_t keep1 = ((alive_in >> ((lane & 31) * 2 + 1)) & 1) ? ~0ull : 0ull;
+  const uint64_t keep0 = ((alive_in >> (lane & 31)) & 1) ? ~0ull : 0ull;
+  const uint64_t keep1 = ((alive_in >> ((lane & 31) + 32)) & 1) ? ~0ull : 0ull;
+  const bool all_alive = alive_in == ~0ull;
+  const unsigned long long pad_job = make_job(static_cast<uint64_t>(L.n_strips) * 64, 0);  // the zero row
+  // Per-strip values of the chunk (at most kChunkStrips strips), one strip per lane, loaded
+  // once: row pointers, previous Z and N masks, and S_k & alive for the tile counts.
+  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
+  const uint32_t ys = ch.strip_begin + lane;
+  const uint32_t off_l = lane <= n_cs ? off[ys] : 0u;
+  const uint64_t zp_l = lane < n_cs ? as_global(L.z_out)[ys] : 0ull;
+  const uint64_t np_l = lane < n_cs ? as_global(L.n_out)[ys] : 0ull;
+  const uint64_t left_l = lane < n_cs ? (s_in[ys] & alive_in) : 0ull;
   uint32_t cnt[kBlockCounters];
 #pragma unroll
   for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
   if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
+  if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
   __syncthreads();
 
-  for (uint32_t y = ch.strip_begin; y < ch.strip_end; ++y) {
-    const uint32_t r0 = off[y], r1 = off[y + 1];
+  WaveTileState st{jobs_all[wave], 0u, 0ull};
+  for (uint32_t i = 0; i < n_cs; ++i) {
+    const uint32_t y = ch.strip_begin + i;
+    const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i)));
+    const uint32_t r1 =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i + 1)));
     const uint32_t seg0 = r0 > ch.tile_begin ? r0 : ch.tile_begin;
     const uint32_t seg1 = r1 < ch.tile_end ? r1 : ch.tile_end;
     const bool split = (seg0 != r0) || (seg1 != r1);
-    for (int i = threadIdx.x; i < kWords; i += kNumBlock) acc[i] = 0;
-    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
-    __syncthreads();
-
-    uint32_t len = 0;  // jobs in this wave's list (wave-uniform)
-    uint64_t sym = 0;
-    for (uint32_t g0 = seg0 + 64 * wave; g0 < seg1; g0 += 64 * kNumWaves) {
-      const uint32_t p = g0 + lane;
-      const bool valid = p < seg1;
-      // The tile index and payload streams are read once per hop.
-      const uint32_t t_l = valid ? __builtin_nontemporal_load(col + p) : 0u;
-      const uint64_t pl = valid ? __builtin_nontemporal_load(payload + p) : 0ull;
-      const uint64_t vm_l = valid ? v_in[t_l] : 0ull;
-      const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0ull;
-      sym |= sm_l;
-      cnt[bTP] += __popcll(sm_l);
-      cnt[bUsed] += (sm_l != 0);
-      const bool act = vm_l != 0 && pl != 0;
-      const bool dense = act && (pl & kPayloadDense) != 0;
-      // Sparse tiles: entries of this lane's payload whose source vertex is non-empty.
-      const uint64_t ent = (act && !dense) ? pl : 0ull;
-      const uint32_t n_ent = static_cast<uint32_t>(ent >> 48) & 7u;
-#pragma unroll
-      for (uint32_t i = 0; i < kPayloadMax; ++i) {
-        const uint32_t e = static_cast<uint32_t>(ent >> (12 * i)) & 0xFFFu;
-        const bool ok = i < n_ent && ((vm_l >> (e & 63u)) & 1);
-        cnt[bEdges] += ok;
-        const uint64_t tag = (static_cast<uint64_t>(e >> 6) << 32) | (static_cast<uint64_t>(t_l) * 64 + (e & 63u));
-        const uint64_t has = ballot(ok);
-        if (has) {
-          if (len + 64 > kJobCap) {
-            wave_lds_sync();
-            drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
-            wave_lds_sync();
-            len = 0;
-          }
-          if (ok) jobs[len + lanes_below(has)] = tag;
-          len += __popcll(has);
-        }
-      }
-      // Dense tiles: the whole wave loads one (lane = row) and emits its jobs.
-      for (uint64_t dm = ballot(dense); dm; dm &= dm - 1) {
-        const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
-        const uint32_t pk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), k));
-        const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
-        const uint64_t vm = read_lane(vm_l, k);
-        const uint64_t rm = rmask[pk];
-        const uint64_t bits =
-            ((rm >> lane) & 1) ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) & vm) : 0ull;
-        cnt[bEdges] += __popcll(bits);
-        emit_jobs(bits, (static_cast<uint64_t>(lane) << 32) | (static_cast<uint64_t>(t) * 64), jobs, len, x_in, acc,
-                  lane, keep0, keep1);
-      }
+    st.len = 0;
+    st.sym = 0;
+    // Two groups of 64 tiles per step, their index and mask loads issued together. The
+    // tile index and payload streams are read once per hop.
+    for (uint32_t g0 = seg0 + 64 * wave; g0 < seg1; g0 += 128 * kNumWaves) {
+      const uint32_t pa = g0 + lane, pb = g0 + 64 * kNumWaves + lane;
+      const bool va = pa < seg1, vb = pb < seg1;
+      const uint32_t ta = va ? __builtin_nontemporal_load(col + pa) : 0u;
+      const uint32_t tb = vb ? __builtin_nontemporal_load(col + pb) : 0u;
+      const uint64_t pla = va ? __builtin_nontemporal_load(payload + pa) : 0ull;
+      const uint64_t plb = vb ? __builtin_nontemporal_load(payload + pb) : 0ull;
+      const uint64_t vma = va ? v_in[ta] : 0ull, vmb = vb ? v_in[tb] : 0ull;
+      const uint64_t sma = va ? (s_in[ta] & alive_in) : 0ull, smb = vb ? (s_in[tb] & alive_in) : 0ull;
+      decode_group(st, pa, va, ta, pla, vma, sma, data, rmask, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
+                   cnt);
+      decode_group(st, pb, vb, tb, plb, vmb, smb, data, rmask, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
+                   cnt);
     }
-    if (len) {
-      wave_lds_sync();
-      drain_jobs(jobs, len, x_in, acc, lane, keep0, keep1);
-    }
-    sym = wave_or(sym);
+    if (st.len) drain_jobs(st.jobs, st.len, x_in, acc, lane, keep0, keep1, all_alive, pad_job);
+    const uint64_t sym = wave_or(st.sym);
     if (lane == 0 && sym) atomicOr(&s_sym, static_cast<unsigned long long>(sym));
     __syncthreads();
     write_strip<W, kNumBlock>(acc, s_n, s_v, s_z, s_sym, y, split, as_global(L.x_out), as_global(L.n_out),
-                              as_global(L.v_out), as_global(L.z_out), as_global(L.s_out), s_in, alive_in, alive_next,
-                              cnt);
+                              as_global(L.v_out), as_global(L.z_out), as_global(L.s_out), read_lane(zp_l, i),
+                              read_lane(np_l, i), read_lane(left_l, i), alive_next, cnt);
+    __syncthreads();
+    // Ready the accumulator and reductions for the next strip.
+    for (int k = threadIdx.x; k < kWords; k += kNumBlock) acc[k] = 0;
+    if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
     __syncthreads();
   }
   flush_block_counters(cnt, s_cnt, as_global(L.counters));
@@ -731,12 +824,13 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
     L.n_out[it.strip] = s_n;
     L.v_out[it.strip] = s_v;
     L.z_out[it.strip] = s_z;
-    if (s_pop) atomicAdd(L.counters + kPop, s_pop);
-    if (s_n) atomicAdd(L.counters + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
+    unsigned long long* const slot = L.counters + (blockIdx.x % kCounterSlots) * kNumCounters;
+    if (s_pop) atomicAdd(slot + kPop, s_pop);
+    if (s_n) atomicAdd(slot + kOutNumeric, static_cast<unsigned long long>(__popcll(s_n)));
     const uint64_t so = L.s_out[it.strip];
-    if (so) atomicAdd(L.counters + kOut, static_cast<unsigned long long>(__popcll(so)));
+    if (so) atomicAdd(slot + kOut, static_cast<unsigned long long>(__popcll(so)));
     const uint64_t left = L.s_in[it.strip] & L.alive_in;
-    if (left) atomicAdd(L.counters + kLeft, static_cast<unsigned long long>(__popcll(left)));
+    if (left) atomicAdd(slot + kLeft, static_cast<unsigned long long>(__popcll(left)));
   }
 }
 
@@ -864,7 +958,7 @@ struct Engine {
   std::vector<GraphPlan> graphs;
   std::vector<Panel> panels;
   uint32_t max_hops = 0;
-  DevBuf<unsigned long long> counters;  // panels * (max_hops+1) * kNumCounters
+  DevBuf<unsigned long long> counters;  // panels * (max_hops+1) * kCounterStride
   // per level
   std::vector<DevBuf<PanelLevel>> lv;
   std::vector<DevBuf<Chunk>> chunks;
@@ -881,7 +975,7 @@ struct Engine {
   int prune = 0;
 
   unsigned long long* ctr(uint32_t p, uint32_t level) {
-    return counters.get() + (static_cast<size_t>(p) * (max_hops + 1) + level) * kNumCounters;
+    return counters.get() + (static_cast<size_t>(p) * (max_hops + 1) + level) * kCounterStride;
   }
 };
 
@@ -900,7 +994,7 @@ void build_engine(Engine* e) {
     p.v.resize(p.hops + 1);
     p.z.resize(p.hops + 1);
     for (uint32_t k = 0; k <= p.hops; ++k) {
-      p.x[k].alloc(nv * e->W);
+      p.x[k].alloc((nv + 1) * e->W);  // plus one all-zero vertex row (drain padding reads it)
       p.s[k].alloc(p.n_strips);
       p.n[k].alloc(p.n_strips);
       p.v[k].alloc(p.n_strips);
@@ -914,7 +1008,7 @@ void build_engine(Engine* e) {
       MG_HIP(hipMemset(p.z[k].get(), 0, p.z[k].bytes()));
     }
   }
-  e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kNumCounters);
+  e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kCounterStride);
   e->lv.resize(e->max_hops);
   e->chunks.resize(e->max_hops);
   e->n_chunks.assign(e->max_hops, 0);
@@ -1062,7 +1156,11 @@ uint32_t words_for(uint32_t n_sources) {
   return W;
 }
 
-void hop_stats_from(const unsigned long long* c, uint32_t rows, mg_hop_stats* out) {
+// c: the kCounterSlots slots of one (panel, level).
+void hop_stats_from(const unsigned long long* c_slots, uint32_t rows, mg_hop_stats* out) {
+  unsigned long long c[kNumCounters] = {};
+  for (uint32_t sl = 0; sl < kCounterSlots; ++sl)
+    for (uint32_t i = 0; i < kNumCounters; ++i) c[i] += c_slots[sl * kNumCounters + i];
   std::memset(out, 0, sizeof(*out));
   out->tile_products = c[kTP];
   out->left_tiles = c[kLeft];
@@ -1200,7 +1298,7 @@ extern "C" int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_h
     MG_REQUIRE(ctx && w && out, "mg_walk_hop_stats: NULL argument");
     MG_REQUIRE(hop >= 1 && hop <= w->hops, "mg_walk_hop_stats: hop out of range");
     DeviceGuard g(ctx->device);
-    unsigned long long c[kNumCounters];
+    unsigned long long c[kCounterStride];
     MG_HIP(hipMemcpyAsync(c, w->e.ctr(0, hop), sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
     hop_stats_from(c, (w->n_sources + 63) / 64, out);
@@ -1252,7 +1350,8 @@ extern "C" int mg_walk_frontier_download(mg_context* ctx, const mg_walk* w, uint
     MG_REQUIRE(hop <= w->hops, "mg_walk_frontier_download: hop out of range");
     DeviceGuard g(ctx->device);
     const auto& x = w->e.panels[0].x[hop];
-    MG_HIP(hipMemcpyAsync(xt, x.get(), x.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    const uint64_t words = static_cast<uint64_t>(w->e.panels[0].n_strips) * 64 * w->e.W;
+    MG_HIP(hipMemcpyAsync(xt, x.get(), words * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -1517,7 +1616,7 @@ extern "C" int mg_walk_batch_query_stats(mg_context* ctx, const mg_walk_batch* b
     const QueryPlace& P = b->queries[q];
     MG_REQUIRE(hop >= 1 && hop <= b->e.panels[P.panel].hops, "mg_walk_batch_query_stats: hop out of range");
     DeviceGuard g(ctx->device);
-    unsigned long long c[kNumCounters];
+    unsigned long long c[kCounterStride];
     MG_HIP(hipMemcpyAsync(c, const_cast<Engine&>(b->e).ctr(P.panel, hop), sizeof(c), hipMemcpyDeviceToHost,
                           ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
@@ -1536,7 +1635,7 @@ extern "C" int mg_walk_batch_totals(mg_context* ctx, const mg_walk_batch* b, uin
     uint64_t tp = 0, by = 0, ed = 0;
     for (uint32_t pi = 0; pi < b->e.panels.size(); ++pi)
       for (uint32_t k = 1; k <= b->e.panels[pi].hops; ++k) {
-        const unsigned long long* x = c.data() + (static_cast<size_t>(pi) * (b->e.max_hops + 1) + k) * kNumCounters;
+        const unsigned long long* x = c.data() + (static_cast<size_t>(pi) * (b->e.max_hops + 1) + k) * kCounterStride;
         mg_hop_stats s;
         hop_stats_from(x, b->e.panels[pi].words_used, &s);
         tp += s.tile_products;
