@@ -66,22 +66,9 @@ def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
     }
 
 
-def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int):
-    """Uneven query batch: 512 queries over 8 graphs (R-MAT scales 16..20, three ER
-    n=2^18 p=2^-13), sizes log-uniform in [1, 4096], hops uniform in [1, 6].
-    Plain = every query run alone (its own walk, words per vertex sized to the query);
-    batched = one captured HIP graph over panels, launch options from the autotuner.
-    Per-query X_hops digests must agree between the two forms."""
-    from matrixgraph_amd import autotune
-    from matrixgraph_amd import workloads as WL
-
-    specs = WL.batch_graph_specs(seed0)
-    t0 = time.perf_counter()
-    built = [WL.build_graph(ctx, s) for s in specs]
-    graphs = [g for g, _ in built]
-    queries = WL.batch_queries(specs, n_queries, query_seed)
-    log(f"batch: {len(specs)} graphs built in {time.perf_counter() - t0:.1f}s, {len(queries)} queries")
-
+def plain_walks(M, ctx, graphs, queries, reps):
+    """Every query run alone (its own walk, words per vertex sized to the query): seconds per
+    pass over all queries, and each query's X_hops digest."""
     walks = {}
 
     def walk_for(q):
@@ -93,25 +80,43 @@ def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int
             walks[key] = M.Walk(ctx, graphs[q.graph], 64 * W, 6)
         return walks[key]
 
-    def run_plain():
+    def run_all():
         for q in queries:
             walk_for(q).run(q.sources, q.hops)
 
-    run_plain()
+    run_all()
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        run_plain()
+        run_all()
     ctx.synchronize()
-    plain_s = (time.perf_counter() - t0) / reps
-    plain_digests = []
-    for q in queries:  # last run of each (graph, W) walk is not the query's: rerun singly for digests
+    secs = (time.perf_counter() - t0) / reps
+    digests = []
+    for q in queries:  # a walk object is shared by queries: rerun each for its own digest
         w = walk_for(q)
         w.run(q.sources, q.hops)
-        plain_digests.append(w.frontier_digest(q.hops))
+        digests.append(w.frontier_digest(q.hops))
     for w in walks.values():
         w.close()
-    walks.clear()
+    return secs, digests
+
+
+def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int):
+    """Uneven query batch: 512 queries over 8 graphs (R-MAT scales 16..20, three ER
+    n=2^18 p=2^-13), sizes log-uniform in [1, 4096], hops uniform in [1, 6].
+    Plain = every query run alone; batched = one captured HIP graph over panels, launch
+    options from the autotuner. Per-query X_hops digests must agree between the two forms."""
+    from matrixgraph_amd import autotune
+    from matrixgraph_amd import workloads as WL
+
+    specs = WL.batch_graph_specs(seed0)
+    t0 = time.perf_counter()
+    built = [WL.build_graph(ctx, s) for s in specs]
+    graphs = [g for g, _ in built]
+    queries = WL.batch_queries(specs, n_queries, query_seed)
+    log(f"batch: {len(specs)} graphs built in {time.perf_counter() - t0:.1f}s, {len(queries)} queries")
+
+    plain_s, plain_digests = plain_walks(M, ctx, graphs, queries, reps)
     log(f"batch plain: {plain_s * 1e3:.1f} ms per batch ({n_queries / plain_s:.1f} queries/s)")
 
     best, table = autotune.tune(ctx, graphs, queries, reps=2, log=log)
@@ -131,7 +136,7 @@ def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int
         f"{n_panels} panels, bit-exact={exact}")
     if not exact:
         raise SystemExit("batch: batched digests differ from the per-query runs")
-    return {
+    result = {
         "workload": f"{n_queries} walk queries over {len(specs)} graphs (" + "; ".join(s.describe() for s in specs)
                     + f"), query seed {query_seed}, sizes log-uniform [1,4096], hops uniform [1,6]",
         "plain_queries_per_s": round(n_queries / plain_s, 2),
@@ -144,6 +149,72 @@ def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int
         "chunks": n_chunks,
         "autotune": {"best": best, "table": table},
         "bit_exact": exact,
+    }
+    return result, specs, graphs, [m for _, m in built], best
+
+
+def learned_section(M, ctx, train_specs, train_graphs, train_edges, chunk_tiles, n_queries, reps):
+    """Cost model trained on logged runs of the uneven-query batch graphs (seeds 10-17,
+    fresh query seeds), evaluated on a batch over new graphs (seeds 20-27): plain per-query
+    walks vs the batch in its static launch order, the learned order (with the exact
+    fallback), and the order by exact logged costs."""
+    from matrixgraph_amd import learned as LN
+    from matrixgraph_amd import workloads as WL
+
+    t0 = time.perf_counter()
+    gstats_tr = LN.graph_stats(train_graphs, train_edges)
+    train_sets = [WL.batch_queries(train_specs, n_queries, s) for s in range(101, 109)]
+    X, T = LN.collect(ctx, train_graphs, gstats_tr, train_sets, chunk_tiles=chunk_tiles, log=log)
+    mlp, tinfo = LN.train(ctx, X, T, epochs=6, batch=4096, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t0
+
+    specs = WL.batch_graph_specs(20)
+    built = [WL.build_graph(ctx, s) for s in specs]
+    graphs = [g for g, _ in built]
+    gstats = LN.graph_stats(graphs, [m for _, m in built])
+    queries = WL.batch_queries(specs, n_queries, 2)
+    plain_s, plain_digests = plain_walks(M, ctx, graphs, queries, reps)
+
+    def make():
+        return M.WalkBatch(ctx, graphs, queries, chunk_tiles=chunk_tiles, use_graph=True)
+
+    static = make()
+    static_s = LN.time_batch(ctx, static, reps)
+    ok_static = all(static.query_digest(i) == plain_digests[i] for i in range(len(queries)))
+    static.close()
+    learned = make()
+    metrics = LN.schedule_learned(ctx, learned, mlp, gstats)
+    learned_s = LN.time_batch(ctx, learned, reps)
+    ok_learned = all(learned.query_digest(i) == plain_digests[i] for i in range(len(queries)))
+    learned.close()
+    exact_b = make()
+    LN.schedule_exact(ctx, exact_b)
+    exact_s = LN.time_batch(ctx, exact_b, reps)
+    exact_b.close()
+    for g in graphs:
+        g.close()
+    log(f"learned: plain {plain_s * 1e3:.1f} ms, static {static_s * 1e3:.1f} ms, learned {learned_s * 1e3:.1f} ms, "
+        f"exact-order {exact_s * 1e3:.1f} ms, mispredict {metrics['jobs_misprediction_rate']:.3f}, "
+        f"fallback {metrics['fallback_levels']}/{metrics['levels']}, bit-exact={ok_static and ok_learned}")
+    if not (ok_static and ok_learned):
+        raise SystemExit("learned: batched digests differ from the per-query runs")
+    return {
+        "workload": f"{n_queries} walk queries over 8 fresh graphs (" + "; ".join(s.describe() for s in specs)
+                    + "), query seed 2",
+        "model": "MLP 8->64->64->2, BF16 weights, FP32 accumulation, trained on the GPU by the library's kernels",
+        "training": dict(tinfo, seconds=round(t_train, 1),
+                         source="per-chunk outcomes logged from batch runs over the seeds 10-17 graphs, "
+                                "query seeds 101-108"),
+        "plain_queries_per_s": round(n_queries / plain_s, 2),
+        "static_queries_per_s": round(n_queries / static_s, 2),
+        "learned_queries_per_s": round(n_queries / learned_s, 2),
+        "exact_order_queries_per_s": round(n_queries / exact_s, 2),
+        "learned_batched_vs_plain": round(plain_s / learned_s, 3),
+        "learned_vs_static": round(static_s / learned_s, 4),
+        "exact_order_vs_static": round(static_s / exact_s, 4),
+        "chunk_tiles": chunk_tiles,
+        "metrics": metrics,
+        "bit_exact": True,
     }
 
 
@@ -160,6 +231,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=64, help="CPU baseline samples every n-th strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
@@ -285,7 +357,11 @@ def main():
                                               threads)
     if not args.no_batch:
         w.close()
-        result["batch"] = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)
+        result["batch"], specs, graphs, edges, best = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)
+        if not args.no_learned:
+            result["learned"] = learned_section(M, ctx, specs, graphs, edges, best["chunk_tiles"], args.batch_queries,
+                                                args.batch_reps)
+            result["learned_batched_vs_plain"] = result["learned"]["learned_batched_vs_plain"]
     print(json.dumps(result), flush=True)
 
 

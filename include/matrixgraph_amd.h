@@ -340,6 +340,30 @@ int mg_walk_batch_query_download(mg_context* ctx, const mg_walk_batch* b, uint32
 int mg_walk_batch_query_export(mg_context* ctx, const mg_walk_batch* b, uint32_t q, mg_tiled_matrix** out);
 int mg_walk_batch_destroy(mg_walk_batch* b);
 
+/* Scheduler control for cost models (no counterpart in the reference). A hop level's work is
+ * a list of chunks (strip ranges of one panel's A^T tile stream) launched in list order.
+ * Logging, when enabled, records every chunk's outcome on each later run. */
+typedef struct mg_chunk_record {
+  uint32_t panel;                   /* panel the chunk belongs to */
+  uint32_t strip_begin, strip_end;  /* target strips */
+  uint32_t tile_begin, tile_end;    /* A^T tiles (positions in the tile stream) */
+  uint32_t bits;                    /* set bits of those tiles */
+  uint32_t jobs;                    /* logged: gathers done (bits with a non-empty source) */
+  uint32_t out_tiles;               /* logged: numerically non-empty output tiles (unsplit strips) */
+} mg_chunk_record;
+int mg_walk_batch_set_logging(mg_walk_batch* b, int enable);
+/* level in [0, max_hops): the X_level -> X_level+1 launch. out == NULL only sets *n.
+ * Synchronises. */
+int mg_walk_batch_level_chunks(mg_context* ctx, const mg_walk_batch* b, uint32_t level, uint64_t* n,
+                               mg_chunk_record* out);
+/* Panel p: its graph, hop count, words holding sources, and alive[k] (k < hops): the words
+ * that propagate from X_k to X_k+1. */
+int mg_walk_batch_panel_info(const mg_walk_batch* b, uint32_t p, uint32_t* graph, uint32_t* hops,
+                             uint32_t* words_used, uint64_t* alive);
+/* Reorders level's chunks: the i-th launched chunk becomes the current chunk order[i]
+ * (order is a permutation of 0..n-1). Results do not depend on the order. */
+int mg_walk_batch_set_level_order(mg_walk_batch* b, uint32_t level, const uint32_t* order, uint64_t n);
+
 /* ---------------------------------------------------------------------------
  * Learned cost model: MLP n_in -> 64 -> 64 -> n_out (ReLU), BF16 weights rounded to
  * nearest-even from FP32 masters, FP32 accumulation, hand-written forward / backward /

@@ -513,6 +513,33 @@ class WalkBatch:
         _check(lib().mg_walk_batch_query_download(self.ctx.h, self.h, q, _ptr(out)), "mg_walk_batch_query_download")
         return out
 
+    # --- scheduler control (cost models) ---------------------------------------------
+    def set_logging(self, on: bool):
+        _check(lib().mg_walk_batch_set_logging(self.h, 1 if on else 0), "mg_walk_batch_set_logging")
+
+    def level_chunks(self, level: int) -> np.ndarray:
+        """Chunks of hop level `level` in launch order (structured array of mg_chunk_record)."""
+        n = C.c_uint64()
+        _check(lib().mg_walk_batch_level_chunks(self.ctx.h, self.h, level, C.byref(n), None),
+               "mg_walk_batch_level_chunks")
+        arr = (_capi.ChunkRecord * max(1, n.value))()
+        _check(lib().mg_walk_batch_level_chunks(self.ctx.h, self.h, level, C.byref(n), arr),
+               "mg_walk_batch_level_chunks")
+        dt = np.dtype([(f, np.uint32) for f, _ in _capi.ChunkRecord._fields_])
+        return np.frombuffer(bytes(arr), dtype=dt, count=n.value).copy()
+
+    def panel_info(self, p: int):
+        g, h, w = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().mg_walk_batch_panel_info(self.h, p, C.byref(g), C.byref(h), C.byref(w), None),
+               "mg_walk_batch_panel_info")
+        alive = np.zeros(max(1, h.value), np.uint64)
+        _check(lib().mg_walk_batch_panel_info(self.h, p, None, None, None, _ptr(alive)), "mg_walk_batch_panel_info")
+        return {"graph": g.value, "hops": h.value, "words_used": w.value, "alive": alive[:h.value]}
+
+    def set_level_order(self, level: int, order):
+        o = np.ascontiguousarray(order, np.uint32)
+        _check(lib().mg_walk_batch_set_level_order(self.h, level, _ptr(o), len(o)), "mg_walk_batch_set_level_order")
+
     def query_export(self, q: int) -> TiledMatrix:
         h = C.c_void_p()
         _check(lib().mg_walk_batch_query_export(self.ctx.h, self.h, q, C.byref(h)), "mg_walk_batch_query_export")

@@ -40,6 +40,11 @@ class HopStats(C.Structure):
     ]
 
 
+class ChunkRecord(C.Structure):
+    _fields_ = [("panel", u32), ("strip_begin", u32), ("strip_end", u32), ("tile_begin", u32), ("tile_end", u32),
+                ("bits", u32), ("jobs", u32), ("out_tiles", u32)]
+
+
 class WalkQuery(C.Structure):
     _fields_ = [("graph", u32), ("n_sources", u32), ("sources", C.POINTER(u32)), ("hops", u32)]
 
@@ -117,6 +122,10 @@ SIGNATURES = {
     "mg_walk_batch_query_download": (i32, [vp, vp, u32, vp]),
     "mg_walk_batch_query_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
     "mg_walk_batch_destroy": (i32, [vp]),
+    "mg_walk_batch_set_logging": (i32, [vp, i32]),
+    "mg_walk_batch_level_chunks": (i32, [vp, vp, u32, P_u64, vp]),
+    "mg_walk_batch_panel_info": (i32, [vp, u32, P_u32, P_u32, P_u32, vp]),
+    "mg_walk_batch_set_level_order": (i32, [vp, u32, vp, u64]),
     # mlp
     "mg_mlp_create": (i32, [vp, u32, u32, u64, C.POINTER(vp)]),
     "mg_mlp_num_params": (i32, [vp, P_u32]),

@@ -116,6 +116,12 @@ struct FillRange {
   uint64_t* ptr;
   uint64_t n;
 };
+// Outcome of one scheduler chunk at one hop level (cost-model training data).
+struct ChunkLog {
+  uint32_t jobs;       // A^T bits whose source vertex was non-empty (gathers)
+  uint32_t out_tiles;  // numerically non-empty output tiles of its unsplit strips
+};
+static_assert(sizeof(ChunkLog) == 8, "written as one 8-byte word");
 
 // --- fills and X_0 ------------------------------------------------------------
 // Zero-fill of a list of ranges (a kernel, so it can sit inside a captured graph).
@@ -184,7 +190,8 @@ __global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __rest
 
 // Compact payload of an A^T tile with at most kPayloadMax set bits: entry i = (row << 6) |
 // column in bits 12i..12i+11 and the entry count in bits 48..50. A tile with more bits has
-// kPayloadDense set and is read row by row through its row-occupancy mask.
+// kPayloadDense set, its bit count in the low 13 bits, and is read row by row through its
+// row-occupancy mask.
 constexpr uint32_t kPayloadMax = 4;
 constexpr uint64_t kPayloadDense = 1ull << 63;
 
@@ -195,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
     const uint64_t word = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
     const uint64_t total = wave_sum(__popcll(word));
-    uint64_t pl = kPayloadDense;
+    uint64_t pl = kPayloadDense | total;
     if (total <= kPayloadMax) {
       pl = 0;
       uint32_t n = 0;
@@ -332,7 +339,8 @@ __device__ inline void tile_jobs(uint32_t t, uint64_t b, uint64_t vm, GatherQueu
 // Block-wide sum of per-thread counters into LDS (one atomic per wave), then one global
 // atomic per non-zero counter.
 __device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters], unsigned long long* s_cnt,
-                                            gptr<unsigned long long> __restrict__ counters) {
+                                            gptr<unsigned long long> __restrict__ counters,
+                                            ChunkLog* __restrict__ log) {
   const uint32_t lane = lane_id();
 #pragma unroll
   for (int i = 0; i < kBlockCounters; ++i) {
@@ -346,6 +354,11 @@ __device__ inline void flush_block_counters(const uint32_t (&c)[kBlockCounters],
 #pragma unroll
     for (int i = 0; i < kBlockCounters; ++i)
       if (s_cnt[i]) atomicAdd((unsigned long long*)(slot + map[i]), s_cnt[i]);
+    if (log) {
+      // ChunkLog {jobs, out_tiles} as one 8-byte store.
+      const uint64_t rec = (s_cnt[bEdges] & 0xFFFFFFFFull) | ((s_cnt[bNumeric] & 0xFFFFFFFFull) << 32);
+      as_global(reinterpret_cast<uint64_t*>(log))[blockIdx.x] = rec;
+    }
   }
 }
 
@@ -417,7 +430,8 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
 // strips' structural masks) and the tile-product counts.
 template <int W>
 __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk_numeric_kernel(const Chunk* __restrict__ chunks,
-                                                              const PanelLevel* __restrict__ lv) {
+                                                              const PanelLevel* __restrict__ lv,
+                                                              ChunkLog* __restrict__ log) {
   constexpr int kWords = 64 * W;  // accumulator words per strip
   __shared__ unsigned long long acc[kWords];
   __shared__ unsigned long long s_n, s_v, s_z, s_sym;
@@ -517,7 +531,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
                               as_global(L.z_out)[y], n_out[y], s_in[y] & alive_in, alive_next, cnt);
     __syncthreads();
   }
-  flush_block_counters(cnt, s_cnt, as_global(L.counters));
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), log);
 }
 
 // --- numeric, 64 words per vertex ---------------------------------------------
@@ -692,7 +706,8 @@ __device__ inline void decode_group(WaveTileState& st, uint32_t p, bool valid, u
 }
 
 __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
-    walk_numeric64_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv) {
+    walk_numeric64_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv,
+                          ChunkLog* __restrict__ log) {
   constexpr int W = 64;
   constexpr int kWords = 64 * W;
   __shared__ unsigned long long acc[kWords];
@@ -773,7 +788,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
     if (threadIdx.x == 0) { s_n = 0; s_v = 0; s_z = 0; s_sym = 0; }
     __syncthreads();
   }
-  flush_block_counters(cnt, s_cnt, as_global(L.counters));
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), log);
 }
 
 // Zero the strips that are written by several chunks (atomic OR targets) and their
@@ -859,6 +874,26 @@ __global__ void __launch_bounds__(kBlock) frontier_slice_kernel(const uint64_t* 
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride)
     out[i] = x[(i / n_words) * W + w0 + (i % n_words)];
+}
+
+// Set bits of each chunk's A^T tiles (from the tile payloads).
+__global__ void __launch_bounds__(kBlock) chunk_bits_kernel(const Chunk* __restrict__ chunks,
+                                                            const PanelLevel* __restrict__ lv,
+                                                            uint32_t* __restrict__ bits_out) {
+  __shared__ unsigned long long s_bits;
+  if (threadIdx.x == 0) s_bits = 0;
+  __syncthreads();
+  const Chunk ch = chunks[blockIdx.x];
+  const uint64_t* payload = lv[ch.panel].payload;
+  uint64_t b = 0;
+  for (uint32_t p = ch.tile_begin + threadIdx.x; p < ch.tile_end; p += kBlock) {
+    const uint64_t pl = payload[p];
+    b += (pl & kPayloadDense) ? (pl & 0x1FFFu) : ((pl >> 48) & 7u);
+  }
+  b = wave_sum(b);
+  if (lane_id() == 0 && b) atomicAdd(&s_bits, static_cast<unsigned long long>(b));
+  __syncthreads();
+  if (threadIdx.x == 0) bits_out[blockIdx.x] = static_cast<uint32_t>(s_bits);
 }
 
 // Export tile (xr, y) of X_k in the reference orientation: X tile word c (source xr*64+c)
@@ -960,6 +995,9 @@ struct Engine {
   uint32_t max_hops = 0;
   DevBuf<unsigned long long> counters;  // panels * (max_hops+1) * kCounterStride
   // per level
+  std::vector<std::vector<Chunk>> host_chunks;  // launch order of each level
+  std::vector<DevBuf<ChunkLog>> logs;           // per-chunk outcomes (when logging)
+  bool logging = false;
   std::vector<DevBuf<PanelLevel>> lv;
   std::vector<DevBuf<Chunk>> chunks;
   std::vector<uint32_t> n_chunks;
@@ -1011,6 +1049,7 @@ void build_engine(Engine* e) {
   e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kCounterStride);
   e->lv.resize(e->max_hops);
   e->chunks.resize(e->max_hops);
+  e->host_chunks.assign(e->max_hops, {});
   e->n_chunks.assign(e->max_hops, 0);
   e->split.resize(e->max_hops);
   e->n_split.assign(e->max_hops, 0);
@@ -1061,6 +1100,7 @@ void build_engine(Engine* e) {
     e->lv[k].alloc(P);
     MG_HIP(hipMemcpy(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice));
     e->n_chunks[k] = static_cast<uint32_t>(ch.size());
+    e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
     if (!ch.empty()) MG_HIP(hipMemcpy(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
     e->n_split[k] = static_cast<uint32_t>(sp.size());
@@ -1097,10 +1137,11 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   }
   if (ev) MG_HIP(hipEventRecord(ev[2], st));
   if (e->n_chunks[k]) {
+    ChunkLog* const log = e->logging ? e->logs[k].get() : nullptr;
     if constexpr (W == 64)
-      walk_numeric64_kernel<<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+      walk_numeric64_kernel<<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log);
     else
-      walk_numeric_kernel<W><<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+      walk_numeric_kernel<W><<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log);
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
@@ -1693,5 +1734,100 @@ extern "C" int mg_walk_batch_destroy(mg_walk_batch* b) {
     if (!b) return;
     DeviceGuard g(b->device);
     delete b;
+  });
+}
+
+// ----------------------------------------------------------------------------
+// Scheduler control (cost models): logging, chunk records, panel info, launch order.
+// ----------------------------------------------------------------------------
+namespace {
+void drop_graph(mg_walk_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+}  // namespace
+
+extern "C" int mg_walk_batch_set_logging(mg_walk_batch* b, int enable) {
+  return guarded([&] {
+    MG_REQUIRE(b != nullptr, "mg_walk_batch_set_logging: NULL batch");
+    DeviceGuard g(b->device);
+    MG_HIP(hipDeviceSynchronize());  // no run of this batch may still be reading its tables
+    Engine& e = b->e;
+    const bool on = enable != 0;
+    if (on && e.logs.empty()) {
+      e.logs.resize(e.max_hops);
+      for (uint32_t k = 0; k < e.max_hops; ++k) {
+        e.logs[k].alloc(std::max<uint32_t>(1, e.n_chunks[k]));
+        MG_HIP(hipMemset(e.logs[k].get(), 0, e.logs[k].bytes()));
+      }
+    }
+    if (on != e.logging) drop_graph(b);  // kernel arguments change: capture again
+    e.logging = on;
+  });
+}
+
+extern "C" int mg_walk_batch_level_chunks(mg_context* ctx, const mg_walk_batch* b, uint32_t level, uint64_t* n,
+                                          mg_chunk_record* out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b && n, "mg_walk_batch_level_chunks: NULL argument");
+    const Engine& e = b->e;
+    MG_REQUIRE(level < e.max_hops, "mg_walk_batch_level_chunks: level out of range");
+    const auto& ch = e.host_chunks[level];
+    *n = ch.size();
+    if (!out || ch.empty()) return;
+    DeviceGuard g(ctx->device);
+    DevBuf<uint32_t> bits(ch.size());
+    chunk_bits_kernel<<<static_cast<uint32_t>(ch.size()), kBlock, 0, ctx->stream>>>(e.chunks[level].get(),
+                                                                                    e.lv[level].get(), bits.get());
+    MG_LAUNCH_CHECK();
+    std::vector<uint32_t> hb(ch.size());
+    std::vector<ChunkLog> hl(ch.size(), ChunkLog{0, 0});
+    MG_HIP(hipMemcpyAsync(hb.data(), bits.get(), sizeof(uint32_t) * ch.size(), hipMemcpyDeviceToHost, ctx->stream));
+    if (!e.logs.empty())
+      MG_HIP(hipMemcpyAsync(hl.data(), e.logs[level].get(), sizeof(ChunkLog) * ch.size(), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ch.size(); ++i)
+      out[i] = {ch[i].panel, ch[i].strip_begin, ch[i].strip_end, ch[i].tile_begin, ch[i].tile_end, hb[i],
+                hl[i].jobs, hl[i].out_tiles};
+  });
+}
+
+extern "C" int mg_walk_batch_panel_info(const mg_walk_batch* b, uint32_t p, uint32_t* graph, uint32_t* hops,
+                                        uint32_t* words_used, uint64_t* alive) {
+  return guarded([&] {
+    MG_REQUIRE(b != nullptr, "mg_walk_batch_panel_info: NULL batch");
+    MG_REQUIRE(p < b->e.panels.size(), "mg_walk_batch_panel_info: panel out of range");
+    const Panel& P = b->e.panels[p];
+    if (graph) *graph = P.graph;
+    if (hops) *hops = P.hops;
+    if (words_used) *words_used = static_cast<uint32_t>(P.words_used);
+    if (alive)
+      for (uint32_t k = 0; k < P.hops; ++k) alive[k] = P.alive[k];
+  });
+}
+
+extern "C" int mg_walk_batch_set_level_order(mg_walk_batch* b, uint32_t level, const uint32_t* order, uint64_t n) {
+  return guarded([&] {
+    MG_REQUIRE(b && order, "mg_walk_batch_set_level_order: NULL argument");
+    Engine& e = b->e;
+    MG_REQUIRE(level < e.max_hops, "mg_walk_batch_set_level_order: level out of range");
+    auto& ch = e.host_chunks[level];
+    MG_REQUIRE(n == ch.size(), "mg_walk_batch_set_level_order: order must list every chunk of the level");
+    std::vector<char> seen(ch.size(), 0);
+    std::vector<Chunk> next(ch.size());
+    for (uint64_t i = 0; i < n; ++i) {
+      MG_REQUIRE(order[i] < n && !seen[order[i]], "mg_walk_batch_set_level_order: order is not a permutation");
+      seen[order[i]] = 1;
+      next[i] = ch[order[i]];
+    }
+    DeviceGuard g(b->device);
+    MG_HIP(hipDeviceSynchronize());  // no run of this batch may still be reading the chunk list
+    ch.swap(next);
+    if (!ch.empty())
+      MG_HIP(hipMemcpy(e.chunks[level].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    if (!e.logs.empty()) MG_HIP(hipMemset(e.logs[level].get(), 0, e.logs[level].bytes()));
   });
 }

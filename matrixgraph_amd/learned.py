@@ -1,0 +1,211 @@
+"""Learned cost model for scheduling batched walks.
+
+The work of one hop level of a batch is a list of scheduler chunks (strip ranges of one
+panel's A^T tile stream), launched in list order; the launch is balanced best when the
+expensive chunks start first (longest-processing-time order). A chunk's cost is dominated
+by its gathers (jobs: A^T bits whose source vertex is non-empty at that level), which depend
+on how far the panel's queries have spread — unknown until the level runs. An MLP
+(n_in=8 -> 64 -> 64 -> 2, BF16 weights, FP32 accumulation, trained on the GPU by the
+library's own kernels) predicts, per chunk and level, the jobs and the output-tile occupancy
+from features known before the run:
+
+  f0 log1p(tiles)/10        f1 log1p(set bits)/12     f2 log1p(strips)/4
+  f3 alive words/64         f4 level/6                f5 log(avg degree)/4
+  f6 log2(vertices)/24      f7 set bits per tile/4
+
+Targets: log1p(jobs)/12 and the fraction of the chunk's 64 x strips output tiles that are
+numerically non-empty. Training samples are logged from the method's own runs
+(mg_walk_batch_set_logging). Predictions only order the launches, so results are
+bit-identical whatever the model says. The exact outcomes of the first run are the judge:
+a level whose predicted total jobs are off by more than `fallback_ratio` is re-ordered by the
+exact logged costs (fallback), and both the per-chunk misprediction rate and the fallback
+frequency are reported.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+N_FEATURES = 8
+N_TARGETS = 2
+# Relative chunk cost: jobs dominate; the scan of a tile and the write-out of a strip are
+# fixed costs (job equivalents, from the measured rates of the hop kernel).
+COST_PER_JOB = 1.0
+COST_PER_TILE = 0.35
+COST_PER_STRIP = 24.0
+
+
+def graph_stats(graphs, edge_counts):
+    out = []
+    for g, m in zip(graphs, edge_counts):
+        nv = g.n_strips * 64
+        out.append({"n_vertices": nv, "edges": int(m), "avg_degree": max(int(m), 1) / nv})
+    return out
+
+
+def level_features(batch, level: int, gstats, panel_cache=None):
+    """(records, X) for hop level `level` of a batch: X is float32 (n_chunks, 8)."""
+    rec = batch.level_chunks(level)
+    if panel_cache is None:
+        panel_cache = {}
+    n = len(rec)
+    X = np.zeros((n, N_FEATURES), np.float32)
+    if n == 0:
+        return rec, X
+    panels = rec["panel"]
+    alive_words = np.zeros(n, np.float32)
+    avg_deg = np.zeros(n, np.float32)
+    log_nv = np.zeros(n, np.float32)
+    for p in np.unique(panels):
+        if p not in panel_cache:
+            panel_cache[p] = batch.panel_info(int(p))
+        info = panel_cache[p]
+        sel = panels == p
+        a = int(info["alive"][level]) if level < info["hops"] else 0
+        alive_words[sel] = bin(a).count("1")
+        gs = gstats[info["graph"]]
+        avg_deg[sel] = gs["avg_degree"]
+        log_nv[sel] = np.log2(gs["n_vertices"])
+    tiles = (rec["tile_end"] - rec["tile_begin"]).astype(np.float32)
+    bits = rec["bits"].astype(np.float32)
+    strips = (rec["strip_end"] - rec["strip_begin"]).astype(np.float32)
+    X[:, 0] = np.log1p(tiles) / 10
+    X[:, 1] = np.log1p(bits) / 12
+    X[:, 2] = np.log1p(strips) / 4
+    X[:, 3] = alive_words / 64
+    X[:, 4] = level / 6
+    X[:, 5] = np.log(np.maximum(avg_deg, 1e-3)) / 4
+    X[:, 6] = log_nv / 24
+    X[:, 7] = bits / np.maximum(tiles, 1) / 4
+    return rec, X
+
+
+def level_targets(rec):
+    strips = (rec["strip_end"] - rec["strip_begin"]).astype(np.float32)
+    T = np.zeros((len(rec), N_TARGETS), np.float32)
+    T[:, 0] = np.log1p(rec["jobs"].astype(np.float32)) / 12
+    T[:, 1] = np.minimum(rec["out_tiles"].astype(np.float32) / np.maximum(strips * 64, 1), 1.0)
+    return T
+
+
+def chunk_cost(rec, jobs):
+    tiles = (rec["tile_end"] - rec["tile_begin"]).astype(np.float64)
+    strips = (rec["strip_end"] - rec["strip_begin"]).astype(np.float64)
+    return COST_PER_JOB * np.asarray(jobs, np.float64) + COST_PER_TILE * tiles + COST_PER_STRIP * strips
+
+
+def collect(ctx, graphs, gstats, queries_list, chunk_tiles=0, log=None):
+    """Runs each query batch once with logging; returns (X, T) over all chunk-levels."""
+    import matrixgraph_amd as M
+
+    xs, ts = [], []
+    for qs in queries_list:
+        b = M.WalkBatch(ctx, graphs, qs, chunk_tiles=chunk_tiles, use_graph=False)
+        b.set_logging(True)
+        b.run()
+        ctx.synchronize()
+        cache = {}
+        for k in range(b.info()[1]):
+            rec, X = level_features(b, k, gstats, cache)
+            xs.append(X)
+            ts.append(level_targets(rec))
+        b.close()
+        if log:
+            log(f"collect: {sum(len(x) for x in xs)} samples")
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def train(ctx, X, T, epochs=8, batch=4096, lr=2e-3, seed=1, log=None):
+    import matrixgraph_amd as M
+
+    mlp = M.MLP(ctx, N_FEATURES, N_TARGETS, seed=seed)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(X))
+    X, T = X[perm], T[perm]
+    l0, _ = mlp.loss_grad(X[:65536], T[:65536])
+    t0 = time.perf_counter()
+    l1 = mlp.train(X, T, epochs=epochs, batch=batch, lr=lr)
+    if log:
+        log(f"train: {len(X)} samples, {epochs} epochs, loss {l0:.5f} -> {l1:.5f} in {time.perf_counter() - t0:.1f}s")
+    return mlp, {"samples": int(len(X)), "epochs": epochs, "loss_initial": float(l0), "loss_final": float(l1)}
+
+
+def predict(mlp, X):
+    """(jobs, occupancy) predictions for feature rows X."""
+    if len(X) == 0:
+        return np.zeros(0), np.zeros(0)
+    Y = mlp.forward(X)
+    jobs = np.expm1(np.clip(Y[:, 0], 0, 2.0) * 12)
+    occ = np.clip(Y[:, 1], 0, 1)
+    return jobs, occ
+
+
+def order_by_cost(cost):
+    return np.argsort(-cost, kind="stable").astype(np.uint32)
+
+
+def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_ratio=2.0):
+    """Orders every level by predicted cost, runs once with logging to judge the predictions,
+    re-orders mispredicted levels by their exact costs, and returns the metrics."""
+    levels = batch.info()[1]
+    cache = {}
+    preds = []
+    for k in range(levels):
+        rec, X = level_features(batch, k, gstats, cache)
+        jobs, occ = predict(mlp, X)
+        o = order_by_cost(chunk_cost(rec, jobs))
+        batch.set_level_order(k, o)
+        preds.append((jobs[o], occ[o]))  # aligned with the new launch order
+    batch.set_logging(True)
+    batch.run()
+    ctx.synchronize()
+    n_chunks = n_bad = n_occ_bad = 0
+    fallbacks = 0
+    for k in range(levels):
+        rec = batch.level_chunks(k)
+        jobs_pred, occ_pred = preds[k]
+        actual = rec["jobs"].astype(np.float64)
+        ratio = (jobs_pred + 1.0) / (actual + 1.0)
+        n_chunks += len(rec)
+        n_bad += int(np.sum((ratio > mispredict_ratio) | (ratio < 1.0 / mispredict_ratio)))
+        strips = (rec["strip_end"] - rec["strip_begin"]).astype(np.float64)
+        occ_act = rec["out_tiles"] / np.maximum(strips * 64, 1)
+        n_occ_bad += int(np.sum(np.abs(occ_pred - occ_act) > 0.25))
+        tot_ratio = (jobs_pred.sum() + 1.0) / (actual.sum() + 1.0)
+        if tot_ratio > fallback_ratio or tot_ratio < 1.0 / fallback_ratio:
+            batch.set_level_order(k, order_by_cost(chunk_cost(rec, actual)))
+            fallbacks += 1
+    batch.set_logging(False)
+    return {
+        "chunk_levels": n_chunks,
+        "jobs_misprediction_rate": n_bad / max(n_chunks, 1),
+        "occupancy_misprediction_rate": n_occ_bad / max(n_chunks, 1),
+        "levels": levels,
+        "fallback_levels": fallbacks,
+        "fallback_frequency": fallbacks / max(levels, 1),
+        "misprediction_definition": f"predicted/actual jobs outside [1/{mispredict_ratio}, {mispredict_ratio}] "
+                                    "(+1 smoothing); occupancy off by more than 0.25",
+    }
+
+
+def schedule_exact(ctx, batch):
+    """Orders every level by the exact costs of a logged run (the fallback used for every level)."""
+    levels = batch.info()[1]
+    batch.set_logging(True)
+    batch.run()
+    ctx.synchronize()
+    for k in range(levels):
+        rec = batch.level_chunks(k)
+        batch.set_level_order(k, order_by_cost(chunk_cost(rec, rec["jobs"])))
+    batch.set_logging(False)
+
+
+def time_batch(ctx, batch, reps=3):
+    batch.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run()
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / reps
