@@ -218,6 +218,69 @@ def learned_section(M, ctx, train_specs, train_graphs, train_edges, chunk_tiles,
     }
 
 
+def dense_section(M, ctx, thresholds=(4097, 2048, 1024, 512, 0)):
+    """Density-mixed 3-hop walk: R-MAT scale 20 (edge factor 16, seed 3) union 64 planted
+    communities of 4096 vertices (community c = vertices [16384c, 16384c + 4096), p = 0.3),
+    4096 sources (seed 3). The gather walk is timed, then the same walk by tile products
+    X_{k+1} = X_k * A with the matrix-core path for tile pairs whose two tiles both have at
+    least `min_bits` set bits (4097 = vector path only); every variant's X_3 is compared with
+    the gather walk's export."""
+    from matrixgraph_amd import workloads as WL
+
+    scale, hops, n_src = 20, 3, 4096
+    n = 1 << scale
+    t0 = time.perf_counter()
+    el = WL.union_graph(ctx, scale, 3, 64, 4096, 16384)
+    _, m = el.info()
+    at = el.to_tiled_matrix(transpose=True)
+    el.close()
+    sources = M.sample_sources(n, n_src, 3)
+    log(f"dense: union graph {m} edges, {at.n_nz_tile} tiles, built in {time.perf_counter() - t0:.1f}s")
+    w = M.Walk(ctx, at, n_src, hops)
+    w.run(sources, hops)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        w.run(sources, hops)
+    ctx.synchronize()
+    gather_ms = (time.perf_counter() - t0) / 3 * 1e3
+    ref = w.export(hops)
+    ref_digest = ref.digest()
+    tp_total = sum(w.hop_stats(k)["tile_products"] for k in range(1, hops + 1))
+    ref.close()
+    w.close()
+    variants = []
+    for mb in thresholds:
+        xs, stats = WL.product_walk(ctx, at, n, sources, hops, mb)
+        ok = xs[-1].digest() == ref_digest
+        num_ms = sum(st["numeric_ms"] for st in stats)
+        sym_ms = sum(st["symbolic_ms"] for st in stats)
+        dense = sum(st["dense_products"] for st in stats)
+        tp = sum(st["tile_products"] for st in stats)
+        for x in xs:
+            x.close()
+        log(f"dense: min_bits {mb}: numeric {num_ms:.1f} ms, symbolic {sym_ms:.1f} ms, matrix-core products "
+            f"{dense}/{tp}, bit-exact={ok}")
+        if not ok:
+            raise SystemExit("dense: tile-product walk differs from the gather walk")
+        variants.append({"min_bits": mb, "numeric_ms": round(num_ms, 3), "symbolic_ms": round(sym_ms, 3),
+                         "matrix_core_products": dense, "tile_products": tp,
+                         "tile_products_per_s_numeric": round(tp / (num_ms * 1e-3), 1)})
+    at.close()
+    best = min(variants, key=lambda v: v["numeric_ms"])
+    return {
+        "workload": "3-hop walk from 4096 sources on R-MAT scale 20 (edge factor 16, seed 3) union 64 communities "
+                    "of 4096 vertices at stride 16384, p = 1288490189/2^32, seed 3",
+        "edges": m,
+        "gather_walk_ms": round(gather_ms, 3),
+        "gather_walk_source_walks_per_s": round(n_src / (gather_ms * 1e-3), 1),
+        "tile_products": tp_total,
+        "product_walk_variants": variants,
+        "best_min_bits": best["min_bits"],
+        "bit_exact": True,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +295,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
+    ap.add_argument("--no-dense", action="store_true", help="skip the density-mixed (matrix-core) section")
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
@@ -355,6 +419,8 @@ def main():
         threads = O.max_threads()
         result["cpu_baseline"] = cpu_baseline(M, O, ctx, w, at, n_edges, args.hops, args.sources, args.cpu_stride,
                                               threads)
+    if not args.no_dense:
+        result["dense"] = dense_section(M, ctx)
     if not args.no_batch:
         w.close()
         result["batch"], specs, graphs, edges, best = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)

@@ -264,6 +264,14 @@ def multiply_numeric(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: TiledMatr
     _check(lib().mg_multiply_numeric(ctx.h, a.h, bt.h, c.h), "mg_multiply_numeric")
 
 
+def multiply_numeric_dense(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: TiledMatrix, min_bits: int) -> int:
+    """Numeric pass with matrix-core products for tile pairs whose tiles both have at least
+    min_bits set bits; returns the number of such products."""
+    n = C.c_uint64()
+    _check(lib().mg_multiply_numeric_dense(ctx.h, a.h, bt.h, c.h, min_bits, C.byref(n)), "mg_multiply_numeric_dense")
+    return n.value
+
+
 def multiply(ctx: Context, a: TiledMatrix, bt: TiledMatrix):
     """C = A * B (B supplied transposed): symbolic then numeric."""
     c, st = multiply_symbolic(ctx, a, bt)

@@ -73,6 +73,13 @@ __device__ inline uint64_t shfl_xor64(uint64_t v, int mask) {
   const uint32_t hi = shfl_xor32(static_cast<uint32_t>(v >> 32), mask);
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
+// Value of v in lane src (per-lane src).
+__device__ inline uint64_t shfl64(uint64_t v, uint32_t src) {
+  const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(src), kWave));
+  const uint32_t hi =
+      static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(src), kWave));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 // Bitwise OR over the 64 lanes (result in every lane).
 __device__ inline uint64_t wave_or(uint64_t v) {
 #pragma unroll

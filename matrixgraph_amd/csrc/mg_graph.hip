@@ -140,50 +140,47 @@ __global__ void __launch_bounds__(256) bernoulli_rows_kernel(BernoulliSpec s, ui
 }
 
 __global__ void widen_kernel(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[i];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) out[i] = in[i];
 }
 
 __global__ void pack_edges_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t n,
                                   uint64_t* __restrict__ out) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (static_cast<uint64_t>(src[i]) << 32) | dst[i];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) out[i] = (static_cast<uint64_t>(src[i]) << 32) | dst[i];
 }
 
 // Tile sort key: (tile_row * n_strips + tile_col) << 12 | (row_in_tile << 6 | col_in_tile).
 __global__ void tile_key_kernel(const uint64_t* __restrict__ edges, uint64_t n, uint32_t n_strips, int transpose,
                                 uint64_t* __restrict__ out) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t a = static_cast<uint32_t>(edges[i] >> 32);
-  uint32_t b = static_cast<uint32_t>(edges[i]);
-  if (transpose) { const uint32_t t = a; a = b; b = t; }
-  const uint64_t tile = static_cast<uint64_t>(a >> 6) * n_strips + (b >> 6);
-  out[i] = (tile << 12) | ((a & 63u) << 6) | (b & 63u);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t a = static_cast<uint32_t>(edges[i] >> 32);
+    uint32_t b = static_cast<uint32_t>(edges[i]);
+    if (transpose) { const uint32_t t = a; a = b; b = t; }
+    const uint64_t tile = static_cast<uint64_t>(a >> 6) * n_strips + (b >> 6);
+    out[i] = (tile << 12) | ((a & 63u) << 6) | (b & 63u);
+  }
 }
 
 __global__ void tile_flag_kernel(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flag) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) flag[i] = (i == 0 || (keys[i] >> 12) != (keys[i - 1] >> 12)) ? 1u : 0u;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) flag[i] = (i == 0 || (keys[i] >> 12) != (keys[i - 1] >> 12)) ? 1u : 0u;
 }
 
 __global__ void tile_fill_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
                                  const uint32_t* __restrict__ incl, uint64_t n, uint32_t n_strips,
                                  uint32_t* __restrict__ row_idx, uint32_t* __restrict__ col_idx,
                                  uint32_t* __restrict__ row_cnt, unsigned long long* __restrict__ data) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = keys[i];
-  const uint32_t t = incl[i] - 1;
-  const uint64_t tile = k >> 12;
-  if (flag[i]) {
-    const uint32_t x = static_cast<uint32_t>(tile / n_strips);
-    row_idx[t] = x;
-    col_idx[t] = static_cast<uint32_t>(tile % n_strips);
-    atomicAdd(row_cnt + x, 1u);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    const uint32_t t = incl[i] - 1;
+    const uint64_t tile = k >> 12;
+    if (flag[i]) {
+      const uint32_t x = static_cast<uint32_t>(tile / n_strips);
+      row_idx[t] = x;
+      col_idx[t] = static_cast<uint32_t>(tile % n_strips);
+      atomicAdd(row_cnt + x, 1u);
+    }
+    const uint32_t r = static_cast<uint32_t>((k >> 6) & 63), c = static_cast<uint32_t>(k & 63);
+    atomicOr(data + static_cast<uint64_t>(t) * kTile + r, 1ull << c);
   }
-  const uint32_t r = static_cast<uint32_t>((k >> 6) & 63), c = static_cast<uint32_t>(k & 63);
-  atomicOr(data + static_cast<uint64_t>(t) * kTile + r, 1ull << c);
 }
 
 __global__ void __launch_bounds__(256) edge_digest_kernel(const uint64_t* __restrict__ keys, uint64_t n,

@@ -21,8 +21,7 @@ __global__ void __launch_bounds__(256) popcount_kernel(const uint64_t* __restric
 
 __global__ void mark_dups_kernel(const uint64_t* __restrict__ keys, uint64_t n,
                                  uint32_t* __restrict__ keep) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) keep[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) keep[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
 }  // namespace

@@ -31,6 +31,62 @@ __device__ inline uint64_t tile_product(uint64_t l_row, uint64_t rt_row) {
   return out;
 }
 
+// Dense tile pairs on the matrix cores. Out(r, c) = OR_k L(r, k) AND R(k, c) with
+// R(k, c) = bit k of Rt[c], computed as int8 products of 0/1 bytes accumulated in int32
+// (v_mfma_i32_32x32x16_i8: a 32 x 32 output block per instruction, K = 16), then count > 0:
+// four 32 x 32 blocks, four K-steps each. Operand lane layout (32x32x16 i8): lane l holds
+// A[l % 32][8 * (l / 32) + j] and B[8 * (l / 32) + j][l % 32], j = 0..7; result register v of
+// lane l is C[8 * (v / 4) + 4 * (l / 32) + v % 4][l % 32].
+typedef int mg_v16i __attribute__((ext_vector_type(16)));
+
+// Bits k0..k0+7 of x as eight bytes of value 0 or 1 (byte j = bit k0 + j).
+__device__ inline long bits_to_bytes(uint64_t x, uint32_t k0) {
+  uint64_t b = (x >> k0) & 0xFFull;
+  b *= 0x0101010101010101ull;   // the byte in every byte position
+  b &= 0x8040201008040201ull;   // byte j keeps bit j
+  b = (b + 0x7F7F7F7F7F7F7F7Full) & 0x8080808080808080ull;  // non-zero byte -> 0x80
+  return static_cast<long>(b >> 7);
+}
+
+// Same contract as tile_product: lane r holds L[r] and Rt[r]; returns output row `lane`.
+__device__ inline uint64_t tile_product_mfma(uint64_t l_row, uint64_t rt_row, uint32_t lane) {
+  const uint32_t lo = lane & 31, hi = lane >> 5;
+  uint64_t out = 0;
+#pragma unroll
+  for (uint32_t mb = 0; mb < 2; ++mb) {
+    const uint64_t a_src = shfl64(l_row, mb * 32 + lo);
+#pragma unroll
+    for (uint32_t nb = 0; nb < 2; ++nb) {
+      const uint64_t b_src = shfl64(rt_row, nb * 32 + lo);
+      mg_v16i acc = {};
+#pragma unroll
+      for (uint32_t st = 0; st < 4; ++st) {
+        const uint32_t k0 = 16 * st + 8 * hi;
+        acc = __builtin_amdgcn_mfma_i32_32x32x16_i8(bits_to_bytes(a_src, k0), bits_to_bytes(b_src, k0), acc, 0, 0,
+                                                     0);
+      }
+#pragma unroll
+      for (uint32_t v = 0; v < 16; ++v) {
+        const uint64_t bal = ballot(acc[v] > 0);
+        const uint32_t i0 = mb * 32 + 8 * (v / 4) + (v % 4);
+        if (lane == i0) out |= (bal & 0xFFFFFFFFull) << (nb * 32);
+        if (lane == i0 + 4) out |= (bal >> 32) << (nb * 32);
+      }
+    }
+  }
+  return out;
+}
+
+// Set bits of every tile (for the dense-pair dispatch).
+__global__ void __launch_bounds__(kBlock) tile_bits_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                           uint32_t* __restrict__ bits) {
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
+    const uint64_t c = wave_sum(__popcll(data[static_cast<uint64_t>(i) * kTile + lane_id()]));
+    if (lane_id() == 0) bits[i] = static_cast<uint32_t>(c);
+  }
+}
+
 // Index of key in sorted a[lo, hi), or -1.
 __device__ inline int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t key) {
   uint32_t l = lo, h = hi;
@@ -49,8 +105,12 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
     uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
-    uint64_t* __restrict__ data_c) {
+    uint64_t* __restrict__ data_c, const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt,
+    uint32_t min_bits, unsigned long long* __restrict__ dense_count) {
+  // bits_a / bits_bt (may be NULL: VALU only): tile pairs whose two tiles both have at least
+  // min_bits set bits take the matrix-core product.
   const uint32_t lane = lane_id();
+  uint64_t dense = 0;
   const uint32_t n_waves = gridDim.x * kWavesPerBlock;
   for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_c; i += n_waves) {
     const uint32_t x = row_c[i];
@@ -77,12 +137,18 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
         const uint64_t ib = a_short ? jl : js;
         const uint64_t l_row = data_a[ia * kTile + lane];
         const uint64_t rt_row = data_bt[ib * kTile + lane];
-        acc |= tile_product(l_row, rt_row);
+        if (bits_a && bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits) {
+          acc |= tile_product_mfma(l_row, rt_row, lane);
+          ++dense;
+        } else {
+          acc |= tile_product(l_row, rt_row);
+        }
       }
     }
     uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
     if (acc) *dst |= acc;
   }
+  if (dense_count && lane == 0 && dense) atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(dense));
 }
 
 // len (<= 64) bits of a bit-packed array starting at bit `start`.
@@ -169,29 +235,25 @@ __global__ void __launch_bounds__(kBlock) layout_emit_kernel(const uint64_t* __r
 }
 
 __global__ void add_counts_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] += src[i];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) dst[i] += src[i];
 }
 
 // --- symbolic pass ---------------------------------------------------------
 
 // Row form of B from Bt: histogram of Bt's column index.
 __global__ void col_hist_kernel(const uint32_t* __restrict__ col, uint32_t n, uint32_t* __restrict__ hist) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(hist + col[i], 1u);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) atomicAdd(hist + col[i], 1u);
 }
 __global__ void col_scatter_kernel(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, uint32_t n,
                                    const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
                                    uint32_t* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t t = col[i];
     out[start[t] + atomicAdd(cursor + t, 1u)] = row[i];
   }
 }
 __global__ void mark_referenced_kernel(const uint32_t* __restrict__ col, uint32_t n, uint8_t* __restrict__ ref) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ref[col[i]] = 1;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) ref[col[i]] = 1;
 }
 __global__ void __launch_bounds__(kBlock) used_right_kernel(const uint8_t* __restrict__ ref,
                                                             const uint32_t* __restrict__ off_b, uint32_t n,
@@ -301,10 +363,12 @@ constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
 void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const uint32_t* off_bt,
                        const uint32_t* row_c, const uint32_t* col_c, const uint32_t* col_a,
                        const uint32_t* col_bt, const uint64_t* data_a, const uint64_t* data_bt,
-                       uint64_t* data_c) {
+                       uint64_t* data_c, const uint32_t* bits_a = nullptr, const uint32_t* bits_bt = nullptr,
+                       uint32_t min_bits = 0, unsigned long long* dense_count = nullptr) {
   if (n_c == 0) return;
-  fill_tiles_kernel<<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(n_c, off_a, off_bt, row_c, col_c, col_a,
-                                                                            col_bt, data_a, data_bt, data_c);
+  fill_tiles_kernel<<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+      n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, bits_a, bits_bt, min_bits,
+      dense_count);
   MG_LAUNCH_CHECK();
 }
 
@@ -460,5 +524,40 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     launch_fill_tiles(ctx->stream, c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(),
                       c->col_idx.get(), a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(),
                       c->data.get());
+  });
+}
+
+extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                         mg_tiled_matrix* c, uint32_t min_bits, uint64_t* dense_products) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && c, "mg_multiply_numeric_dense: NULL argument");
+    MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
+               "mg_multiply_numeric_dense: operands differ in n_strips");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    DevBuf<uint32_t> ba(std::max<uint32_t>(1, a->n_nz_tile)), bb(std::max<uint32_t>(1, bt->n_nz_tile));
+    if (a->n_nz_tile) {
+      tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
+                                                                                         ba.get());
+      MG_LAUNCH_CHECK();
+    }
+    if (bt->n_nz_tile) {
+      tile_bits_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(),
+                                                                                          bt->n_nz_tile, bb.get());
+      MG_LAUNCH_CHECK();
+    }
+    unsigned long long* cnt = ctx->counters.get();  // 64 slots
+    MG_HIP(hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned long long), st));
+    launch_fill_tiles(st, c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(),
+                      c->col_idx.get(), a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(),
+                      c->data.get(), ba.get(), bb.get(), min_bits, cnt);
+    unsigned long long h[64];
+    MG_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    if (dense_products) {
+      uint64_t sum = 0;
+      for (unsigned long long v : h) sum += v;
+      *dense_products = sum;
+    }
   });
 }

@@ -165,16 +165,16 @@ __global__ void __launch_bounds__(kBlock) clear_x0_kernel(const PanelInit* __res
 
 __global__ void init_sources_kernel(const SourceItem* __restrict__ items, uint32_t n_items,
                                     const PanelInit* __restrict__ panels, uint32_t W) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_items) return;
-  const SourceItem it = items[i];
-  const PanelInit p = panels[it.panel];
-  const uint32_t v = it.vertex;
-  atomicOr(reinterpret_cast<unsigned long long*>(p.x0) + static_cast<uint64_t>(v) * W + it.col / 64,
-           1ull << (it.col % 64));
-  atomicOr(reinterpret_cast<unsigned long long*>(p.s0) + v / 64, 1ull << (it.col / 64));
-  atomicOr(reinterpret_cast<unsigned long long*>(p.n0) + v / 64, 1ull << (it.col / 64));
-  atomicOr(reinterpret_cast<unsigned long long*>(p.v0) + v / 64, 1ull << (v % 64));
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
+    const SourceItem it = items[i];
+    const PanelInit p = panels[it.panel];
+    const uint32_t v = it.vertex;
+    atomicOr(reinterpret_cast<unsigned long long*>(p.x0) + static_cast<uint64_t>(v) * W + it.col / 64,
+             1ull << (it.col % 64));
+    atomicOr(reinterpret_cast<unsigned long long*>(p.s0) + v / 64, 1ull << (it.col / 64));
+    atomicOr(reinterpret_cast<unsigned long long*>(p.n0) + v / 64, 1ull << (it.col / 64));
+    atomicOr(reinterpret_cast<unsigned long long*>(p.v0) + v / 64, 1ull << (v % 64));
+  }
 }
 
 // Row-occupancy mask of every A^T tile: bit r set when word r (target row r) is non-zero.

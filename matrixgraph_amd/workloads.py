@@ -85,3 +85,71 @@ def batch_queries(graph_specs, n_queries: int = 512, seed: int = 1, max_sources:
         src = M.sample_sources(graph_specs[g].n_vertices, size, qseed)
         out.append(QuerySpec(g, src, hops))
     return out
+
+
+# ----------------------------------------------------------------------------
+# Density-mixed union graph and walks by tile products
+# ----------------------------------------------------------------------------
+# Planted communities: community c is vertices [c*stride, c*stride + size), tile-aligned and
+# spread over the vertex range; intra-community edge probability p = 1288490189 / 2^32
+# (0.3 to 10 digits; the generator draws exact dyadic Bernoulli bits).
+COMMUNITY_P_NUM = 1288490189
+COMMUNITY_P_LOG2 = 32
+
+
+def union_graph(ctx, scale: int, seed: int, n_comm: int, comm_size: int, stride: int):
+    """R-MAT (edge factor 16) union planted dense communities; returns the edge list."""
+    import matrixgraph_amd as M
+
+    rm = M.EdgeList.rmat(ctx, scale, 16, *RMAT_ABC, seed)
+    cm = M.EdgeList.communities(ctx, 1 << scale, n_comm, comm_size, stride, COMMUNITY_P_NUM, COMMUNITY_P_LOG2, seed)
+    u = rm.union(cm)
+    rm.close()
+    cm.close()
+    return u
+
+
+def union_graph_oracle(scale: int, seed: int, n_comm: int, comm_size: int, stride: int):
+    """The same union graph from the CPU oracle (test use only)."""
+    import oracle as O
+
+    s1, d1 = O.rmat_edges(scale, 16, *RMAT_ABC, seed)
+    s2, d2 = O.community_edges(1 << scale, n_comm, comm_size, stride, COMMUNITY_P_NUM, COMMUNITY_P_LOG2, seed)
+    key = np.unique(np.concatenate([s1.astype(np.uint64) << np.uint64(32) | d1, s2.astype(np.uint64) << np.uint64(32) | d2]))
+    return (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def sources_matrix(ctx, n_vertices: int, sources):
+    """X_0 as a bit-tiled matrix in the reference layout: row s (0..len-1) has the single bit
+    at column sources[s] (padded-square n x n grid, tile rows 0..ceil(len/64)-1 used)."""
+    import matrixgraph_amd as M
+
+    src = np.ascontiguousarray(sources, np.uint32)
+    el = M.EdgeList.upload(ctx, n_vertices, np.arange(len(src), dtype=np.uint32), src)
+    x = el.to_tiled_matrix(transpose=False)
+    el.close()
+    return x
+
+
+def product_walk(ctx, at, n_vertices: int, sources, hops: int, min_bits: int):
+    """X_{k+1} = X_k * A by tile products (symbolic pass, then the numeric pass with
+    matrix-core products for tile pairs whose tiles both have >= min_bits bits).
+    Returns the list X_0..X_hops and per-hop stats (tile products, matrix-core products,
+    symbolic and numeric milliseconds)."""
+    import time
+
+    import matrixgraph_amd as M
+
+    xs = [sources_matrix(ctx, n_vertices, sources)]
+    stats = []
+    for _ in range(hops):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        c, st = M.multiply_symbolic(ctx, xs[-1], at)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        dense = M.multiply_numeric_dense(ctx, xs[-1], at, c, min_bits)
+        t2 = time.perf_counter()
+        stats.append(dict(st, dense_products=dense, symbolic_ms=(t1 - t0) * 1e3, numeric_ms=(t2 - t1) * 1e3))
+        xs.append(c)
+    return xs, stats

@@ -88,3 +88,18 @@ def test_tile_upload_roundtrip_and_validation(ctx):
     # empty matrix
     empty = M.TiledMatrix.upload(ctx, M.HostTiledMatrix.empty(4))
     assert empty.n_nz_tile == 0 and empty.popcount() == 0
+
+
+def test_edge_lists_beyond_2_28_entries(ctx):
+    """40 communities of 4096 vertices at p = 1/2: about 335 M edges in one list, past the
+    2^28 threads of a 2^20-block grid (element-wise kernels must stride over the grid)."""
+    M = mg()
+    n, n_comm, size = 40 * 4096, 40, 4096
+    el = M.EdgeList.communities(ctx, n, n_comm, size, size, 1, 1, 5)
+    _, m = el.info()
+    assert m > (1 << 28)
+    s, d = O.community_edges(n, n_comm, size, size, 1, 1, 5)
+    assert m == len(s)
+    assert el.digest() == O.edges_digest(s, d)
+    t = el.to_tiled_matrix()
+    assert t.popcount() == m
