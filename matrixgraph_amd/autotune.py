@@ -18,6 +18,7 @@ DEFAULT_CANDIDATES = [
     {"packing": 0, "chunk_tiles": 512},
     {"packing": 0, "chunk_tiles": 1024},
     {"packing": 0, "chunk_tiles": 2048},
+    {"packing": 0, "chunk_tiles": 4096},
 ]
 # MG_PACK_ONE_PER_PANEL is not a default candidate: every query then holds 4096-source
 # frontiers for each of its levels, which does not fit for hundreds of queries on large graphs.
