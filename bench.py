@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
+    ap.add_argument("--sweep-first-hop", action="store_true",
+                    help="sweep every strip at the first hop too (no push from the source strips)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (5 times): the known-size reference for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling")
@@ -327,7 +329,8 @@ def main():
     ns, n_tiles, _ = at.metadata()
     log(f"graph: 2^{args.scale} vertices, {n_edges} edges, {n_tiles} A^T tiles, setup {time.perf_counter() - t0:.1f}s")
     sources = M.sample_sources(n_vertices, args.sources, args.seed)
-    w = M.Walk(ctx, at, args.sources, args.hops, M.WALK_PRUNE if args.prune else 0)
+    w = M.Walk(ctx, at, args.sources, args.hops,
+               (M.WALK_PRUNE if args.prune else 0) | (M.WALK_SWEEP_FIRST_HOP if args.sweep_first_hop else 0))
 
     for _ in range(args.warmup):
         w.run(sources, args.hops)
@@ -346,32 +349,38 @@ def main():
     total_tp = 0
     sum_bytes = 0
     sum_numeric_ms = 0.0
+    n_launch = 0
     for k in range(1, args.hops + 1):
         st = w.hop_stats(k)
         sym_ms, num_ms, n_runs = w.kernel_times(k)
         avg_num = num_ms / max(n_runs, 1)
         avg_sym = sym_ms / max(n_runs, 1)
         total_tp += st["tile_products"]
-        sum_bytes += st["alg_bytes"]
-        sum_numeric_ms += avg_num
+        sweep = st["kernel"] == M.HOP_SWEEP
+        if sweep:  # the roofline kernel: hops that sweep every strip (walk_numeric64_kernel)
+            sum_bytes += st["alg_bytes"]
+            sum_numeric_ms += avg_num
+            n_launch += 1
         per_hop.append({
-            "hop": k, "tile_products": st["tile_products"], "alg_bytes": st["alg_bytes"],
+            "hop": k, "kernel": "sweep" if sweep else "push", "tile_products": st["tile_products"],
+            "alg_bytes": st["alg_bytes"],
             "left_tiles": st["left_tiles"], "right_tiles_used": st["right_tiles_used"],
             "out_tiles": st["out_tiles"], "out_tiles_numeric": st["out_tiles_numeric"],
             "popcount": st["popcount"], "edges_visited": st["edges_visited"],
             "numeric_kernel_ms": round(avg_num, 4), "symbolic_ms": round(avg_sym, 4),
             "numeric_GBps_alg": round(st["alg_bytes"] / (avg_num * 1e-3) / 1e9, 1) if avg_num > 0 else None,
         })
-        log(f"hop {k}: TP={st['tile_products']} B_alg={st['alg_bytes'] / 1e9:.2f}GB numeric={avg_num:.3f}ms "
-            f"sym={avg_sym:.3f}ms popcount={st['popcount']}")
-    n_launch = args.hops
+        log(f"hop {k} ({'sweep' if sweep else 'push'}): TP={st['tile_products']} B_alg={st['alg_bytes'] / 1e9:.2f}GB "
+            f"kernels={avg_num:.3f}ms popcount={st['popcount']}")
+    n_launch = max(n_launch, 1)
     achieved = (sum_bytes / n_launch) / ((sum_numeric_ms / n_launch) * 1e-3) / 1e9 if sum_numeric_ms > 0 else 0.0
     traffic = None
     tfile = ROOT / "profiles" / "walk_numeric_rmat22_traffic.json"
     if tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune:
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
     roofline = {
-        "kernel": "walk_numeric_kernel<64>",
+        "kernel": "walk_numeric64_kernel",
+        "launches_per_walk": sum(1 for h in per_hop if h["kernel"] == "sweep"),
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,

@@ -256,8 +256,14 @@ int mg_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t seed, uint32
  * ------------------------------------------------------------------------- */
 
 enum {
-  MG_WALK_PRUNE = 1u  /* structural tile set of X_{k+1} built from numerically
-                         non-empty X_k tiles instead of structural ones */
+  MG_WALK_PRUNE = 1u,          /* structural tile set of X_{k+1} built from numerically
+                                  non-empty X_k tiles instead of structural ones */
+  MG_WALK_SWEEP_FIRST_HOP = 2u /* never push the first hop: every hop sweeps all strips
+                                  (no column index of A^T is built) */
+};
+enum {
+  MG_HOP_SWEEP = 0u, /* the hop swept every strip of A^T (walk_numeric kernels) */
+  MG_HOP_PUSH = 1u   /* first hop pushed from the source strips' A^T columns */
 };
 
 typedef struct mg_hop_stats {
@@ -270,12 +276,17 @@ typedef struct mg_hop_stats {
   uint64_t alg_bytes;         /* 520*(T_L+T_R,used+T_C) + 4*(2*rows+2), rows = tile rows of X */
   uint64_t edges_visited;     /* set bits of A read by the numeric pass */
   double symbolic_ms;         /* 0: the structural pass runs inside the numeric kernel */
-  double numeric_ms;          /* hop kernel (structural + numeric) of the last logged run, else 0 */
+  double numeric_ms;          /* hop kernels (structural + numeric) of the last logged run, else 0 */
+  uint32_t kernel;            /* MG_HOP_SWEEP or MG_HOP_PUSH */
+  uint32_t reserved;
 } mg_hop_stats;
 
 /* Allocates X_0..X_max_hops (W = ceil(max_sources/64) rounded up to a power of two,
- * max_sources <= 4096) plus the strip scheduler and the per-tile row-occupancy masks of
- * at. at must outlive w and keep its contents (the masks are taken here). */
+ * max_sources <= 4096) plus the strip scheduler, the per-tile row-occupancy masks of at
+ * and (unless MG_WALK_SWEEP_FIRST_HOP) a column index of at (16 bytes per tile). at must
+ * outlive w and keep its contents (the masks are taken here).
+ * The first hop of a run is pushed from the source strips when their columns of at hold at
+ * most half of its tiles; results and statistics are the same either way. */
 int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sources,
                    uint32_t max_hops, uint32_t flags, mg_walk** out);
 /* Runs hops 1..hops from the given distinct sources (async; stats after sync). */

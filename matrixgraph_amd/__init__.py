@@ -360,6 +360,9 @@ def sample_sources(n_vertices: int, count: int, seed: int) -> np.ndarray:
 
 
 WALK_PRUNE = 1
+WALK_SWEEP_FIRST_HOP = 2
+HOP_SWEEP = 0
+HOP_PUSH = 1
 
 
 class Walk:

@@ -37,6 +37,8 @@ class HopStats(C.Structure):
         ("edges_visited", u64),
         ("symbolic_ms", f64),
         ("numeric_ms", f64),
+        ("kernel", u32),
+        ("reserved", u32),
     ]
 
 

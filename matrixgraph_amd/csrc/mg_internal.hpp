@@ -171,6 +171,7 @@ void alloc_tiled_matrix(mg_tiled_matrix* m, uint32_t n_strips, uint32_t n_nz_til
 uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words);
 // Exclusive scan of n u32 counts into out[0..n] (out[n] = total), on ctx->stream.
 void exclusive_scan_u32(mg_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n);
+void exclusive_scan_u32_on(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n);
 // Sort u64 keys ascending and drop duplicates; returns the unique count.
 uint64_t sort_unique_u64(mg_context* ctx, DevBuf<uint64_t>& keys, uint64_t n, int end_bit);
 

@@ -39,14 +39,18 @@ uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words
   return h;
 }
 
-void exclusive_scan_u32(mg_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+void exclusive_scan_u32_on(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n) {
   // out[0..n-1] = exclusive prefix, out[n] = total: scan n+1 items with a zero appended
   // is avoided by scanning n items then adding the last count on device.
   size_t temp_bytes = 0;
-  MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, in, out + 1, n, ctx->stream));
+  MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, in, out + 1, n, st));
   DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
-  MG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), ctx->stream));
-  if (n) MG_HIP(hipcub::DeviceScan::InclusiveSum(temp.get(), temp_bytes, in, out + 1, n, ctx->stream));
+  MG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), st));
+  if (n) MG_HIP(hipcub::DeviceScan::InclusiveSum(temp.get(), temp_bytes, in, out + 1, n, st));
+}
+
+void exclusive_scan_u32(mg_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+  exclusive_scan_u32_on(ctx->stream, in, out, n);
 }
 
 uint64_t sort_unique_u64(mg_context* ctx, DevBuf<uint64_t>& keys, uint64_t n, int end_bit) {

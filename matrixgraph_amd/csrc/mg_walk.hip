@@ -76,9 +76,11 @@ enum BlockCounter { bTP = 0, bUsed, bLeft, bOut, bEdges, bPop, bNumeric, kBlockC
 struct PanelLevel {
   const uint32_t* off;
   const uint32_t* col;
+  const uint32_t* row;      // per A^T tile: its strip (row index)
   const uint64_t* data;
   const uint64_t* rmask;    // per A^T tile: bit r = row r non-zero
   const uint64_t* payload;  // per A^T tile: compact bit list (see tile_payload_kernel)
+  const uint64_t* dense;    // rows of the dense A^T tiles, 64 words each, at their payload slot
   const uint64_t* x_in;
   const uint64_t* v_in;
   const uint64_t* s_in;
@@ -190,8 +192,9 @@ __global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __rest
 
 // Compact payload of an A^T tile with at most kPayloadMax set bits: entry i = (row << 6) |
 // column in bits 12i..12i+11 and the entry count in bits 48..50. A tile with more bits has
-// kPayloadDense set, its bit count in the low 13 bits, and is read row by row through its
-// row-occupancy mask.
+// kPayloadDense set, its bit count in the low 13 bits and, once dense_compact_kernel has run,
+// the index of its copy in the dense-row array in bits 13..44; it is read row by row through
+// its row-occupancy mask.
 constexpr uint32_t kPayloadMax = 4;
 constexpr uint64_t kPayloadDense = 1ull << 63;
 
@@ -217,6 +220,30 @@ __global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __
       pl |= static_cast<uint64_t>(n) << 48;
     }
     if (lane == 0) payload[i] = pl;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) dense_flag_kernel(const uint64_t* __restrict__ payload, uint32_t n,
+                                                            uint32_t* __restrict__ flag) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flag[i] = (payload[i] & kPayloadDense) ? 1u : 0u;
+}
+
+// Copies the rows of every dense tile to its slot pos[i] of a compact array (64 words per
+// tile) and records the slot in the payload: the hop kernels then read dense rows from a
+// few hundred MB instead of the whole tile store.
+__global__ void __launch_bounds__(kBlock) dense_compact_kernel(const uint64_t* __restrict__ data, uint32_t n,
+                                                               const uint32_t* __restrict__ pos,
+                                                               uint64_t* __restrict__ payload,
+                                                               uint64_t* __restrict__ dense) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n; i += n_waves) {
+    const uint64_t pl = payload[i];
+    if (!(pl & kPayloadDense)) continue;
+    const uint64_t slot = pos[i];
+    dense[slot * kTile + lane] = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
+    if (lane == 0) payload[i] = (pl & ~(0xFFFFFFFFull << 13)) | (slot << 13);
   }
 }
 
@@ -546,6 +573,10 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
 // target row * 8, so the drain needs no index arithmetic beyond two masks. The list is
 // padded to whole drain steps with jobs that read the frontier's extra all-zero row (vertex
 // n_vertices, allocated past the last strip and never written), so no slot is tested.
+#ifndef MG_DENSE_BATCH
+#define MG_DENSE_BATCH 2
+#endif
+constexpr int kDenseBatch = MG_DENSE_BATCH;  // dense A^T tiles whose rows a wave loads together
 constexpr int kJobCap = 232;     // jobs per wave list (a multiple of 2 * kDrainUnroll)
 constexpr int kDrainUnroll = 4;  // job pairs per drain step (two steps in flight)
 
@@ -661,8 +692,8 @@ struct WaveTileState {
 
 // Decodes one group of 64 tiles (lane = tile) into jobs.
 __device__ inline void decode_group(WaveTileState& st, uint32_t p, bool valid, uint32_t t_l, uint64_t pl,
-                                    uint64_t vm_l, uint64_t sm_l, gptr<const uint64_t> __restrict__ data,
-                                    gptr<const uint64_t> __restrict__ rmask, gptr<const uint64_t> __restrict__ x_in,
+                                    uint64_t rm_l, uint64_t vm_l, uint64_t sm_l, gptr<const uint64_t> __restrict__ dense_rows,
+                                    gptr<const uint64_t> __restrict__ x_in,
                                     unsigned long long* __restrict__ acc, uint32_t lane, uint64_t keep0,
                                     uint64_t keep1, bool all_alive, unsigned long long pad_job,
                                     uint32_t (&cnt)[kBlockCounters]) {
@@ -690,18 +721,27 @@ __device__ inline void decode_group(WaveTileState& st, uint32_t p, bool valid, u
       st.len += __popcll(has);
     }
   }
-  // Dense tiles: the whole wave loads one (lane = row) and emits its jobs.
-  for (uint64_t dm = ballot(dense); dm; dm &= dm - 1) {
-    const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
-    const uint32_t pk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), k));
-    const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
-    const uint64_t vm = read_lane(vm_l, k);
-    const uint64_t rm = rmask[pk];
-    const uint64_t bits =
-        ((rm >> lane) & 1) ? (__builtin_nontemporal_load(data + static_cast<uint64_t>(pk) * kTile + lane) & vm) : 0ull;
-    cnt[bEdges] += __popcll(bits);
-    emit_jobs(bits, static_cast<uint64_t>(t) * 64, lane, st.jobs, st.len, x_in, acc, lane, keep0, keep1, all_alive,
-              pad_job);
+  // Dense tiles: the whole wave loads them (lane = row, only rows in the tile's row mask, which
+  // rm_l holds per lane), kDenseBatch tiles at a time, and emits their jobs.
+  for (uint64_t dm = ballot(dense); dm;) {
+    uint64_t bits[kDenseBatch];
+    uint32_t tb[kDenseBatch];
+#pragma unroll
+    for (int q = 0; q < kDenseBatch; ++q) {
+      const bool any = dm != 0;
+      const int k = any ? __ffsll(static_cast<unsigned long long>(dm)) - 1 : 0;
+      dm &= dm - 1;
+      const uint64_t slot = (read_lane(pl, k) >> 13) & 0xFFFFFFFFull;
+      tb[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t_l), k));
+      const uint64_t vm = read_lane(vm_l, k);
+      const uint64_t rm = any ? read_lane(rm_l, k) : 0ull;
+      bits[q] = ((rm >> lane) & 1) ? (__builtin_nontemporal_load(dense_rows + slot * kTile + lane) & vm) : 0ull;
+      cnt[bEdges] += __popcll(bits[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kDenseBatch; ++q)
+      emit_jobs(bits[q], static_cast<uint64_t>(tb[q]) * 64, lane, st.jobs, st.len, x_in, acc, lane, keep0, keep1,
+                all_alive, pad_job);
   }
 }
 
@@ -720,7 +760,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
   const PanelLevel& L = lv[ch.panel];
   const gptr<const uint32_t> __restrict__ off = as_global(L.off);
   const gptr<const uint32_t> __restrict__ col = as_global(L.col);
-  const gptr<const uint64_t> __restrict__ data = as_global(L.data);
+  const gptr<const uint64_t> __restrict__ dense_rows = as_global(L.dense);
   const gptr<const uint64_t> __restrict__ rmask = as_global(L.rmask);
   const gptr<const uint64_t> __restrict__ payload = as_global(L.payload);
   const gptr<const uint64_t> __restrict__ x_in = as_global(L.x_in);
@@ -770,9 +810,12 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
       const uint64_t plb = vb ? __builtin_nontemporal_load(payload + pb) : 0ull;
       const uint64_t vma = va ? v_in[ta] : 0ull, vmb = vb ? v_in[tb] : 0ull;
       const uint64_t sma = va ? (s_in[ta] & alive_in) : 0ull, smb = vb ? (s_in[tb] & alive_in) : 0ull;
-      decode_group(st, pa, va, ta, pla, vma, sma, data, rmask, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
+      // Row masks of the dense tiles, loaded beside the strip masks.
+      const uint64_t rma = (pla & kPayloadDense) ? __builtin_nontemporal_load(rmask + pa) : 0ull;
+      const uint64_t rmb = (plb & kPayloadDense) ? __builtin_nontemporal_load(rmask + pb) : 0ull;
+      decode_group(st, pa, va, ta, pla, rma, vma, sma, dense_rows, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
                    cnt);
-      decode_group(st, pb, vb, tb, plb, vmb, smb, data, rmask, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
+      decode_group(st, pb, vb, tb, plb, rmb, vmb, smb, dense_rows, x_in, acc, lane, keep0, keep1, all_alive, pad_job,
                    cnt);
     }
     if (st.len) drain_jobs(st.jobs, st.len, x_in, acc, lane, keep0, keep1, all_alive, pad_job);
@@ -847,6 +890,288 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
     const uint64_t left = L.s_in[it.strip] & L.alive_in;
     if (left) atomicAdd(slot + kLeft, static_cast<unsigned long long>(__popcll(left)));
   }
+}
+
+// --- first hop by push from the source strips ----------------------------------
+// X_0 is one-hot: only the A^T tile columns of the strips that hold sources carry numeric
+// work into X_1. The first hop of a walk can therefore read just those columns (through a
+// column index of A^T) and OR each source row into its target rows with global atomics,
+// instead of sweeping all strips with the LDS accumulator. Four launches produce the same
+// X_1, masks and counters as the sweep: push_reset (clear the previous run's words of X_1,
+// reset its masks), push_structural (S_1 and the tile-product counts from the A^T index, one
+// workgroup per scheduler chunk, no global atomics on hot strips), push_first_hop (the
+// jobs) and push_finalize (N, V masks, popcount and strip counts, read back from the rows
+// the pushes marked in Z).
+
+// Positions [begin, end) of the A^T column index that belong to source strip `strip`.
+struct PushItem {
+  uint32_t strip, begin, end, pad;
+};
+constexpr uint32_t kPushItemTiles = 512;   // column positions per item
+constexpr int kPushCap = 128;              // jobs per wave list
+constexpr int kPushUnroll = 8;             // source rows loaded per step
+
+// One wave per strip: zero X_1 words inside the previous Z x N masks, reset X_1's masks.
+template <int W>
+__global__ void __launch_bounds__(kBlock) push_reset_kernel(const PanelLevel* __restrict__ lv) {
+  const PanelLevel& L = lv[0];
+  const uint32_t lane = lane_id();
+  const gptr<uint64_t> x = as_global(L.x_out);
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t y = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); y < L.n_strips; y += n_waves) {
+    const uint64_t z = as_global(L.z_out)[y];
+    const uint64_t n = as_global(L.n_out)[y];
+    if (z != 0 && n != 0) {
+      const bool mine = lane < W && ((n >> lane) & 1);
+      for (uint64_t m = z; m; m &= m - 1) {
+        const uint64_t r = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+        if (mine) x[(static_cast<uint64_t>(y) * 64 + r) * W + lane] = 0;
+      }
+    }
+    if (lane == 0) {
+      as_global(L.n_out)[y] = 0;
+      as_global(L.v_out)[y] = 0;
+      as_global(L.z_out)[y] = 0;
+      as_global(L.s_out)[y] = 0;
+    }
+  }
+}
+
+// S_1 and the tile-product counts of the first hop: per A^T tile (y, t), S_1[y] |= S_0[t] &
+// alive. One workgroup per scheduler chunk; the chunk's strips are reduced in LDS and stored
+// (pieces of split strips OR into S_1, zeroed by push_reset).
+__global__ void __launch_bounds__(kBlock) push_structural_kernel(const Chunk* __restrict__ chunks,
+                                                                 const PanelLevel* __restrict__ lv) {
+  __shared__ unsigned long long s_sym[kChunkStrips];
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const PanelLevel& L = lv[0];
+  const Chunk ch = chunks[blockIdx.x];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const gptr<const uint32_t> col = as_global(L.col);
+  const gptr<const uint32_t> row = as_global(L.row);
+  const gptr<const uint64_t> s_in = as_global(L.s_in);
+  const uint64_t alive = L.alive_in;
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < kChunkStrips) s_sym[threadIdx.x] = 0;
+  __syncthreads();
+  // kGroups groups of 64 tiles per wave step, their index loads issued together.
+  constexpr int kGroups = 4;
+  for (uint32_t g0 = ch.tile_begin + 64 * wave; g0 < ch.tile_end; g0 += 64 * kWaves * kGroups) {
+    uint32_t t[kGroups], y[kGroups];
+    uint64_t sv[kGroups];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      const uint32_t p = g0 + 64 * kWaves * q + lane;
+      const bool v = p < ch.tile_end;
+      t[q] = v ? __builtin_nontemporal_load(col + p) : 0xFFFFFFFFu;
+      y[q] = v ? __builtin_nontemporal_load(row + p) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) sv[q] = t[q] != 0xFFFFFFFFu ? (s_in[t[q]] & alive) : 0ull;
+    // A group mostly lies in one strip: lanes in lane 0's strip are reduced across the wave
+    // first, so the LDS ORs do not all queue on one address.
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      cnt[bTP] += __popcll(sv[q]);
+      cnt[bUsed] += sv[q] != 0;
+      const uint32_t y0 = read_first_lane(y[q]);
+      const bool same = y[q] == y0;
+      const uint64_t red = wave_or(same ? sv[q] : 0ull);
+      if (lane == 0 && red) atomicOr(&s_sym[y0 - ch.strip_begin], static_cast<unsigned long long>(red));
+      if (!same && sv[q]) atomicOr(&s_sym[y[q] - ch.strip_begin], static_cast<unsigned long long>(sv[q]));
+    }
+  }
+  __syncthreads();
+  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
+  if (threadIdx.x < n_cs) {
+    const uint32_t y = ch.strip_begin + threadIdx.x;
+    const gptr<const uint32_t> off = as_global(L.off);
+    const bool split = ch.tile_begin != off[y] || ch.tile_end != off[y + 1];
+    if (split) {
+      if (s_sym[threadIdx.x]) atomicOr((unsigned long long*)(L.s_out + y), s_sym[threadIdx.x]);
+    } else {
+      as_global(L.s_out)[y] = s_sym[threadIdx.x];
+    }
+  }
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
+}
+
+// Pushes the first n jobs of a wave list; a job is (target vertex << 32) | source vertex.
+// Each source row is loaded whole (lane = word) and its non-zero words are OR-ed into the
+// target row; the target row is marked in Z.
+template <int W>
+__device__ inline void push_drain(unsigned long long* __restrict__ jobs, uint32_t n, gptr<const uint64_t> x_in,
+                                  gptr<uint64_t> x_out, gptr<uint64_t> z_out, uint32_t lane,
+                                  unsigned long long pad_job) {
+  const uint32_t padded = (n + kPushUnroll - 1) / kPushUnroll * kPushUnroll;
+  if (lane < padded - n) jobs[n + lane] = pad_job;
+  wave_lds_sync();
+  for (uint32_t j = 0; j < padded; j += kPushUnroll) {
+    unsigned long long e[kPushUnroll];
+    uint64_t val[kPushUnroll];
+#pragma unroll
+    for (int k = 0; k < kPushUnroll; ++k) {
+      e[k] = jobs[j + k];
+      val[k] = lane < W ? x_in[(e[k] & 0xFFFFFFFFull) * W + lane] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kPushUnroll; ++k) {
+      const uint64_t v = e[k] >> 32;
+      if (val[k]) atomicOr((unsigned long long*)(x_out + v * W + lane), static_cast<unsigned long long>(val[k]));
+      const uint64_t any = ballot(val[k] != 0);
+      if (lane == 0 && any != 0) atomicOr((unsigned long long*)(z_out + (v >> 6)), 1ull << (v & 63));
+    }
+  }
+  wave_lds_sync();
+}
+
+// One workgroup per item; its waves take the item's column positions in groups of 64
+// (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job.
+template <int W>
+__global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* __restrict__ items,
+                                                                const PanelLevel* __restrict__ lv,
+                                                                const uint32_t* __restrict__ csc_tile_p,
+                                                                const uint32_t* __restrict__ csc_row_p,
+                                                                const uint64_t* __restrict__ csc_payload_p) {
+  __shared__ unsigned long long jobs_all[kWaves][kPushCap];
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const PanelLevel& L = lv[0];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const PushItem it = items[blockIdx.x];
+  const gptr<const uint32_t> csc_tile = as_global(csc_tile_p);
+  const gptr<const uint32_t> csc_row = as_global(csc_row_p);
+  const gptr<const uint64_t> csc_payload = as_global(csc_payload_p);
+  const gptr<const uint64_t> dense_rows = as_global(L.dense);
+  const gptr<const uint64_t> rmask = as_global(L.rmask);
+  const gptr<const uint64_t> x_in = as_global(L.x_in);
+  const gptr<uint64_t> x_out = as_global(L.x_out);
+  const gptr<uint64_t> z_out = as_global(L.z_out);
+  const uint32_t t = it.strip;
+  const uint64_t vm = as_global(L.v_in)[t];
+  const unsigned long long pad_job = static_cast<unsigned long long>(L.n_strips) * 64;  // zero source row
+  unsigned long long* const jobs = jobs_all[wave];
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t len = 0;
+  for (uint32_t g = it.begin + 64 * wave; g < it.end; g += 64 * kWaves) {
+    const uint32_t p = g + lane;
+    const bool valid = p < it.end;
+    const uint32_t i = valid ? __builtin_nontemporal_load(csc_tile + p) : 0u;
+    const uint32_t y = valid ? __builtin_nontemporal_load(csc_row + p) : 0u;
+    const uint64_t pl = valid ? __builtin_nontemporal_load(csc_payload + p) : 0ull;
+    const uint64_t rm_l = (pl & kPayloadDense) ? rmask[i] : 0ull;
+    const bool act = valid && vm != 0 && pl != 0;
+    const bool dense = act && (pl & kPayloadDense) != 0;
+    const uint64_t ent = (act && !dense) ? pl : 0ull;
+    const uint32_t n_ent = static_cast<uint32_t>(ent >> 48) & 7u;
+#pragma unroll
+    for (uint32_t k = 0; k < kPayloadMax; ++k) {
+      const uint32_t e = static_cast<uint32_t>(ent >> (12 * k)) & 0xFFFu;
+      const bool ok = k < n_ent && ((vm >> (e & 63u)) & 1);
+      const uint64_t has = ballot(ok);
+      if (has) {
+        if (len + 64 > kPushCap) {
+          push_drain<W>(jobs, len, x_in, x_out, z_out, lane, pad_job);
+          len = 0;
+        }
+        if (ok)
+          jobs[len + lanes_below(has)] = (static_cast<unsigned long long>(y * 64 + (e >> 6)) << 32) | (t * 64 + (e & 63u));
+        len += __popcll(has);
+        if (lane == 0) cnt[bEdges] += __popcll(has);
+      }
+    }
+    // Dense tiles: the whole wave loads them (lane = row), kDenseBatch at a time, with the
+    // row masks loaded per lane beside the payloads.
+    for (uint64_t dm = ballot(dense); dm;) {
+      uint64_t bits[kDenseBatch];
+      uint32_t yb[kDenseBatch];
+#pragma unroll
+      for (int q = 0; q < kDenseBatch; ++q) {
+        const bool any = dm != 0;
+        const int k = any ? __ffsll(static_cast<unsigned long long>(dm)) - 1 : 0;
+        dm &= dm - 1;
+          const uint64_t slot = (read_lane(pl, k) >> 13) & 0xFFFFFFFFull;
+        yb[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(y), k));
+        const uint64_t rm = any ? read_lane(rm_l, k) : 0ull;
+        bits[q] = ((rm >> lane) & 1) ? (dense_rows[slot * kTile + lane] & vm) : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < kDenseBatch; ++q) {
+        for (uint64_t has = ballot(bits[q] != 0); has; has = ballot(bits[q] != 0)) {
+          if (len + 64 > kPushCap) {
+            push_drain<W>(jobs, len, x_in, x_out, z_out, lane, pad_job);
+            len = 0;
+          }
+          if (bits[q]) {
+            const uint32_t c = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(bits[q])) - 1);
+            jobs[len + lanes_below(has)] = (static_cast<unsigned long long>(yb[q] * 64 + lane) << 32) | (t * 64 + c);
+            bits[q] &= bits[q] - 1;
+          }
+          len += __popcll(has);
+          if (lane == 0) cnt[bEdges] += __popcll(has);
+        }
+      }
+    }
+  }
+  if (len) push_drain<W>(jobs, len, x_in, x_out, z_out, lane, pad_job);
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
+}
+
+// One wave per strip: N and V masks and popcount of X_1 from the rows marked in Z, and the
+// per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1).
+template <int W>
+__global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel* __restrict__ lv) {
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const PanelLevel& L = lv[0];
+  const uint32_t lane = lane_id();
+  const gptr<const uint64_t> x = as_global(L.x_out);
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t y = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); y < L.n_strips; y += n_waves) {
+    const uint64_t z = as_global(L.z_out)[y];
+    uint64_t nb = 0, vb = 0;
+    // Rows of Z four at a time, their loads issued together.
+    for (uint64_t m = z; m;) {
+      uint64_t val[4];
+      uint32_t r[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const bool any = m != 0;
+        r[q] = any ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) : 0u;
+        m &= m - 1;
+        val[q] = (any && lane < W) ? x[(static_cast<uint64_t>(y) * 64 + r[q]) * W + lane] : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cnt[bPop] += __popcll(val[q]);
+        const uint64_t words = ballot(val[q] != 0);
+        nb |= words;
+        if (words & L.alive_next) vb |= 1ull << r[q];
+      }
+    }
+    if (lane == 0) {
+      if (z != 0) {
+        as_global(L.n_out)[y] = nb;
+        as_global(L.v_out)[y] = vb;
+      }
+      cnt[bNumeric] += __popcll(nb);
+      cnt[bOut] += __popcll(as_global(L.s_out)[y]);
+      cnt[bLeft] += __popcll(as_global(L.s_in)[y] & L.alive_in);
+    }
+  }
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
 }
 
 // Digest of words [w0, w0+n_words) of a vertex-major frontier (word index relative to w0).
@@ -953,6 +1278,14 @@ struct GraphPlan {
   std::vector<uint32_t> split;
   DevBuf<uint64_t> rmask;    // row-occupancy mask per A^T tile, taken when the plan is made
   DevBuf<uint64_t> payload;  // compact bit list per A^T tile (tile_payload_kernel)
+  DevBuf<uint64_t> dense;    // rows of the dense tiles (dense_compact_kernel)
+  // Column index of A^T (first-hop push): tile indices grouped by tile column, each with a
+  // copy of its strip and payload so the push streams them, and the column offsets into it
+  // (host copy, n_strips + 1). Empty unless build_column_index ran.
+  DevBuf<uint32_t> csc_tile;
+  DevBuf<uint32_t> csc_row;
+  DevBuf<uint64_t> csc_payload;
+  std::vector<uint32_t> host_csc_off;
 };
 
 GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
@@ -970,9 +1303,77 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
     tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile,
                                                                             g.payload.get());
     MG_LAUNCH_CHECK();
+    const uint32_t n = at->n_nz_tile;
+    DevBuf<uint32_t> flag(n), pos(n + 1ull);
+    dense_flag_kernel<<<grid_for(n, kBlock), kBlock>>>(g.payload.get(), n, flag.get());
+    MG_LAUNCH_CHECK();
+    exclusive_scan_u32_on(nullptr, flag.get(), pos.get(), n);
+    uint32_t n_dense = 0;
+    MG_HIP(hipMemcpy(&n_dense, pos.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    g.dense.alloc(std::max<uint64_t>(1, static_cast<uint64_t>(n_dense) * kTile));
+    if (n_dense) {
+      dense_compact_kernel<<<grid_for(n, kWaves, 65536), kBlock>>>(at->data.get(), n, pos.get(), g.payload.get(),
+                                                                    g.dense.get());
+      MG_LAUNCH_CHECK();
+    }
     MG_HIP(hipDeviceSynchronize());
   }
   return g;
+}
+
+__global__ void __launch_bounds__(kBlock) column_count_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                              uint32_t* __restrict__ count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(count + col[i], 1u);
+}
+
+// Order within a column is whatever the atomics give: every consumer ORs or sums.
+__global__ void __launch_bounds__(kBlock) column_scatter_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                                uint32_t* __restrict__ cursor,
+                                                                uint32_t* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[atomicAdd(cursor + col[i], 1u)] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) column_copy_kernel(const uint32_t* __restrict__ tile, uint32_t n,
+                                                             const uint32_t* __restrict__ row,
+                                                             const uint64_t* __restrict__ payload,
+                                                             uint32_t* __restrict__ out_row,
+                                                             uint64_t* __restrict__ out_payload) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t i = tile[j];
+    out_row[j] = row[i];
+    out_payload[j] = payload[i];
+  }
+}
+
+// Counting sort of the A^T tiles by column (16 bytes per tile on the device).
+void build_column_index(GraphPlan& g) {
+  const mg_tiled_matrix* at = g.at;
+  const uint32_t ns = at->n_strips, n = at->n_nz_tile;
+  DevBuf<uint32_t> cursor;
+  cursor.alloc(std::max<uint32_t>(1, ns));
+  MG_HIP(hipMemset(cursor.get(), 0, sizeof(uint32_t) * std::max<uint32_t>(1, ns)));
+  if (n) {
+    column_count_kernel<<<grid_for(n, kBlock), kBlock>>>(at->col_idx.get(), n, cursor.get());
+    MG_LAUNCH_CHECK();
+  }
+  std::vector<uint32_t> count(ns);
+  if (ns) MG_HIP(hipMemcpy(count.data(), cursor.get(), sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+  g.host_csc_off.assign(ns + 1ull, 0);
+  for (uint32_t t = 0; t < ns; ++t) g.host_csc_off[t + 1] = g.host_csc_off[t] + count[t];
+  if (ns) MG_HIP(hipMemcpy(cursor.get(), g.host_csc_off.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+  g.csc_tile.alloc(std::max<uint32_t>(1, n));
+  g.csc_row.alloc(std::max<uint32_t>(1, n));
+  g.csc_payload.alloc(std::max<uint32_t>(1, n));
+  if (n) {
+    column_scatter_kernel<<<grid_for(n, kBlock), kBlock>>>(at->col_idx.get(), n, cursor.get(), g.csc_tile.get());
+    MG_LAUNCH_CHECK();
+    column_copy_kernel<<<grid_for(n, kBlock), kBlock>>>(g.csc_tile.get(), n, at->row_idx.get(), g.payload.get(),
+                                                        g.csc_row.get(), g.csc_payload.get());
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipDeviceSynchronize());
 }
 
 }  // namespace
@@ -1064,9 +1465,11 @@ void build_engine(Engine* e) {
       PanelLevel& L = lv[pi];
       L.off = g.at->offset_row.get();
       L.col = g.at->col_idx.get();
+      L.row = g.at->row_idx.get();
       L.data = g.at->data.get();
       L.rmask = g.rmask.get();
       L.payload = g.payload.get();
+      L.dense = g.dense.get();
       L.x_in = p.x[k].get();
       L.v_in = p.v[k].get();
       L.s_in = e->prune ? p.n[k].get() : p.s[k].get();
@@ -1164,6 +1567,45 @@ void launch_level(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   }
 }
 
+// First hop of a single walk by push (panel 0, level 0); items already on the device.
+template <int W>
+void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushItem* items, uint32_t n_items, hipEvent_t ev[4]) {
+  const GraphPlan& g = e->graphs[0];
+  const uint32_t ns = e->panels[0].n_strips;
+  if (ev) {
+    MG_HIP(hipEventRecord(ev[0], st));
+    MG_HIP(hipEventRecord(ev[1], st));
+    MG_HIP(hipEventRecord(ev[2], st));
+  }
+  push_reset_kernel<W><<<grid_for(ns, kWaves, 4096), kBlock, 0, st>>>(e->lv[0].get());
+  MG_LAUNCH_CHECK();
+  if (e->n_chunks[0]) {
+    push_structural_kernel<<<e->n_chunks[0], kBlock, 0, st>>>(e->chunks[0].get(), e->lv[0].get());
+    MG_LAUNCH_CHECK();
+  }
+  if (n_items) {
+    push_first_hop_kernel<W><<<n_items, kBlock, 0, st>>>(items, e->lv[0].get(), g.csc_tile.get(), g.csc_row.get(),
+                                                        g.csc_payload.get());
+    MG_LAUNCH_CHECK();
+  }
+  push_finalize_kernel<W><<<grid_for(ns, kWaves, 16384), kBlock, 0, st>>>(e->lv[0].get());
+  MG_LAUNCH_CHECK();
+  if (ev) MG_HIP(hipEventRecord(ev[3], st));
+}
+
+void launch_push_first_hop(hipStream_t st, Engine* e, const PushItem* items, uint32_t n_items, hipEvent_t ev[4]) {
+  switch (e->W) {
+    case 1: launch_push_first_hop_w<1>(st, e, items, n_items, ev); break;
+    case 2: launch_push_first_hop_w<2>(st, e, items, n_items, ev); break;
+    case 4: launch_push_first_hop_w<4>(st, e, items, n_items, ev); break;
+    case 8: launch_push_first_hop_w<8>(st, e, items, n_items, ev); break;
+    case 16: launch_push_first_hop_w<16>(st, e, items, n_items, ev); break;
+    case 32: launch_push_first_hop_w<32>(st, e, items, n_items, ev); break;
+    case 64: launch_push_first_hop_w<64>(st, e, items, n_items, ev); break;
+    default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex " + std::to_string(e->W));
+  }
+}
+
 // X_0 fills + source bits (sources already on the device).
 template <int W>
 void launch_clear_x0(hipStream_t st, Engine* e) {
@@ -1235,6 +1677,12 @@ struct mg_walk {
   uint32_t hops = 0;       // of the last run
   bool timing = false;
   PinnedBuf<SourceItem> staging;
+  // First hop by push: per-run items (source strips' column ranges) and their staging.
+  PinnedBuf<PushItem> push_staging;
+  DevBuf<PushItem> push_items;
+  uint32_t max_push_items = 0;
+  bool last_push = false;              // the last run's first hop was pushed
+  hipEvent_t staging_read = nullptr;   // the last run's uploads have left the staging buffers
   // Timing log: per timed run, 4 events per hop (symbolic start/end, numeric kernel
   // start/end), kept for every run since timing was enabled.
   std::vector<std::vector<hipEvent_t>> run_events;
@@ -1243,6 +1691,7 @@ struct mg_walk {
     for (auto& r : run_events)
       for (auto ev : r) (void)hipEventDestroy(ev);
     for (auto ev : event_pool) (void)hipEventDestroy(ev);
+    if (staging_read) (void)hipEventDestroy(staging_read);
   }
   hipEvent_t take_event() {
     if (event_pool.empty()) {
@@ -1268,7 +1717,7 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     *out = nullptr;
     MG_REQUIRE(max_sources >= 1 && max_sources <= 4096, "mg_walk_create: max_sources must be in [1, 4096]");
     MG_REQUIRE(max_hops >= 1 && max_hops <= 64, "mg_walk_create: max_hops must be in [1, 64]");
-    MG_REQUIRE((flags & ~MG_WALK_PRUNE) == 0, "mg_walk_create: unknown flags");
+    MG_REQUIRE((flags & ~(MG_WALK_PRUNE | MG_WALK_SWEEP_FIRST_HOP)) == 0, "mg_walk_create: unknown flags");
     DeviceGuard g(ctx->device);
     auto w = std::make_unique<mg_walk>();
     w->device = ctx->device;
@@ -1288,6 +1737,14 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     build_engine(&e);
     e.sources.alloc(max_sources);
     w->staging.alloc(max_sources);
+    if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) {
+      build_column_index(e.graphs[0]);
+      // Each source strip's column is cut into pieces of kPushItemTiles positions.
+      w->max_push_items = max_sources + at->n_nz_tile / kPushItemTiles + 1;
+      w->push_items.alloc(w->max_push_items);
+      w->push_staging.alloc(w->max_push_items);
+    }
+    MG_HIP(hipEventCreateWithFlags(&w->staging_read, hipEventDisableTiming));
     *out = w.release();
   });
 }
@@ -1320,16 +1777,49 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     hipStream_t st = ctx->stream;
     w->n_sources = n_sources;
     w->hops = hops;
+    // The previous run's uploads read the staging buffers asynchronously.
+    MG_HIP(hipEventSynchronize(w->staging_read));
     SourceItem* stg = w->staging.get();
     for (uint32_t i = 0; i < n_sources; ++i) stg[i] = {0, i, sources[i]};
     MG_HIP(hipMemcpyAsync(w->e.sources.get(), stg, sizeof(SourceItem) * n_sources, hipMemcpyHostToDevice, st));
+    // First hop: push from the source strips when their A^T columns hold at most half of the
+    // tiles (the sweep reads every tile once; the push reads the source columns).
+    uint32_t n_items = 0;
+    bool push = false;
+    if (w->max_push_items) {
+      const std::vector<uint32_t>& off = w->e.graphs[0].host_csc_off;
+      std::vector<uint32_t> strips(n_sources);
+      for (uint32_t i = 0; i < n_sources; ++i) strips[i] = sources[i] / 64;
+      std::sort(strips.begin(), strips.end());
+      strips.erase(std::unique(strips.begin(), strips.end()), strips.end());
+      uint64_t tiles = 0;
+      for (uint32_t t : strips) tiles += off[t + 1] - off[t];
+      push = 2 * tiles <= w->e.graphs[0].at->n_nz_tile;
+      if (push) {
+        PushItem* it = w->push_staging.get();
+        for (uint32_t t : strips)
+          for (uint32_t b = off[t]; b < off[t + 1]; b += kPushItemTiles)
+            it[n_items++] = {t, b, std::min(off[t + 1], b + kPushItemTiles), 0u};
+        MG_REQUIRE(n_items <= w->max_push_items, "mg_walk_run: internal push item overflow");
+        if (n_items)
+          MG_HIP(hipMemcpyAsync(w->push_items.get(), it, sizeof(PushItem) * n_items, hipMemcpyHostToDevice, st));
+      }
+    }
+    MG_HIP(hipEventRecord(w->staging_read, st));
     launch_init(st, &w->e, n_sources);
     std::vector<hipEvent_t> ev;
     if (w->timing) {
       ev.resize(4 * static_cast<size_t>(hops));
       for (auto& x : ev) x = w->take_event();
     }
-    for (uint32_t k = 0; k < hops; ++k) launch_level(st, &w->e, k, w->timing ? &ev[4 * k] : nullptr);
+    for (uint32_t k = 0; k < hops; ++k) {
+      hipEvent_t* const evk = w->timing ? &ev[4 * k] : nullptr;
+      if (k == 0 && push)
+        launch_push_first_hop(st, &w->e, w->push_items.get(), n_items, evk);
+      else
+        launch_level(st, &w->e, k, evk);
+    }
+    w->last_push = push;
     if (w->timing) w->run_events.push_back(std::move(ev));
   });
 }
@@ -1343,6 +1833,7 @@ extern "C" int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_h
     MG_HIP(hipMemcpyAsync(c, w->e.ctr(0, hop), sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
     hop_stats_from(c, (w->n_sources + 63) / 64, out);
+    out->kernel = (hop == 1 && w->last_push) ? MG_HOP_PUSH : MG_HOP_SWEEP;
     if (!w->run_events.empty() && w->run_events.back().size() >= 4ull * hop) {
       const auto& ev = w->run_events.back();
       float a = 0, b = 0;

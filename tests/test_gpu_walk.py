@@ -156,6 +156,42 @@ def test_walk_rerun_smaller_and_errors(ctx):
         w.hop_stats(3)
 
 
+_STAT_KEYS = ("tile_products", "left_tiles", "right_tiles_used", "out_tiles", "out_tiles_numeric", "popcount",
+              "alg_bytes", "edges_visited")
+
+
+@pytest.mark.parametrize("scale,seed,sizes,flags", [(12, 3, (1, 50, 700), 0), (12, 3, (700, 1, 50), 1),
+                                                    (14, 7, (100, 2000, 100), 0), (14, 5, (20, 130, 20), 1)])
+def test_walk_first_hop_push_equals_sweep(ctx, scale, seed, sizes, flags):
+    """The first hop pushed from the source strips' A^T columns gives the same frontiers,
+    masks (seen through the later hops and re-runs) and statistics as sweeping every strip,
+    over a sequence of runs whose source sets grow, shrink and repeat; the last run is also
+    checked against the oracle."""
+    M = mg()
+    n = 1 << scale
+    el, at, s, d = setup_graph(ctx, scale, seed)
+    hops = 3
+    wp = M.Walk(ctx, at, max(sizes), hops, flags=flags)
+    ws = M.Walk(ctx, at, max(sizes), hops, flags=flags | M.WALK_SWEEP_FIRST_HOP)
+    pushed = 0
+    for r, size in enumerate(sizes):
+        sources = O.sample_sources(n, size, 100 + r)
+        wp.run(sources, hops)
+        ws.run(sources, hops)
+        for k in range(1, hops + 1):
+            assert np.array_equal(wp.frontier(k), ws.frontier(k)), f"run {r} hop {k}"
+            a, b = wp.hop_stats(k), ws.hop_stats(k)
+            assert {f: a[f] for f in _STAT_KEYS} == {f: b[f] for f in _STAT_KEYS}, f"run {r} hop {k}"
+            assert b["kernel"] == M.HOP_SWEEP
+            if k > 1:
+                assert a["kernel"] == M.HOP_SWEEP
+        pushed += wp.hop_stats(1)["kernel"] == M.HOP_PUSH
+    assert pushed >= 1
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=wp.words_per_vertex)
+    for k in range(1, hops + 1):
+        assert np.array_equal(wp.frontier(k), xs[k])
+
+
 @pytest.mark.skipif(os.environ.get("MG_SKIP_FULL") == "1", reason="full-size check disabled by MG_SKIP_FULL=1")
 def test_walk_rmat22_full_scale_digests(ctx):
     """Configuration: R-MAT scale 22 (seed 2), 4096 sources (seed 2), 4 hops; every hop's
@@ -173,6 +209,7 @@ def test_walk_rmat22_full_scale_digests(ctx):
     off, isrc = O.in_csr(n, s, d)
     del s, d
     x = O.one_hot_frontier(n, sources, 64)
+    assert w.hop_stats(1)["kernel"] == M.HOP_PUSH
     for k in range(1, hops + 1):
         x = O.frontier_step(n, off, isrc, x)
         assert w.frontier_digest(k) == O.frontier_digest(x, n_src), f"hop {k}"
