@@ -115,6 +115,7 @@ SIGNATURES = {
     "mg_walk_frontier_digest": (i32, [vp, vp, u32, P_u64]),
     "mg_walk_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
     "mg_walk_destroy": (i32, [vp]),
+    "mg_probe_gather_rows": (i32, [vp, vp, u64, vp, u64, u32, u32, C.POINTER(f64)]),
     # batched walks
     "mg_walk_batch_create": (i32, [vp, vp, u32, vp, u32, vp, C.POINTER(vp)]),
     "mg_walk_batch_run": (i32, [vp, vp]),
