@@ -238,6 +238,18 @@ int mg_edges_digest(mg_context* ctx, const mg_edge_list* el, uint64_t* digest);
  * n_strips = ceil(n_vertices / 64), block scope = n_vertices (n_chunks = 1). */
 int mg_edges_to_tiled_matrix(mg_context* ctx, const mg_edge_list* el, int transpose,
                              mg_tiled_matrix** out);
+/* Grid-cut partition (replaces GridCutPartitioner::RunPartitioner,
+ * tools/graph_partitioner/partitioner/grid_cut.cu:38-206, with the block conversion of
+ * tools/graph_converter/converter/to_bit_tiled_matrix.cuh:34-74): block (bx, by) of the
+ * n_chunks x n_chunks grid holds the edges (u, v) with u / S == bx and v / S == by, in local
+ * ids (u - bx*S, v - by*S), as an edge list on S vertices, S = ceil(n_vertices / n_chunks) for
+ * both the partition and the conversion (the reference's partitioner uses
+ * ceil((max_vid+1)/n) and its converter num_vertices/n, grid_cut.cu:88-89 and
+ * to_bit_tiled_matrix.cuh:50, which place tiles wrongly unless n divides the vertex count).
+ * mg_edges_to_tiled_matrix(block, 0) is then block (bx, by) of the grid of A, and
+ * mg_edges_to_tiled_matrix(block of (by, bx), 1) is block (bx, by) of the grid of A^T. */
+int mg_edges_grid_block(mg_context* ctx, const mg_edge_list* el, uint32_t n_chunks, uint32_t bx, uint32_t by,
+                        mg_edge_list** out);
 int mg_edges_destroy(mg_edge_list* el);
 
 /* count distinct vertices in [0, n_vertices) by Floyd's sampling over SplitMix64

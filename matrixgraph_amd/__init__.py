@@ -335,6 +335,12 @@ class EdgeList:
         _check(lib().mg_edges_digest(self.ctx.h, self.h, C.byref(d)), "mg_edges_digest")
         return d.value
 
+    def grid_block(self, n_chunks: int, bx: int, by: int) -> "EdgeList":
+        """Block (bx, by) of the grid-cut partition in local ids, on ceil(n/n_chunks) vertices."""
+        h = C.c_void_p()
+        _check(lib().mg_edges_grid_block(self.ctx.h, self.h, n_chunks, bx, by, C.byref(h)), "mg_edges_grid_block")
+        return EdgeList(self.ctx, h)
+
     def to_tiled_matrix(self, transpose: bool = False) -> TiledMatrix:
         h = C.c_void_p()
         _check(lib().mg_edges_to_tiled_matrix(self.ctx.h, self.h, 1 if transpose else 0, C.byref(h)),
@@ -473,6 +479,22 @@ def write_grid(path, blocks, n_vertices: int, n_edges: int) -> None:
     for gid, b in enumerate(blocks):
         write_block(f"{str(path).rstrip('/')}/block{gid}/", b)
     _check(lib().mg_io_write_grid_metadata(_dir(path), n_chunks, n_vertices, n_edges), "mg_io_write_grid_metadata")
+
+
+def write_grid_from_edges(path, el: "EdgeList", n_chunks: int, transpose: bool = False) -> None:
+    """Partitions el into an n_chunks x n_chunks grid on the device (grid cut, then one
+    tile conversion per block) and writes it in the reference's on-disk grid format; with
+    transpose the grid is that of A^T (block (x, y) = transpose of A's block (y, x))."""
+    n_vertices, n_edges = el.info()
+    blocks = []
+    for x in range(n_chunks):
+        for y in range(n_chunks):
+            b = el.grid_block(n_chunks, y, x) if transpose else el.grid_block(n_chunks, x, y)
+            tm = b.to_tiled_matrix(transpose=transpose)
+            blocks.append(tm.download())
+            tm.close()
+            b.close()
+    write_grid(path, blocks, n_vertices, n_edges)
 
 
 def read_grid(path):

@@ -341,6 +341,75 @@ extern "C" int mg_edges_union(mg_context* ctx, const mg_edge_list* a, const mg_e
   });
 }
 
+namespace mgamd {
+namespace {
+
+// Grid-cut block membership of each edge key (src << 32 | dst): src in block bx, dst in by.
+__global__ void grid_block_flag_kernel(const uint64_t* __restrict__ keys, uint64_t n, uint32_t scope, uint32_t bx,
+                                       uint32_t by, uint32_t* __restrict__ flag) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    const uint32_t s = static_cast<uint32_t>(k >> 32), d = static_cast<uint32_t>(k);
+    flag[i] = (s / scope == bx && d / scope == by) ? 1u : 0u;
+  }
+}
+
+// Stable compaction of the block's edges into local ids (u - bx*S, v - by*S): the order of
+// the input keys is kept, so the output is sorted and unique as well.
+__global__ void grid_block_scatter_kernel(const uint64_t* __restrict__ keys, uint64_t n, uint32_t scope, uint32_t bx,
+                                          uint32_t by, const uint32_t* __restrict__ flag,
+                                          const uint32_t* __restrict__ pos, uint64_t* __restrict__ out) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (!flag[i]) continue;
+    const uint64_t k = keys[i];
+    const uint64_t s = (k >> 32) - static_cast<uint64_t>(bx) * scope;
+    const uint64_t d = (k & 0xFFFFFFFFull) - static_cast<uint64_t>(by) * scope;
+    out[pos[i]] = (s << 32) | d;
+  }
+}
+
+}  // namespace
+}  // namespace mgamd
+
+extern "C" int mg_edges_grid_block(mg_context* ctx, const mg_edge_list* el, uint32_t n_chunks, uint32_t bx,
+                                   uint32_t by, mg_edge_list** out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && el && out, "mg_edges_grid_block: NULL argument");
+    *out = nullptr;
+    MG_REQUIRE(n_chunks >= 1 && n_chunks <= 256, "mg_edges_grid_block: n_chunks must be in [1, 256]");
+    MG_REQUIRE(bx < n_chunks && by < n_chunks, "mg_edges_grid_block: block index out of range");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    // One block scope for the partition and the conversion (the reference's two differ).
+    const uint32_t scope = static_cast<uint32_t>((static_cast<uint64_t>(el->n_vertices) + n_chunks - 1) / n_chunks);
+    const uint64_t n = el->n_edges;
+    MG_REQUIRE(n < (1ull << 31), "mg_edges_grid_block: more than 2^31 edges");
+    uint32_t count = 0;
+    DevBuf<uint32_t> flag(n ? n : 1), pos(n + 1);
+    if (n) {
+      grid_block_flag_kernel<<<grid_for(n, 256), 256, 0, st>>>(el->keys.get(), n, scope, bx, by, flag.get());
+      MG_LAUNCH_CHECK();
+      exclusive_scan_u32(ctx, flag.get(), pos.get(), static_cast<uint32_t>(n));
+      MG_HIP(hipMemcpyAsync(&count, pos.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      MG_HIP(hipStreamSynchronize(st));
+    }
+    auto b = std::make_unique<mg_edge_list>();
+    b->device = ctx->device;
+    b->n_vertices = scope;
+    b->n_edges = count;
+    b->keys.alloc(count ? count : 1);
+    if (count) {
+      grid_block_scatter_kernel<<<grid_for(n, 256), 256, 0, st>>>(el->keys.get(), n, scope, bx, by, flag.get(),
+                                                                  pos.get(), b->keys.get());
+      MG_LAUNCH_CHECK();
+    }
+    MG_HIP(hipStreamSynchronize(st));
+    *out = b.release();
+  });
+}
+
 extern "C" int mg_edges_upload(mg_context* ctx, uint32_t n_vertices, uint64_t n_edges, const uint32_t* src,
                                const uint32_t* dst, mg_edge_list** out) {
   return guarded([&] {

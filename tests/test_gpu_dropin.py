@@ -22,10 +22,10 @@ def mg():
 
 
 def grid_blocks(n, src, dst, n_chunks):
-    """Grid-cut partition with block scope S = n / n_chunks (tools/graph_partitioner/
+    """Grid-cut partition with block scope S = ceil(n / n_chunks) (tools/graph_partitioner/
     partitioner/grid_cut.cu:88-89,117-118): block (x, y) holds edges with src in block x and
     dst in block y, in local ids (u mod S, v mod S)."""
-    S = n // n_chunks
+    S = -(-n // n_chunks)
     blocks = []
     for x in range(n_chunks):
         for y in range(n_chunks):
@@ -74,3 +74,38 @@ def test_ppr_query_exec_reports_bad_input(tmp_path):
     r = subprocess.run([str(EXE), "-i", f"{tmp_path}/missing/", "-it", f"{tmp_path}/missing/"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1 and "meta.yaml" in r.stderr
+
+
+@pytest.mark.parametrize("n_chunks", [2, 3])
+def test_device_grid_cut_matches_host_and_runs(tmp_path, ctx, n_chunks):
+    """The device grid cut (mg_edges_grid_block + conversion) writes the same A and A^T grids
+    as the host partition, including a block scope that does not divide the vertex count
+    (n_chunks = 3), and PPRQuery over them gives the oracle's A * A."""
+    M = mg()
+    scale = 11
+    n = 1 << scale
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, 22)
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 22)
+    M.write_grid_from_edges(tmp_path / "A", el, n_chunks)
+    M.write_grid_from_edges(tmp_path / "AT", el, n_chunks, transpose=True)
+    for name, blocks in (("A", grid_blocks(n, s, d, n_chunks)), ("AT", grid_blocks(n, d, s, n_chunks))):
+        got, meta = M.read_grid(tmp_path / name)
+        assert meta == {"n_chunks": n_chunks, "n_vertices": n, "n_edges": len(s)}
+        for g, (a, b) in enumerate(zip(got, blocks)):
+            for f in ("offset_row", "row_idx", "col_idx", "data"):
+                assert np.array_equal(getattr(a, f), getattr(b, f)), (name, g, f)
+    total, _ = run_exec(tmp_path / "A", tmp_path / "AT", tmp_path / "C")
+    a = np.zeros((n, n), np.uint8)
+    a[s, d] = 1
+    expect = O.unpack_rows(O.bitset_matmul(O.pack_rows(a), O.pack_rows(a), n, n, n), n)
+    assert total == int(expect.sum())
+    blocks, meta = M.read_grid(tmp_path / "C")
+    S = -(-n // n_chunks)
+    for x in range(n_chunks):
+        for y in range(n_chunks):
+            got = blocks[x * n_chunks + y].to_dense(S)
+            want = expect[x * S:(x + 1) * S, y * S:(y + 1) * S]
+            assert np.array_equal(got[:want.shape[0], :want.shape[1]], want), (x, y)
+            assert got.sum() == want.sum(), (x, y)
+    with pytest.raises(M.MGError):
+        el.grid_block(n_chunks, n_chunks, 0)
