@@ -322,6 +322,26 @@ int mg_walk_frontier_digest(mg_context* ctx, const mg_walk* w, uint32_t hop, uin
 int mg_walk_export(mg_context* ctx, const mg_walk* w, uint32_t hop, mg_tiled_matrix** out);
 int mg_walk_destroy(mg_walk* w);
 
+/* ---------------------------------------------------------------------------
+ * Host-pointer surface for foreign callers (cgo and the like), in the style of the
+ * reference's Go API (core/go_api/matrixgraph_go_api.h:8-55): host memory only, 0 on
+ * success and 1 on error with the message on stderr, device from MATRIXGRAPH_HIP_DEVICE
+ * (else 0); each call sets up, runs and releases its own context.
+ * ------------------------------------------------------------------------- */
+
+/* C = A * B over the Boolean semiring with B passed transposed (the bit path's convention):
+ * A m x k, Bt n x k, C m x n, each packed row-major as one bit string (element (i, j) of an
+ * r x s matrix at bit i*s + j, ceil(r*s/64) words). C is overwritten. Bit-matrix counterpart
+ * of matrixgraph_matmult (matrixgraph_go_api.h:11, matrixgraph_go_api.cu:42-104). */
+int matrixgraph_bit_matmult(const uint64_t* A, const uint64_t* Bt, uint64_t* C, int m, int k, int n);
+
+/* X_hops of an exact-length walk from n_sources distinct sources (<= 4096) over the directed
+ * graph (src[i] -> dst[i]) on n_vertices vertices: x_out receives n_vertices rows of
+ * ceil(n_sources/64) words (bit s % 64 of word s / 64 of row v set iff vertex v is reached
+ * from sources[s] by a walk of exactly `hops` edges); popcount (may be NULL) gets its set bits. */
+int matrixgraph_bit_walk(const uint32_t* src, const uint32_t* dst, int n_edges, int n_vertices,
+                         const uint32_t* sources, int n_sources, int hops, uint64_t* x_out, uint64_t* popcount);
+
 /* Measurement helper (no counterpart in the reference): the gather floor of the hop kernel.
  * Gathers the 512-byte rows x[rows[i]] (64 words each; x holds n_x_rows rows) exactly as the
  * hop kernel gathers frontier rows, with no other work, over `grid` workgroups of 256

@@ -15,7 +15,7 @@ def declared_symbols():
     names = set()
     for h in (ROOT / "include").glob("*.h"):
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        names |= set(re.findall(r"\b(mg_[a-z0-9_]+)\s*\(", text))
+        names |= set(re.findall(r"\b((?:mg|matrixgraph)_[a-z0-9_]+)\s*\(", text))
     return names
 
 
