@@ -178,6 +178,11 @@ int mg_tiled_matrix_digest(mg_context* ctx, const mg_tiled_matrix* m, uint64_t* 
 /* Total set bits (the reference's total_count, ppr_query.cu:874-956). */
 int mg_tiled_matrix_popcount(mg_context* ctx, const mg_tiled_matrix* m, uint64_t* count);
 int mg_tiled_matrix_destroy(mg_tiled_matrix* m);
+/* Result egress: every set bit (row r, column c) of tile (x, y) as the edge
+ * (x*64 + r, y*64 + c), sorted, on n_strips*64 vertices (the tile -> edge-list expansion
+ * the reference leaves to its callers, core/task/gpu_task/ppr_query.cu:874-956 counts only).
+ * For an exported X_k the edge (s, v) says source s reaches vertex v. */
+int mg_tiled_matrix_to_edges(mg_context* ctx, const mg_tiled_matrix* m, mg_edge_list** out);
 
 typedef struct mg_product_stats {
   uint64_t tile_products;  /* (r,k,c) with A(r,k) and B(k,c) both present */

@@ -241,6 +241,12 @@ class TiledMatrix:
         _check(lib().mg_tiled_matrix_popcount(self.ctx.h, self.h, C.byref(d)), "mg_tiled_matrix_popcount")
         return d.value
 
+    def to_edges(self) -> "EdgeList":
+        """Every set bit as an edge (row, column), sorted, on n_strips*64 vertices."""
+        h = C.c_void_p()
+        _check(lib().mg_tiled_matrix_to_edges(self.ctx.h, self.h, C.byref(h)), "mg_tiled_matrix_to_edges")
+        return EdgeList(self.ctx, h)
+
     def close(self):
         if self.h:
             lib().mg_tiled_matrix_destroy(self.h)

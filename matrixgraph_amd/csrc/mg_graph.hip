@@ -370,8 +370,80 @@ __global__ void grid_block_scatter_kernel(const uint64_t* __restrict__ keys, uin
   }
 }
 
+// Set bits of every tile (one wave per tile, lane = tile row).
+__global__ void __launch_bounds__(256) tile_bits_count_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                              uint64_t* __restrict__ count) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_tiles;
+       i += n_waves) {
+    const uint64_t t = wave_sum(__popcll(data[static_cast<uint64_t>(i) * 64 + lane]));
+    if (lane == 0) count[i] = t;
+  }
+}
+
+// Every set bit (row r, column c) of tile (x, y) becomes the edge key (x*64 + r) << 32 |
+// (y*64 + c), written at the tile's offset plus the bits of the rows before r.
+__global__ void __launch_bounds__(256) tile_expand_kernel(const uint32_t* __restrict__ row_idx,
+                                                          const uint32_t* __restrict__ col_idx,
+                                                          const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                          const uint64_t* __restrict__ incl,
+                                                          uint64_t* __restrict__ out) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_tiles;
+       i += n_waves) {
+    uint64_t w = data[static_cast<uint64_t>(i) * 64 + lane];
+    const uint32_t pc = __popcll(w);
+    uint32_t before = pc;  // inclusive prefix over lanes, made exclusive below
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), d, kWave));
+      if (lane >= static_cast<uint32_t>(d)) before += o;
+    }
+    const uint64_t tile_total = incl[i] - (i ? incl[i - 1] : 0ull);
+    uint64_t p = incl[i] - tile_total + (before - pc);
+    const uint64_t src = static_cast<uint64_t>(row_idx[i]) * 64 + lane;
+    const uint64_t dst0 = static_cast<uint64_t>(col_idx[i]) * 64;
+    for (; w; w &= w - 1) out[p++] = (src << 32) | (dst0 + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+  }
+}
+
 }  // namespace
 }  // namespace mgamd
+
+extern "C" int mg_tiled_matrix_to_edges(mg_context* ctx, const mg_tiled_matrix* m, mg_edge_list** out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && m && out, "mg_tiled_matrix_to_edges: NULL argument");
+    *out = nullptr;
+    MG_REQUIRE(m->tile_size == 64, "mg_tiled_matrix_to_edges: tile_size must be 64");
+    MG_REQUIRE(static_cast<uint64_t>(m->n_strips) * 64 < (1ull << 32), "mg_tiled_matrix_to_edges: too many vertices");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const uint32_t n = m->n_nz_tile;
+    uint64_t total = 0;
+    DevBuf<uint64_t> cnt(n ? n : 1), incl(n ? n : 1);
+    if (n) {
+      tile_bits_count_kernel<<<grid_for(n, 4, 65536), 256, 0, st>>>(m->data.get(), n, cnt.get());
+      MG_LAUNCH_CHECK();
+      size_t tb = 0;
+      MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt.get(), incl.get(), static_cast<int>(n), st));
+      DevBuf<uint8_t> tmp(tb ? tb : 1);
+      MG_HIP(hipcub::DeviceScan::InclusiveSum(tmp.get(), tb, cnt.get(), incl.get(), static_cast<int>(n), st));
+      MG_HIP(hipMemcpyAsync(&total, incl.get() + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+      MG_HIP(hipStreamSynchronize(st));
+    }
+    MG_REQUIRE(total < (1ull << 31), "mg_tiled_matrix_to_edges: more than 2^31 set bits");
+    DevBuf<uint64_t> keys(total ? total : 1);
+    if (total) {
+      tile_expand_kernel<<<grid_for(n, 4, 65536), 256, 0, st>>>(m->row_idx.get(), m->col_idx.get(), m->data.get(), n,
+                                                                incl.get(), keys.get());
+      MG_LAUNCH_CHECK();
+    }
+    *out = finish_edges(ctx, std::move(keys), total, m->n_strips * 64, false).release();
+    MG_HIP(hipStreamSynchronize(st));
+  });
+}
 
 extern "C" int mg_edges_grid_block(mg_context* ctx, const mg_edge_list* el, uint32_t n_chunks, uint32_t bx,
                                    uint32_t by, mg_edge_list** out) {

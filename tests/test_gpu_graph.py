@@ -103,3 +103,38 @@ def test_edge_lists_beyond_2_28_entries(ctx):
     assert el.digest() == O.edges_digest(s, d)
     t = el.to_tiled_matrix()
     assert t.popcount() == m
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 3), (14, 9)])
+def test_tiles_to_edges_roundtrip(ctx, scale, seed):
+    """Tile -> edge-list expansion: A's tiles give back the edge list, A^T's its transpose."""
+    M = mg()
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed)
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, seed)
+    back = el.to_tiled_matrix().to_edges()
+    assert back.info() == (1 << scale, len(s))
+    assert back.digest() == el.digest() == O.edges_digest(s, d)
+    bs, bd = el.to_tiled_matrix(transpose=True).to_edges().download()
+    ts, td = O.sort_unique_edges(d, s)
+    assert np.array_equal(bs, ts) and np.array_equal(bd, td)
+
+
+def test_walk_export_to_edges_matches_oracle(ctx):
+    """X_k exported and expanded: edge (s, v) iff source s reaches v in exactly k hops."""
+    M = mg()
+    scale, n_src, hops = 12, 150, 2
+    n = 1 << scale
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 4)
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, 4)
+    sources = O.sample_sources(n, n_src, 8)
+    w = M.Walk(ctx, el.to_tiled_matrix(transpose=True), n_src, hops)
+    w.run(sources, hops)
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=w.words_per_vertex)
+    for k in range(1, hops + 1):
+        es, ev = w.export(k).to_edges().download()
+        bits = np.unpackbits(xs[k].view(np.uint8), bitorder="little").reshape(n, -1)
+        v_idx, s_idx = np.nonzero(bits)
+        keep = s_idx < n_src
+        order = np.lexsort((v_idx[keep], s_idx[keep]))
+        assert np.array_equal(es, s_idx[keep][order].astype(np.uint32))
+        assert np.array_equal(ev, v_idx[keep][order].astype(np.uint32))
