@@ -31,6 +31,38 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def two_hop(M, ctx, scale):
+    """Two-hop pair enumeration (the CSR-tiled walk's result) on the device, and the oracle's
+    bitset product A*A on the host cores for the same graph (checked equal in count)."""
+    import oracle as O
+
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 2)
+    n, m = el.info()
+    el.two_hop().close()
+    ctx.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p = el.two_hop()
+        n_pairs = p.info()[1]
+        p.close()
+    gpu_s = (time.perf_counter() - t0) / reps
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, 2)
+    rows = np.zeros((n, (n + 63) // 64), np.uint64)
+    np.bitwise_or.at(rows, (s, d // 64), np.left_shift(np.uint64(1), (d % 64).astype(np.uint64)))
+    t0 = time.perf_counter()
+    c = O.bitset_matmul(rows, rows, n, n, n, threads=O.max_threads())
+    cpu_s = time.perf_counter() - t0
+    idx = np.arange(n)
+    diag = (c[idx, idx // 64] >> (idx % 64).astype(np.uint64)) & np.uint64(1)
+    cpu_pairs = int(np.unpackbits(c.view(np.uint8)).sum()) - int(diag.sum())
+    print(json.dumps({"two_hop": {
+        "graph": f"R-MAT scale {scale}, edge factor 16, seed 2, {m} edges", "pairs": n_pairs,
+        "device_ms": round(gpu_s * 1e3, 3), "pairs_per_s": round(n_pairs / gpu_s, 1),
+        "host_oracle_ms": round(cpu_s * 1e3, 3), "host_threads": O.max_threads(),
+        "host_pairs_equal": cpu_pairs == n_pairs}}), flush=True)
+
+
 def gather_floor(M, ctx, el, at, n_vertices, args):
     """Gathers the source row of every A^T bit (strip order: target strip, then source) with no
     other work, and times the walk's sweep hops beside it in the same process."""
@@ -343,6 +375,9 @@ def main():
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
     ap.add_argument("--sweep-first-hop", action="store_true",
                     help="sweep every strip at the first hop too (no push from the source strips)")
+    ap.add_argument("--two-hop", type=int, default=0, metavar="SCALE",
+                    help="only time the two-hop pair enumeration (mg_edges_two_hop) on an R-MAT graph of this "
+                         "scale (edge factor 16, seed 2), beside the oracle's bitset product on the host")
     ap.add_argument("--gather-floor", action="store_true",
                     help="only measure the hop kernel's gather floor: the 512-byte frontier row of every A^T bit "
                          "gathered in strip order with no other work, beside the walk's steady hops")
@@ -365,6 +400,9 @@ def main():
             M._check(M.lib().mg_buffer_copy(ctx.h, dst.ptr, src.ptr, nbytes), "mg_buffer_copy")
         ctx.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+        return
+    if args.two_hop:
+        two_hop(M, ctx, args.two_hop)
         return
     t0 = time.perf_counter()
     el = M.EdgeList.rmat(ctx, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)

@@ -184,6 +184,13 @@ int mg_tiled_matrix_destroy(mg_tiled_matrix* m);
  * For an exported X_k the edge (s, v) says source s reaches vertex v. */
 int mg_tiled_matrix_to_edges(mg_context* ctx, const mg_tiled_matrix* m, mg_edge_list** out);
 
+/* Two-hop walk in enumerating form (the result of the reference's CSR-tiled GEMM walk,
+ * core/task/gpu_task/gemm.cu:452-794 and walk_kernel, matrix_operations.cu:505-621, whose
+ * output copy is commented out): the distinct pairs (s, v) with s -> m -> v, m != s, m != v
+ * (self-loops are never a step) and v != s, as a sorted edge list on el's vertices. Computed
+ * on the bit tiles: A' = A without self-loops, C = A' * A', C without its diagonal. */
+int mg_edges_two_hop(mg_context* ctx, const mg_edge_list* el, mg_edge_list** out);
+
 typedef struct mg_product_stats {
   uint64_t tile_products;  /* (r,k,c) with A(r,k) and B(k,c) both present */
   uint64_t out_tiles;      /* structural tiles of C */

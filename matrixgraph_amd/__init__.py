@@ -341,6 +341,12 @@ class EdgeList:
         _check(lib().mg_edges_digest(self.ctx.h, self.h, C.byref(d)), "mg_edges_digest")
         return d.value
 
+    def two_hop(self) -> "EdgeList":
+        """Distinct pairs (s, v), v != s, joined by a two-edge walk without self-loop steps."""
+        h = C.c_void_p()
+        _check(lib().mg_edges_two_hop(self.ctx.h, self.h, C.byref(h)), "mg_edges_two_hop")
+        return EdgeList(self.ctx, h)
+
     def grid_block(self, n_chunks: int, bx: int, by: int) -> "EdgeList":
         """Block (bx, by) of the grid-cut partition in local ids, on ceil(n/n_chunks) vertices."""
         h = C.c_void_p()

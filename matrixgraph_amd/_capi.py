@@ -105,6 +105,7 @@ SIGNATURES = {
     "mg_edges_destroy": (i32, [vp]),
     "mg_edges_grid_block": (i32, [vp, vp, u32, u32, u32, C.POINTER(vp)]),
     "mg_tiled_matrix_to_edges": (i32, [vp, vp, C.POINTER(vp)]),
+    "mg_edges_two_hop": (i32, [vp, vp, C.POINTER(vp)]),
     "mg_sample_sources": (i32, [u32, u32, u64, vp]),
     # walks
     "mg_walk_create": (i32, [vp, vp, u32, u32, u32, C.POINTER(vp)]),

@@ -409,8 +409,81 @@ __global__ void __launch_bounds__(256) tile_expand_kernel(const uint32_t* __rest
   }
 }
 
+// Keep-flags of a stable filter: edge keys whose source and destination differ.
+__global__ void off_diagonal_flag_kernel(const uint64_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ flag) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    flag[i] = (keys[i] >> 32) != (keys[i] & 0xFFFFFFFFull) ? 1u : 0u;
+}
+
+__global__ void compact_keys_kernel(const uint64_t* __restrict__ keys, uint64_t n, const uint32_t* __restrict__ flag,
+                                    const uint32_t* __restrict__ pos, uint64_t* __restrict__ out) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    if (flag[i]) out[pos[i]] = keys[i];
+}
+
+// The edges of el whose source and destination differ, as a new list (order kept).
+std::unique_ptr<mg_edge_list> without_self_loops(mg_context* ctx, const mg_edge_list* el) {
+  hipStream_t st = ctx->stream;
+  const uint64_t n = el->n_edges;
+  MG_REQUIRE(n < (1ull << 31), "edge list filter: more than 2^31 edges");
+  uint32_t count = 0;
+  DevBuf<uint32_t> flag(n ? n : 1), pos(n + 1);
+  if (n) {
+    off_diagonal_flag_kernel<<<grid_for(n, 256), 256, 0, st>>>(el->keys.get(), n, flag.get());
+    MG_LAUNCH_CHECK();
+    exclusive_scan_u32(ctx, flag.get(), pos.get(), static_cast<uint32_t>(n));
+    MG_HIP(hipMemcpyAsync(&count, pos.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+  }
+  auto out = std::make_unique<mg_edge_list>();
+  out->device = ctx->device;
+  out->n_vertices = el->n_vertices;
+  out->n_edges = count;
+  out->keys.alloc(count ? count : 1);
+  if (count) {
+    compact_keys_kernel<<<grid_for(n, 256), 256, 0, st>>>(el->keys.get(), n, flag.get(), pos.get(), out->keys.get());
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipStreamSynchronize(st));
+  return out;
+}
+
 }  // namespace
 }  // namespace mgamd
+
+extern "C" int mg_edges_two_hop(mg_context* ctx, const mg_edge_list* el, mg_edge_list** out) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && el && out, "mg_edges_two_hop: NULL argument");
+    *out = nullptr;
+    DeviceGuard g(ctx->device);
+    // Self-loops are neither first nor second steps, and a walk back to its source is no pair:
+    // A' = A without its diagonal, C = A' * A' on the bit tiles, then C without its diagonal.
+    auto a_el = without_self_loops(ctx, el);
+    struct Owned {
+      mg_tiled_matrix *a = nullptr, *at = nullptr, *c = nullptr;
+      mg_edge_list* pairs = nullptr;
+      ~Owned() {
+        if (a) mg_tiled_matrix_destroy(a);
+        if (at) mg_tiled_matrix_destroy(at);
+        if (c) mg_tiled_matrix_destroy(c);
+        if (pairs) mg_edges_destroy(pairs);
+      }
+    } o;
+    auto check = [](int rc) {
+      if (rc != MG_OK) fail(rc, mg_last_error());
+    };
+    check(mg_edges_to_tiled_matrix(ctx, a_el.get(), 0, &o.a));
+    check(mg_edges_to_tiled_matrix(ctx, a_el.get(), 1, &o.at));
+    check(mg_multiply_symbolic(ctx, o.a, o.at, &o.c, nullptr));
+    check(mg_multiply_numeric(ctx, o.a, o.at, o.c));
+    check(mg_tiled_matrix_to_edges(ctx, o.c, &o.pairs));
+    auto r = without_self_loops(ctx, o.pairs);
+    r->n_vertices = el->n_vertices;
+    *out = r.release();
+  });
+}
 
 extern "C" int mg_tiled_matrix_to_edges(mg_context* ctx, const mg_tiled_matrix* m, mg_edge_list** out) {
   return guarded([&] {

@@ -138,3 +138,51 @@ def test_walk_export_to_edges_matches_oracle(ctx):
         order = np.lexsort((v_idx[keep], s_idx[keep]))
         assert np.array_equal(es, s_idx[keep][order].astype(np.uint32))
         assert np.array_equal(ev, v_idx[keep][order].astype(np.uint32))
+
+
+def _two_hop_rules(n, src, dst):
+    """The CSR-tiled walk's pair rules (core/task/gpu_task/kernel/matrix_operations.cu:536-
+    600): from s over an edge to m != s, then over an edge to v with v != m and v != s."""
+    out = [set() for _ in range(n)]
+    for u, v in zip(src.tolist(), dst.tolist()):
+        out[u].add(v)
+    pairs = set()
+    for s_ in range(n):
+        for m in out[s_]:
+            if m == s_:
+                continue
+            for v in out[m]:
+                if v != m and v != s_:
+                    pairs.add((s_, v))
+    return sorted(pairs)
+
+
+def test_two_hop_pairs_match_rules_and_oracle(ctx):
+    """Two-hop enumeration on the bit tiles equals the walk rules on the toy graph (with a
+    self-loop added) and the oracle's A'*A' without its diagonal on an R-MAT graph with
+    self-loops."""
+    M = mg()
+    toy = json.loads((Path(__file__).parent / "golden" / "toy_graph.json").read_text())
+    n = toy["n_vertices"]
+    e = np.array(toy["edges"], np.uint32).reshape(-1, 2)
+    s = np.concatenate([e[:, 0], np.array([1, 2], np.uint32)])
+    d = np.concatenate([e[:, 1], np.array([1, 2], np.uint32)])
+    got_s, got_d = M.EdgeList.upload(ctx, n, s, d).two_hop().download()
+    assert list(zip(got_s.tolist(), got_d.tolist())) == _two_hop_rules(n, s, d)
+
+    scale = 10
+    n = 1 << scale
+    rs, rd = O.rmat_edges(scale, 8, 0.57, 0.19, 0.19, 7)
+    loops = np.arange(0, n, 7, dtype=np.uint32)
+    s = np.concatenate([rs, loops])
+    d = np.concatenate([rd, loops])
+    pairs = M.EdgeList.upload(ctx, n, s, d).two_hop()
+    gs, gd = pairs.download()
+    a = np.zeros((n, n), np.uint8)
+    a[rs, rd] = 1  # A' (the R-MAT edges have no self-loops)
+    c = O.unpack_rows(O.bitset_matmul(O.pack_rows(a), O.pack_rows(a), n, n, n), n)
+    np.fill_diagonal(c, 0)
+    ws, wd = np.nonzero(c)
+    assert pairs.info() == (n, len(ws))
+    assert np.array_equal(gs, ws.astype(np.uint32)) and np.array_equal(gd, wd.astype(np.uint32))
+    assert len(ws) == len(_two_hop_rules(n, s, d))
