@@ -141,22 +141,25 @@ __global__ void __launch_bounds__(kBlock) fill_zero_kernel(const FillRange* __re
 }
 
 // Before a run: zero the strips of X_0 that the previous run set (their numeric mask is
-// non-zero) and reset the X_0 masks. X_0 buffers start zeroed at allocation.
+// non-zero) and reset the X_0 masks. X_0 buffers start zeroed at allocation. One wave per
+// strip, over the strips of all panels in turn: every lane reads the strip's mask before
+// lane 0 resets it, so no barrier is needed.
 template <int W>
 __global__ void __launch_bounds__(kBlock) clear_x0_kernel(const PanelInit* __restrict__ panels,
                                                           const uint32_t* __restrict__ n_strips, uint32_t n_panels) {
   constexpr int kWords = 64 * W;
-  uint32_t item = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWaves;
+  const uint32_t first = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave);
   for (uint32_t p = 0; p < n_panels; ++p) {
+    const PanelInit P = panels[p];
     const uint32_t ns = n_strips[p];
-    for (uint32_t y = item; y < ns; y += gridDim.x) {
-      const PanelInit P = panels[p];
+    for (uint32_t y = first; y < ns; y += n_waves) {
       if (P.n0[y] != 0) {
         uint64_t* dst = P.x0 + static_cast<uint64_t>(y) * kWords;
-        for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = 0;
+        for (int i = lane; i < kWords; i += kWave) dst[i] = 0;
       }
-      __syncthreads();
-      if (threadIdx.x == 0) {
+      if (lane == 0) {
         P.n0[y] = 0;
         P.s0[y] = 0;
         P.v0[y] = 0;

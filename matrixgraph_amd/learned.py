@@ -16,7 +16,7 @@ from features known before the run:
 Targets: log1p(jobs)/12 and the fraction of the chunk's 64 x strips output tiles that are
 numerically non-empty. Training samples are logged from the method's own runs
 (mg_walk_batch_set_logging). Predictions only order the launches, so results are
-bit-identical whatever the model says. The exact outcomes of the first run are the judge:
+bit-identical whatever the model says. The exact outcomes of the first run decide:
 a level whose predicted total jobs are off by more than `fallback_ratio` is re-ordered by the
 exact logged costs (fallback), and both the per-chunk misprediction rate and the fallback
 frequency are reported.
