@@ -370,42 +370,61 @@ __global__ void grid_block_scatter_kernel(const uint64_t* __restrict__ keys, uin
   }
 }
 
-// Set bits of every tile (one wave per tile, lane = tile row).
-__global__ void __launch_bounds__(256) tile_bits_count_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
-                                                              uint64_t* __restrict__ count) {
-  const uint32_t lane = lane_id();
-  const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_tiles;
-       i += n_waves) {
-    const uint64_t t = wave_sum(__popcll(data[static_cast<uint64_t>(i) * 64 + lane]));
-    if (lane == 0) count[i] = t;
+// Tile -> edge-list expansion in (source, destination) order without a sort. The tiles of
+// each strip x are cut into pieces of kPieceTiles consecutive tiles (ascending column y); one
+// wave per piece, lane = row r. Counts are laid out in the output order (strip x, row r, piece)
+// so that their exclusive scan gives every (row, piece) its first output slot; the write pass
+// then walks the piece's tiles in order, so each lane emits its row's edges sorted by
+// destination. drop_diagonal leaves out the bits with x*64 + r == y*64 + c.
+constexpr uint32_t kPieceTiles = 64;
+
+struct PieceRef {
+  uint32_t strip, first_tile, n_tiles, n_pieces_in_strip, piece, count_base;
+};
+
+__global__ void __launch_bounds__(256) expand_pieces_kernel(const uint32_t* __restrict__ offset_row, uint32_t n_strips,
+                                                            const uint32_t* __restrict__ piece_off,
+                                                            PieceRef* __restrict__ pieces) {
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n_strips; x += gridDim.x * blockDim.x) {
+    const uint32_t t0 = offset_row[x], t1 = offset_row[x + 1];
+    const uint32_t np = piece_off[x + 1] - piece_off[x];
+    for (uint32_t p = 0; p < np; ++p) {
+      const uint32_t first = t0 + p * kPieceTiles;
+      pieces[piece_off[x] + p] = {x, first, min(kPieceTiles, t1 - first), np, p, piece_off[x] * 64};
+    }
   }
 }
 
-// Every set bit (row r, column c) of tile (x, y) becomes the edge key (x*64 + r) << 32 |
-// (y*64 + c), written at the tile's offset plus the bits of the rows before r.
-__global__ void __launch_bounds__(256) tile_expand_kernel(const uint32_t* __restrict__ row_idx,
+template <bool kWrite>
+__global__ void __launch_bounds__(256) expand_rows_kernel(const PieceRef* __restrict__ pieces, uint32_t n_pieces,
                                                           const uint32_t* __restrict__ col_idx,
-                                                          const uint64_t* __restrict__ data, uint32_t n_tiles,
-                                                          const uint64_t* __restrict__ incl,
+                                                          const uint64_t* __restrict__ data, int drop_diagonal,
+                                                          uint32_t* __restrict__ count,
+                                                          const uint32_t* __restrict__ start,
                                                           uint64_t* __restrict__ out) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_tiles;
+  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_pieces;
        i += n_waves) {
-    uint64_t w = data[static_cast<uint64_t>(i) * 64 + lane];
-    const uint32_t pc = __popcll(w);
-    uint32_t before = pc;  // inclusive prefix over lanes, made exclusive below
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), d, kWave));
-      if (lane >= static_cast<uint32_t>(d)) before += o;
+    const PieceRef pc = pieces[i];
+    const uint64_t slot = static_cast<uint64_t>(pc.count_base) + static_cast<uint64_t>(lane) * pc.n_pieces_in_strip +
+                          pc.piece;
+    const uint64_t src = static_cast<uint64_t>(pc.strip) * 64 + lane;
+    uint64_t pos = kWrite ? start[slot] : 0;
+    uint32_t c_total = 0;
+    for (uint32_t k = 0; k < pc.n_tiles; ++k) {
+      const uint32_t t = pc.first_tile + k;
+      const uint32_t y = col_idx[t];
+      uint64_t w = data[static_cast<uint64_t>(t) * 64 + lane];
+      if (drop_diagonal && y == pc.strip) w &= ~(1ull << lane);
+      if (kWrite) {
+        for (; w; w &= w - 1)
+          out[pos++] = (src << 32) | (static_cast<uint64_t>(y) * 64 + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+      } else {
+        c_total += __popcll(w);
+      }
     }
-    const uint64_t tile_total = incl[i] - (i ? incl[i - 1] : 0ull);
-    uint64_t p = incl[i] - tile_total + (before - pc);
-    const uint64_t src = static_cast<uint64_t>(row_idx[i]) * 64 + lane;
-    const uint64_t dst0 = static_cast<uint64_t>(col_idx[i]) * 64;
-    for (; w; w &= w - 1) out[p++] = (src << 32) | (dst0 + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+    if (!kWrite) count[slot] = c_total;
   }
 }
 
@@ -422,6 +441,8 @@ __global__ void compact_keys_kernel(const uint64_t* __restrict__ keys, uint64_t 
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
     if (flag[i]) out[pos[i]] = keys[i];
 }
+
+std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_matrix* m, bool drop_diagonal);
 
 // The edges of el whose source and destination differ, as a new list (order kept).
 std::unique_ptr<mg_edge_list> without_self_loops(mg_context* ctx, const mg_edge_list* el) {
@@ -463,12 +484,10 @@ extern "C" int mg_edges_two_hop(mg_context* ctx, const mg_edge_list* el, mg_edge
     auto a_el = without_self_loops(ctx, el);
     struct Owned {
       mg_tiled_matrix *a = nullptr, *at = nullptr, *c = nullptr;
-      mg_edge_list* pairs = nullptr;
       ~Owned() {
         if (a) mg_tiled_matrix_destroy(a);
         if (at) mg_tiled_matrix_destroy(at);
         if (c) mg_tiled_matrix_destroy(c);
-        if (pairs) mg_edges_destroy(pairs);
       }
     } o;
     auto check = [](int rc) {
@@ -478,43 +497,71 @@ extern "C" int mg_edges_two_hop(mg_context* ctx, const mg_edge_list* el, mg_edge
     check(mg_edges_to_tiled_matrix(ctx, a_el.get(), 1, &o.at));
     check(mg_multiply_symbolic(ctx, o.a, o.at, &o.c, nullptr));
     check(mg_multiply_numeric(ctx, o.a, o.at, o.c));
-    check(mg_tiled_matrix_to_edges(ctx, o.c, &o.pairs));
-    auto r = without_self_loops(ctx, o.pairs);
+    auto r = tiles_to_edges(ctx, o.c, true);
     r->n_vertices = el->n_vertices;
     *out = r.release();
   });
 }
 
+namespace mgamd {
+namespace {
+
+std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_matrix* m, bool drop_diagonal) {
+  MG_REQUIRE(m->tile_size == 64, "tile -> edge expansion: tile_size must be 64");
+  MG_REQUIRE(static_cast<uint64_t>(m->n_strips) * 64 < (1ull << 32), "tile -> edge expansion: too many vertices");
+  hipStream_t st = ctx->stream;
+  const uint32_t ns = m->n_strips;
+  // Pieces per strip, on the host from the row pointer.
+  std::vector<uint32_t> off(ns + 1ull), piece_off(ns + 1ull, 0);
+  if (ns) MG_HIP(hipMemcpyAsync(off.data(), m->offset_row.get(), sizeof(uint32_t) * (ns + 1ull), hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
+  for (uint32_t x = 0; x < ns; ++x)
+    piece_off[x + 1] = piece_off[x] + (off[x + 1] - off[x] + kPieceTiles - 1) / kPieceTiles;
+  const uint32_t n_pieces = ns ? piece_off[ns] : 0;
+  MG_REQUIRE(static_cast<uint64_t>(n_pieces) * 64 < (1ull << 31), "tile -> edge expansion: too many tiles");
+  auto el = std::make_unique<mg_edge_list>();
+  el->device = ctx->device;
+  el->n_vertices = ns * 64;
+  el->n_edges = 0;
+  if (n_pieces == 0) {
+    el->keys.alloc(1);
+    return el;
+  }
+  DevBuf<uint32_t> d_piece_off(ns + 1ull);
+  MG_HIP(hipMemcpyAsync(d_piece_off.get(), piece_off.data(), sizeof(uint32_t) * (ns + 1ull), hipMemcpyHostToDevice, st));
+  DevBuf<PieceRef> pieces(n_pieces);
+  expand_pieces_kernel<<<grid_for(ns, 256), 256, 0, st>>>(m->offset_row.get(), ns, d_piece_off.get(), pieces.get());
+  MG_LAUNCH_CHECK();
+  const uint32_t n_slots = n_pieces * 64;
+  DevBuf<uint32_t> count(n_slots), start(n_slots + 1ull);
+  const unsigned grid = grid_for(n_pieces, 4, 65536);
+  expand_rows_kernel<false><<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
+                                                  drop_diagonal ? 1 : 0, count.get(), nullptr, nullptr);
+  MG_LAUNCH_CHECK();
+  exclusive_scan_u32(ctx, count.get(), start.get(), n_slots);
+  uint32_t total = 0;
+  MG_HIP(hipMemcpyAsync(&total, start.get() + n_slots, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
+  el->n_edges = total;
+  el->keys.alloc(total ? total : 1);
+  if (total) {
+    expand_rows_kernel<true><<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
+                                                   drop_diagonal ? 1 : 0, nullptr, start.get(), el->keys.get());
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipStreamSynchronize(st));
+  return el;
+}
+
+}  // namespace
+}  // namespace mgamd
+
 extern "C" int mg_tiled_matrix_to_edges(mg_context* ctx, const mg_tiled_matrix* m, mg_edge_list** out) {
   return guarded([&] {
     MG_REQUIRE(ctx && m && out, "mg_tiled_matrix_to_edges: NULL argument");
     *out = nullptr;
-    MG_REQUIRE(m->tile_size == 64, "mg_tiled_matrix_to_edges: tile_size must be 64");
-    MG_REQUIRE(static_cast<uint64_t>(m->n_strips) * 64 < (1ull << 32), "mg_tiled_matrix_to_edges: too many vertices");
     DeviceGuard g(ctx->device);
-    hipStream_t st = ctx->stream;
-    const uint32_t n = m->n_nz_tile;
-    uint64_t total = 0;
-    DevBuf<uint64_t> cnt(n ? n : 1), incl(n ? n : 1);
-    if (n) {
-      tile_bits_count_kernel<<<grid_for(n, 4, 65536), 256, 0, st>>>(m->data.get(), n, cnt.get());
-      MG_LAUNCH_CHECK();
-      size_t tb = 0;
-      MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, cnt.get(), incl.get(), static_cast<int>(n), st));
-      DevBuf<uint8_t> tmp(tb ? tb : 1);
-      MG_HIP(hipcub::DeviceScan::InclusiveSum(tmp.get(), tb, cnt.get(), incl.get(), static_cast<int>(n), st));
-      MG_HIP(hipMemcpyAsync(&total, incl.get() + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-      MG_HIP(hipStreamSynchronize(st));
-    }
-    MG_REQUIRE(total < (1ull << 31), "mg_tiled_matrix_to_edges: more than 2^31 set bits");
-    DevBuf<uint64_t> keys(total ? total : 1);
-    if (total) {
-      tile_expand_kernel<<<grid_for(n, 4, 65536), 256, 0, st>>>(m->row_idx.get(), m->col_idx.get(), m->data.get(), n,
-                                                                incl.get(), keys.get());
-      MG_LAUNCH_CHECK();
-    }
-    *out = finish_edges(ctx, std::move(keys), total, m->n_strips * 64, false).release();
-    MG_HIP(hipStreamSynchronize(st));
+    *out = tiles_to_edges(ctx, m, false).release();
   });
 }
 
