@@ -385,8 +385,12 @@ typedef struct mg_batch_options {
   uint32_t packing;     /* MG_PACK_* */
   uint32_t chunk_tiles; /* scheduler chunk size in A^T tiles (0 = default 2048) */
   uint32_t use_graph;   /* non-zero: capture once, replay as a HIP graph */
-  uint32_t reserved;
+  uint32_t flags;       /* MG_BATCH_* */
 } mg_batch_options;
+enum {
+  MG_BATCH_SWEEP_FIRST_HOP = 1u /* never push a panel's first hop: every level-0 chunk is swept
+                                   (and logged); no column index of A^T is built */
+};
 
 /* graphs_at[g] is graph g supplied transposed (A^T); they must outlive the batch and keep
  * their contents (row-occupancy masks are taken here).
@@ -397,6 +401,10 @@ int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* const* graphs_a
 /* Runs every query (asynchronous; the first call captures the graph). */
 int mg_walk_batch_run(mg_context* ctx, mg_walk_batch* b);
 int mg_walk_batch_info(const mg_walk_batch* b, uint32_t* n_panels, uint32_t* max_hops, uint64_t* n_chunks);
+/* Panels whose first hop is pushed from their source strips' A^T columns (decided per panel
+ * at creation, when those columns hold at most half of the graph's tiles); the level-0
+ * chunk records of mg_walk_batch_level_chunks cover the other panels only. */
+int mg_walk_batch_pushed_panels(const mg_walk_batch* b, uint32_t* n_panels);
 /* Counters of the panel holding query q at hop (its tile products cover all queries of
  * that panel). */
 int mg_walk_batch_query_stats(mg_context* ctx, const mg_walk_batch* b, uint32_t q, uint32_t hop, mg_hop_stats* out);

@@ -391,6 +391,7 @@ WALK_PRUNE = 1
 WALK_SWEEP_FIRST_HOP = 2
 HOP_SWEEP = 0
 HOP_PUSH = 1
+BATCH_SWEEP_FIRST_HOP = 1
 
 
 class Walk:
@@ -524,7 +525,7 @@ class WalkBatch:
     run as one captured HIP graph (mg_walk_batch_*)."""
 
     def __init__(self, ctx: Context, graphs_at, queries, packing: int = 0, chunk_tiles: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, sweep_first_hop: bool = False):
         self.ctx = ctx
         self.graphs = list(graphs_at)
         self.queries = list(queries)
@@ -536,7 +537,8 @@ class WalkBatch:
             arr[i].sources = s.ctypes.data_as(C.POINTER(C.c_uint32))
             arr[i].hops = q.hops
         gp = (C.c_void_p * len(self.graphs))(*[g.h.value if hasattr(g.h, "value") else g.h for g in self.graphs])
-        opt = _capi.BatchOptions(packing, chunk_tiles, 1 if use_graph else 0, 0)
+        opt = _capi.BatchOptions(packing, chunk_tiles, 1 if use_graph else 0,
+                                 BATCH_SWEEP_FIRST_HOP if sweep_first_hop else 0)
         h = C.c_void_p()
         _check(lib().mg_walk_batch_create(ctx.h, gp, len(self.graphs), arr, len(self.queries), C.byref(opt),
                                           C.byref(h)), "mg_walk_batch_create")
@@ -549,6 +551,12 @@ class WalkBatch:
         p, hmax, ch = C.c_uint32(), C.c_uint32(), C.c_uint64()
         _check(lib().mg_walk_batch_info(self.h, C.byref(p), C.byref(hmax), C.byref(ch)), "mg_walk_batch_info")
         return p.value, hmax.value, ch.value
+
+    def pushed_panels(self) -> int:
+        """Panels whose first hop is pushed from their source strips' columns."""
+        n = C.c_uint32()
+        _check(lib().mg_walk_batch_pushed_panels(self.h, C.byref(n)), "mg_walk_batch_pushed_panels")
+        return n.value
 
     def totals(self):
         tp, by, ed = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -52,7 +52,7 @@ class WalkQuery(C.Structure):
 
 
 class BatchOptions(C.Structure):
-    _fields_ = [("packing", u32), ("chunk_tiles", u32), ("use_graph", u32), ("reserved", u32)]
+    _fields_ = [("packing", u32), ("chunk_tiles", u32), ("use_graph", u32), ("flags", u32)]
 
 
 # name -> (restype, argtypes)
@@ -105,6 +105,7 @@ SIGNATURES = {
     "mg_edges_destroy": (i32, [vp]),
     "mg_edges_grid_block": (i32, [vp, vp, u32, u32, u32, C.POINTER(vp)]),
     "mg_tiled_matrix_to_edges": (i32, [vp, vp, C.POINTER(vp)]),
+    "mg_walk_batch_pushed_panels": (i32, [vp, C.POINTER(u32)]),
     "mg_edges_two_hop": (i32, [vp, vp, C.POINTER(vp)]),
     "mg_sample_sources": (i32, [u32, u32, u64, vp]),
     # walks

@@ -81,6 +81,9 @@ struct PanelLevel {
   const uint64_t* rmask;    // per A^T tile: bit r = row r non-zero
   const uint64_t* payload;  // per A^T tile: compact bit list (see tile_payload_kernel)
   const uint64_t* dense;    // rows of the dense A^T tiles, 64 words each, at their payload slot
+  const uint32_t* csc_tile;     // column index of A^T (first-hop push; null unless built)
+  const uint32_t* csc_row;
+  const uint64_t* csc_payload;
   const uint64_t* x_in;
   const uint64_t* v_in;
   const uint64_t* s_in;
@@ -908,16 +911,18 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
 
 // Positions [begin, end) of the A^T column index that belong to source strip `strip`.
 struct PushItem {
-  uint32_t strip, begin, end, pad;
+  uint32_t strip, begin, end, panel;
 };
 constexpr uint32_t kPushItemTiles = 512;   // column positions per item
 constexpr int kPushCap = 128;              // jobs per wave list
 constexpr int kPushUnroll = 8;             // source rows loaded per step
 
-// One wave per strip: zero X_1 words inside the previous Z x N masks, reset X_1's masks.
+// One wave per strip of panel panels[blockIdx.y]: zero X_1 words inside the previous Z x N
+// masks, reset X_1's masks.
 template <int W>
-__global__ void __launch_bounds__(kBlock) push_reset_kernel(const PanelLevel* __restrict__ lv) {
-  const PanelLevel& L = lv[0];
+__global__ void __launch_bounds__(kBlock) push_reset_kernel(const PanelLevel* __restrict__ lv,
+                                                            const uint32_t* __restrict__ panels) {
+  const PanelLevel& L = lv[panels[blockIdx.y]];
   const uint32_t lane = lane_id();
   const gptr<uint64_t> x = as_global(L.x_out);
   const uint32_t n_waves = gridDim.x * kWaves;
@@ -947,8 +952,8 @@ __global__ void __launch_bounds__(kBlock) push_structural_kernel(const Chunk* __
                                                                  const PanelLevel* __restrict__ lv) {
   __shared__ unsigned long long s_sym[kChunkStrips];
   __shared__ unsigned long long s_cnt[kBlockCounters];
-  const PanelLevel& L = lv[0];
   const Chunk ch = chunks[blockIdx.x];
+  const PanelLevel& L = lv[ch.panel];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   const gptr<const uint32_t> col = as_global(L.col);
@@ -1036,19 +1041,16 @@ __device__ inline void push_drain(unsigned long long* __restrict__ jobs, uint32_
 // (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job.
 template <int W>
 __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* __restrict__ items,
-                                                                const PanelLevel* __restrict__ lv,
-                                                                const uint32_t* __restrict__ csc_tile_p,
-                                                                const uint32_t* __restrict__ csc_row_p,
-                                                                const uint64_t* __restrict__ csc_payload_p) {
+                                                                const PanelLevel* __restrict__ lv) {
   __shared__ unsigned long long jobs_all[kWaves][kPushCap];
   __shared__ unsigned long long s_cnt[kBlockCounters];
-  const PanelLevel& L = lv[0];
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   const PushItem it = items[blockIdx.x];
-  const gptr<const uint32_t> csc_tile = as_global(csc_tile_p);
-  const gptr<const uint32_t> csc_row = as_global(csc_row_p);
-  const gptr<const uint64_t> csc_payload = as_global(csc_payload_p);
+  const PanelLevel& L = lv[it.panel];
+  const gptr<const uint32_t> csc_tile = as_global(L.csc_tile);
+  const gptr<const uint32_t> csc_row = as_global(L.csc_row);
+  const gptr<const uint64_t> csc_payload = as_global(L.csc_payload);
   const gptr<const uint64_t> dense_rows = as_global(L.dense);
   const gptr<const uint64_t> rmask = as_global(L.rmask);
   const gptr<const uint64_t> x_in = as_global(L.x_in);
@@ -1131,9 +1133,10 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
 // One wave per strip: N and V masks and popcount of X_1 from the rows marked in Z, and the
 // per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1).
 template <int W>
-__global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel* __restrict__ lv) {
+__global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel* __restrict__ lv,
+                                                               const uint32_t* __restrict__ panels) {
   __shared__ unsigned long long s_cnt[kBlockCounters];
-  const PanelLevel& L = lv[0];
+  const PanelLevel& L = lv[panels[blockIdx.y]];
   const uint32_t lane = lane_id();
   const gptr<const uint64_t> x = as_global(L.x_out);
   uint32_t cnt[kBlockCounters];
@@ -1415,6 +1418,16 @@ struct Engine {
   DevBuf<SourceItem> sources;
   uint32_t n_sources = 0;
   int prune = 0;
+  // First hop by push: panels whose level 0 is pushed (their level-0 chunks go to push_chunks,
+  // which only the structural pass sweeps), their column items and the panel list.
+  std::vector<uint8_t> push0;
+  DevBuf<Chunk> push_chunks;
+  uint32_t n_push_chunks = 0;
+  DevBuf<PushItem> push_items;
+  uint32_t n_push_items = 0;
+  DevBuf<uint32_t> push_panels;
+  uint32_t n_push_panels = 0;
+  uint32_t push_max_strips = 0;
 
   unsigned long long* ctr(uint32_t p, uint32_t level) {
     return counters.get() + (static_cast<size_t>(p) * (max_hops + 1) + level) * kCounterStride;
@@ -1459,7 +1472,7 @@ void build_engine(Engine* e) {
   e->n_split.assign(e->max_hops, 0);
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
-    std::vector<Chunk> ch;
+    std::vector<Chunk> ch, pch;
     std::vector<StripItem> sp;
     for (uint32_t pi = 0; pi < P; ++pi) {
       Panel& p = e->panels[pi];
@@ -1473,6 +1486,9 @@ void build_engine(Engine* e) {
       L.rmask = g.rmask.get();
       L.payload = g.payload.get();
       L.dense = g.dense.get();
+      L.csc_tile = g.csc_tile.get();
+      L.csc_row = g.csc_row.get();
+      L.csc_payload = g.csc_payload.get();
       L.x_in = p.x[k].get();
       L.v_in = p.v[k].get();
       L.s_in = e->prune ? p.n[k].get() : p.s[k].get();
@@ -1485,11 +1501,13 @@ void build_engine(Engine* e) {
       L.alive_in = p.alive[k];
       L.alive_next = (k + 1 < p.hops) ? p.alive[k + 1] : 0;
       L.n_strips = p.n_strips;
+      const bool pushed = k == 0 && pi < e->push0.size() && e->push0[pi];
       for (Chunk c : g.chunks) {
         c.panel = pi;
-        ch.push_back(c);
+        (pushed ? pch : ch).push_back(c);
       }
-      for (uint32_t y : g.split) sp.push_back({pi, y});
+      if (!pushed)
+        for (uint32_t y : g.split) sp.push_back({pi, y});
     }
     // Heaviest chunks first so the tail of the launch is made of light ones. Among equal
     // sizes, the same chunk of every panel of one graph is adjacent, so the panels' reads of
@@ -1509,6 +1527,12 @@ void build_engine(Engine* e) {
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
     if (!ch.empty()) MG_HIP(hipMemcpy(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    if (k == 0) {
+      e->n_push_chunks = static_cast<uint32_t>(pch.size());
+      e->push_chunks.alloc(std::max<size_t>(1, pch.size()));
+      if (!pch.empty())
+        MG_HIP(hipMemcpy(e->push_chunks.get(), pch.data(), sizeof(Chunk) * pch.size(), hipMemcpyHostToDevice));
+    }
     e->n_split[k] = static_cast<uint32_t>(sp.size());
     e->split[k].alloc(std::max<size_t>(1, sp.size()));
     if (!sp.empty()) MG_HIP(hipMemcpy(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice));
@@ -1570,43 +1594,73 @@ void launch_level(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   }
 }
 
-// First hop of a single walk by push (panel 0, level 0); items already on the device.
+// First hop by push for the panels listed in `panels` (level 0): `chunks` are their level-0
+// scheduler chunks (swept by the structural pass only), `items` their source columns; all
+// already on the device. max_strips bounds the strips of any listed panel.
+struct PushLaunch {
+  const PushItem* items;
+  uint32_t n_items;
+  const Chunk* chunks;
+  uint32_t n_chunks;
+  const uint32_t* panels;
+  uint32_t n_panels;
+  uint32_t max_strips;
+};
+
 template <int W>
-void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushItem* items, uint32_t n_items, hipEvent_t ev[4]) {
-  const GraphPlan& g = e->graphs[0];
-  const uint32_t ns = e->panels[0].n_strips;
+void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushLaunch& pl, hipEvent_t ev[4]) {
   if (ev) {
     MG_HIP(hipEventRecord(ev[0], st));
     MG_HIP(hipEventRecord(ev[1], st));
     MG_HIP(hipEventRecord(ev[2], st));
   }
-  push_reset_kernel<W><<<grid_for(ns, kWaves, 4096), kBlock, 0, st>>>(e->lv[0].get());
-  MG_LAUNCH_CHECK();
-  if (e->n_chunks[0]) {
-    push_structural_kernel<<<e->n_chunks[0], kBlock, 0, st>>>(e->chunks[0].get(), e->lv[0].get());
+  if (pl.n_panels) {
+    push_reset_kernel<W><<<dim3(grid_for(pl.max_strips, kWaves, 4096), pl.n_panels), kBlock, 0, st>>>(e->lv[0].get(),
+                                                                                                    pl.panels);
     MG_LAUNCH_CHECK();
   }
-  if (n_items) {
-    push_first_hop_kernel<W><<<n_items, kBlock, 0, st>>>(items, e->lv[0].get(), g.csc_tile.get(), g.csc_row.get(),
-                                                        g.csc_payload.get());
+  if (pl.n_chunks) {
+    push_structural_kernel<<<pl.n_chunks, kBlock, 0, st>>>(pl.chunks, e->lv[0].get());
     MG_LAUNCH_CHECK();
   }
-  push_finalize_kernel<W><<<grid_for(ns, kWaves, 16384), kBlock, 0, st>>>(e->lv[0].get());
-  MG_LAUNCH_CHECK();
+  if (pl.n_items) {
+    push_first_hop_kernel<W><<<pl.n_items, kBlock, 0, st>>>(pl.items, e->lv[0].get());
+    MG_LAUNCH_CHECK();
+  }
+  if (pl.n_panels) {
+    push_finalize_kernel<W><<<dim3(grid_for(pl.max_strips, kWaves, 16384), pl.n_panels), kBlock, 0, st>>>(
+        e->lv[0].get(), pl.panels);
+    MG_LAUNCH_CHECK();
+  }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
 }
 
-void launch_push_first_hop(hipStream_t st, Engine* e, const PushItem* items, uint32_t n_items, hipEvent_t ev[4]) {
+void launch_push_first_hop(hipStream_t st, Engine* e, const PushLaunch& pl, hipEvent_t ev[4]) {
   switch (e->W) {
-    case 1: launch_push_first_hop_w<1>(st, e, items, n_items, ev); break;
-    case 2: launch_push_first_hop_w<2>(st, e, items, n_items, ev); break;
-    case 4: launch_push_first_hop_w<4>(st, e, items, n_items, ev); break;
-    case 8: launch_push_first_hop_w<8>(st, e, items, n_items, ev); break;
-    case 16: launch_push_first_hop_w<16>(st, e, items, n_items, ev); break;
-    case 32: launch_push_first_hop_w<32>(st, e, items, n_items, ev); break;
-    case 64: launch_push_first_hop_w<64>(st, e, items, n_items, ev); break;
+    case 1: launch_push_first_hop_w<1>(st, e, pl, ev); break;
+    case 2: launch_push_first_hop_w<2>(st, e, pl, ev); break;
+    case 4: launch_push_first_hop_w<4>(st, e, pl, ev); break;
+    case 8: launch_push_first_hop_w<8>(st, e, pl, ev); break;
+    case 16: launch_push_first_hop_w<16>(st, e, pl, ev); break;
+    case 32: launch_push_first_hop_w<32>(st, e, pl, ev); break;
+    case 64: launch_push_first_hop_w<64>(st, e, pl, ev); break;
     default: fail(MG_ERR_INTERNAL, "walk: unsupported words per vertex " + std::to_string(e->W));
   }
+}
+
+// Column items of one panel's source strips (sorted, distinct), cut into kPushItemTiles
+// positions; returns the number of A^T tiles in those columns.
+uint64_t push_items_for(const GraphPlan& g, uint32_t panel, const std::vector<uint32_t>& strips,
+                        std::vector<PushItem>* out) {
+  const std::vector<uint32_t>& off = g.host_csc_off;
+  uint64_t tiles = 0;
+  for (uint32_t t : strips) {
+    tiles += off[t + 1] - off[t];
+    if (out)
+      for (uint32_t b = off[t]; b < off[t + 1]; b += kPushItemTiles)
+        out->push_back({t, b, std::min(off[t + 1], b + kPushItemTiles), panel});
+  }
+  return tiles;
 }
 
 // X_0 fills + source bits (sources already on the device).
@@ -1737,15 +1791,19 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     p.words_used = e.W;
     p.alive.assign(max_hops, words_mask(0, e.W));
     e.panels.push_back(std::move(p));
+    // The column index first: the level tables built next point at it.
+    if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) build_column_index(e.graphs[0]);
     build_engine(&e);
     e.sources.alloc(max_sources);
     w->staging.alloc(max_sources);
     if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) {
-      build_column_index(e.graphs[0]);
       // Each source strip's column is cut into pieces of kPushItemTiles positions.
       w->max_push_items = max_sources + at->n_nz_tile / kPushItemTiles + 1;
       w->push_items.alloc(w->max_push_items);
       w->push_staging.alloc(w->max_push_items);
+      const uint32_t zero = 0;
+      e.push_panels.alloc(1);
+      MG_HIP(hipMemcpy(e.push_panels.get(), &zero, sizeof(zero), hipMemcpyHostToDevice));
     }
     MG_HIP(hipEventCreateWithFlags(&w->staging_read, hipEventDisableTiming));
     *out = w.release();
@@ -1790,22 +1848,21 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     uint32_t n_items = 0;
     bool push = false;
     if (w->max_push_items) {
-      const std::vector<uint32_t>& off = w->e.graphs[0].host_csc_off;
       std::vector<uint32_t> strips(n_sources);
       for (uint32_t i = 0; i < n_sources; ++i) strips[i] = sources[i] / 64;
       std::sort(strips.begin(), strips.end());
       strips.erase(std::unique(strips.begin(), strips.end()), strips.end());
-      uint64_t tiles = 0;
-      for (uint32_t t : strips) tiles += off[t + 1] - off[t];
-      push = 2 * tiles <= w->e.graphs[0].at->n_nz_tile;
+      const GraphPlan& g = w->e.graphs[0];
+      push = 2 * push_items_for(g, 0, strips, nullptr) <= g.at->n_nz_tile;
       if (push) {
-        PushItem* it = w->push_staging.get();
-        for (uint32_t t : strips)
-          for (uint32_t b = off[t]; b < off[t + 1]; b += kPushItemTiles)
-            it[n_items++] = {t, b, std::min(off[t + 1], b + kPushItemTiles), 0u};
+        std::vector<PushItem> items;
+        push_items_for(g, 0, strips, &items);
+        n_items = static_cast<uint32_t>(items.size());
         MG_REQUIRE(n_items <= w->max_push_items, "mg_walk_run: internal push item overflow");
+        std::copy(items.begin(), items.end(), w->push_staging.get());
         if (n_items)
-          MG_HIP(hipMemcpyAsync(w->push_items.get(), it, sizeof(PushItem) * n_items, hipMemcpyHostToDevice, st));
+          MG_HIP(hipMemcpyAsync(w->push_items.get(), w->push_staging.get(), sizeof(PushItem) * n_items,
+                                hipMemcpyHostToDevice, st));
       }
     }
     MG_HIP(hipEventRecord(w->staging_read, st));
@@ -1818,7 +1875,10 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     for (uint32_t k = 0; k < hops; ++k) {
       hipEvent_t* const evk = w->timing ? &ev[4 * k] : nullptr;
       if (k == 0 && push)
-        launch_push_first_hop(st, &w->e, w->push_items.get(), n_items, evk);
+        launch_push_first_hop(st, &w->e,
+                              PushLaunch{w->push_items.get(), n_items, w->e.chunks[0].get(), w->e.n_chunks[0],
+                                         w->e.push_panels.get(), 1u, w->e.panels[0].n_strips},
+                              evk);
       else
         launch_level(st, &w->e, k, evk);
     }
@@ -1995,8 +2055,16 @@ struct mg_walk_batch {
 namespace {
 
 void record_batch(hipStream_t st, mg_walk_batch* b) {
-  launch_init(st, &b->e, b->e.n_sources);
-  for (uint32_t k = 0; k < b->e.max_hops; ++k) launch_level(st, &b->e, k, nullptr);
+  Engine& e = b->e;
+  launch_init(st, &e, e.n_sources);
+  for (uint32_t k = 0; k < e.max_hops; ++k) {
+    if (k == 0 && e.n_push_panels)
+      launch_push_first_hop(st, &e,
+                            PushLaunch{e.push_items.get(), e.n_push_items, e.push_chunks.get(), e.n_push_chunks,
+                                       e.push_panels.get(), e.n_push_panels, e.push_max_strips},
+                            nullptr);
+    launch_level(st, &e, k, nullptr);  // the panels whose first hop is swept
+  }
 }
 
 }  // namespace
@@ -2010,7 +2078,9 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     MG_REQUIRE(n_graphs >= 1 && n_queries >= 1, "mg_walk_batch_create: empty batch");
     const uint32_t chunk_tiles = (opt && opt->chunk_tiles) ? opt->chunk_tiles : kDefaultChunkTiles;
     const uint32_t packing = opt ? opt->packing : MG_PACK_BY_HOPS;
+    const uint32_t bflags = opt ? opt->flags : 0u;
     MG_REQUIRE(packing <= MG_PACK_ONE_PER_PANEL, "mg_walk_batch_create: unknown packing");
+    MG_REQUIRE((bflags & ~MG_BATCH_SWEEP_FIRST_HOP) == 0, "mg_walk_batch_create: unknown flags");
     DeviceGuard g(ctx->device);
     auto b = std::make_unique<mg_walk_batch>();
     b->device = ctx->device;
@@ -2089,7 +2159,47 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     }
     for (const QueryPlace& P : b->queries)
       for (uint32_t k = 0; k < P.hops; ++k) e.panels[P.panel].alive[k] |= words_mask(P.w0, P.n_words);
+    // First hop by push, per panel: when the A^T columns of its source strips hold at most
+    // half of its graph's tiles. The column indexes come first: the level tables point at them.
+    std::vector<PushItem> push_items;
+    std::vector<uint32_t> push_panels;
+    if (!(bflags & MG_BATCH_SWEEP_FIRST_HOP)) {
+      std::vector<std::vector<uint32_t>> strips(e.panels.size());
+      for (uint32_t q = 0; q < n_queries; ++q) {
+        const QueryPlace& P = b->queries[q];
+        for (uint32_t i = 0; i < P.n_sources; ++i) strips[P.panel].push_back(queries[q].sources[i] / 64);
+      }
+      std::vector<uint8_t> indexed(e.graphs.size(), 0);
+      e.push0.assign(e.panels.size(), 0);
+      for (uint32_t pi = 0; pi < e.panels.size(); ++pi) {
+        std::vector<uint32_t>& st = strips[pi];
+        std::sort(st.begin(), st.end());
+        st.erase(std::unique(st.begin(), st.end()), st.end());
+        const uint32_t gi = e.panels[pi].graph;
+        GraphPlan& g = e.graphs[gi];
+        if (!indexed[gi]) {
+          build_column_index(g);
+          indexed[gi] = 1;
+        }
+        if (2 * push_items_for(g, pi, st, nullptr) <= g.at->n_nz_tile) {
+          e.push0[pi] = 1;
+          push_items_for(g, pi, st, &push_items);
+          push_panels.push_back(pi);
+          e.push_max_strips = std::max(e.push_max_strips, g.n_strips);
+        }
+      }
+    }
     build_engine(&e);
+    e.n_push_items = static_cast<uint32_t>(push_items.size());
+    e.push_items.alloc(std::max<size_t>(1, push_items.size()));
+    if (!push_items.empty())
+      MG_HIP(hipMemcpy(e.push_items.get(), push_items.data(), sizeof(PushItem) * push_items.size(),
+                       hipMemcpyHostToDevice));
+    e.n_push_panels = static_cast<uint32_t>(push_panels.size());
+    e.push_panels.alloc(std::max<size_t>(1, push_panels.size()));
+    if (!push_panels.empty())
+      MG_HIP(hipMemcpy(e.push_panels.get(), push_panels.data(), sizeof(uint32_t) * push_panels.size(),
+                       hipMemcpyHostToDevice));
     // Sources: all queries' bits, uploaded once.
     std::vector<SourceItem> items;
     for (uint32_t q = 0; q < n_queries; ++q) {
@@ -2127,6 +2237,13 @@ extern "C" int mg_walk_batch_run(mg_context* ctx, mg_walk_batch* b) {
       MG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     MG_HIP(hipGraphLaunch(b->exec, st));
+  });
+}
+
+extern "C" int mg_walk_batch_pushed_panels(const mg_walk_batch* b, uint32_t* n_panels) {
+  return guarded([&] {
+    MG_REQUIRE(b && n_panels, "mg_walk_batch_pushed_panels: NULL argument");
+    *n_panels = b->e.n_push_panels;
   });
 }
 

@@ -69,3 +69,33 @@ def test_batch_export_and_errors(ctx):
     bad = [M.workloads.QuerySpec(7, np.array([1], np.uint32), 2)]
     with pytest.raises(M.MGError):
         M.WalkBatch(ctx, graphs, bad)
+
+
+@pytest.mark.parametrize("packing,n_queries,max_sources", [(0, 12, 60), (1, 30, 300)])
+def test_batch_first_hop_push_equals_sweep(ctx, packing, n_queries, max_sources):
+    """Panels whose first hop is pushed (few source strips on the larger graph) give the same
+    per-query frontiers and batch totals as sweeping every panel's first hop; panels of the
+    small graph stay swept in the same batch."""
+    M = mg()
+    from matrixgraph_amd import workloads as WL
+
+    specs = [WL.GraphSpec("rmat", 16, 51), WL.GraphSpec("er", 12, 52)]
+    graphs = [WL.build_graph(ctx, s)[0] for s in specs]
+    qs = WL.batch_queries(specs, n_queries, 11, max_sources=max_sources, max_hops=4)
+    bp = M.WalkBatch(ctx, graphs, qs, packing=packing)
+    bs = M.WalkBatch(ctx, graphs, qs, packing=packing, sweep_first_hop=True)
+    assert bp.pushed_panels() > 0 and bs.pushed_panels() == 0
+    for b in (bp, bs):
+        b.run()
+        b.run()
+    ctx.synchronize()
+    assert bp.totals() == bs.totals()
+    for i in range(len(qs)):
+        assert bp.query_digest(i) == bs.query_digest(i), i
+    edges = [WL.oracle_edges(s) for s in specs]
+    for i in range(0, len(qs), 5):
+        q = qs[i]
+        n = specs[q.graph].n_vertices
+        xs = O.walk_frontiers(n, edges[q.graph][0], edges[q.graph][1], q.sources, q.hops,
+                              W=(len(q.sources) + 63) // 64)
+        assert np.array_equal(bp.query_frontier(i), xs[q.hops]), i

@@ -19,7 +19,9 @@ def setup(ctx, seeds=(41, 42, 43)):
 def test_logging_records_match_counters(ctx):
     M, WL, specs, graphs, edges = setup(ctx)
     qs = WL.batch_queries(specs, 24, 7, max_sources=600, max_hops=4)
-    b = M.WalkBatch(ctx, graphs, qs, chunk_tiles=64, use_graph=True)
+    # Every level-0 chunk swept (and logged): pushed first hops have no chunk records.
+    b = M.WalkBatch(ctx, graphs, qs, chunk_tiles=64, use_graph=True, sweep_first_hop=True)
+    assert b.pushed_panels() == 0
     b.set_logging(True)
     b.run()
     ctx.synchronize()
