@@ -412,6 +412,7 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
   const uint32_t lane = lane_id();
   const gptr<uint64_t> dst = x_out + static_cast<uint64_t>(y) * kWords;
   if (!split) {
+    // One pass: masks and popcount reduced while the words are stored.
     uint64_t nmask = 0, vmask = 0, zmask = 0, pc = 0;
     for (int i = threadIdx.x; i < kWords; i += BLOCK) {
       const uint64_t val = acc[i];
@@ -421,6 +422,8 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
         if ((alive_next >> (i % W)) & 1) vmask |= 1ull << (i / W);
         pc += __popcll(val);
       }
+      if (val != 0 || (((z_prev >> (i / W)) & 1) && ((n_prev >> (i % W)) & 1)))
+        __builtin_nontemporal_store(val, dst + i);
     }
     nmask = wave_or(nmask);
     vmask = wave_or(vmask);
@@ -432,14 +435,6 @@ __device__ inline void write_strip(const unsigned long long* __restrict__ acc, u
       if (zmask) atomicOr(&s_z, static_cast<unsigned long long>(zmask));
     }
     __syncthreads();
-    if (s_z != 0 || z_prev != 0) {
-      for (int i = threadIdx.x; i < kWords; i += BLOCK) {
-        const uint64_t val = acc[i];
-        if (val != 0 || (((z_prev >> (i / W)) & 1) && ((n_prev >> (i % W)) & 1)))
-          __builtin_nontemporal_store(val, dst + i);
-      }
-    }
-    __syncthreads();  // every thread has read s_n and s_z
     if (threadIdx.x == 0) {
       n_out[y] = s_n;
       v_out[y] = s_v;

@@ -141,8 +141,12 @@ def predict(mlp, X):
     return jobs, occ
 
 
-def order_by_cost(cost):
-    return np.argsort(-cost, kind="stable").astype(np.uint32)
+def order_by_cost(cost, buckets_per_octave: int = 2):
+    """Longest-first launch order over cost buckets (buckets_per_octave per factor of 2); within a
+    bucket the current order is kept, so chunks of the same graph and strip range that the
+    static order places together (and that share cached A^T tiles) stay together."""
+    key = np.floor(np.log2(np.asarray(cost, np.float64) + 1.0) * buckets_per_octave)
+    return np.argsort(-key, kind="stable").astype(np.uint32)
 
 
 def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_ratio=2.0):
