@@ -578,8 +578,9 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
 #define MG_DENSE_BATCH 2
 #endif
 constexpr int kDenseBatch = MG_DENSE_BATCH;  // dense A^T tiles whose rows a wave loads together
-constexpr int kJobCap = 232;     // jobs per wave list (a multiple of 2 * kDrainUnroll)
-constexpr int kDrainUnroll = 4;  // job pairs per drain step (two steps in flight)
+constexpr int kJobCap = 228;     // jobs per wave list (a multiple of 2 * kDrainUnroll)
+constexpr int kDrainUnroll = 3;  // job pairs per drain step (two steps in flight)
+static_assert(kJobCap % (2 * kDrainUnroll) == 0, "job lists are drained in whole steps");
 
 __device__ inline uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
