@@ -1203,15 +1203,17 @@ __global__ void __launch_bounds__(kBlock) frontier_slice_kernel(const uint64_t* 
     out[i] = x[(i / n_words) * W + w0 + (i % n_words)];
 }
 
-// Set bits of each chunk's A^T tiles (from the tile payloads).
+// Set bits of each chunk's A^T tiles (from the tile payloads): those of the chunk's panel
+// when `lv` is given, else those of `payload0` (one graph's plan).
 __global__ void __launch_bounds__(kBlock) chunk_bits_kernel(const Chunk* __restrict__ chunks,
                                                             const PanelLevel* __restrict__ lv,
+                                                            const uint64_t* __restrict__ payload0,
                                                             uint32_t* __restrict__ bits_out) {
   __shared__ unsigned long long s_bits;
   if (threadIdx.x == 0) s_bits = 0;
   __syncthreads();
   const Chunk ch = chunks[blockIdx.x];
-  const uint64_t* payload = lv[ch.panel].payload;
+  const uint64_t* payload = lv ? lv[ch.panel].payload : payload0;
   uint64_t b = 0;
   for (uint32_t p = ch.tile_begin + threadIdx.x; p < ch.tile_end; p += kBlock) {
     const uint64_t pl = payload[p];
@@ -1277,6 +1279,7 @@ struct GraphPlan {
   const mg_tiled_matrix* at = nullptr;
   uint32_t n_strips = 0;
   std::vector<Chunk> chunks;
+  std::vector<uint64_t> chunk_cost;  // launch-order key per chunk (chunk_cost_of)
   std::vector<uint32_t> split;
   DevBuf<uint64_t> rmask;    // row-occupancy mask per A^T tile, taken when the plan is made
   DevBuf<uint64_t> payload;  // compact bit list per A^T tile (tile_payload_kernel)
@@ -1289,6 +1292,29 @@ struct GraphPlan {
   DevBuf<uint64_t> csc_payload;
   std::vector<uint32_t> host_csc_off;
 };
+
+// Launch-order key of each chunk, in hundredths: its set bits + 0.35 per tile + 24 per
+// strip. Set bits count the jobs a full frontier would give, tiles the row-mask and column
+// reads, strips the write-out; heaviest first leaves light chunks for the tail of a launch.
+// Within one allocation this order beat ordering by tiles alone by 2-4 % on hops 2-4 of the
+// 4096-source walk on R-MAT 22 (tiles alone beat strip order by 4 % and reversed by 35 %).
+std::vector<uint64_t> chunk_costs(const GraphPlan& g) {
+  const size_t n = g.chunks.size();
+  std::vector<uint64_t> cost(n, 0);
+  if (!n || !g.at->n_nz_tile) return cost;
+  DevBuf<Chunk> dch(n);
+  DevBuf<uint32_t> bits(n);
+  MG_HIP(hipMemcpy(dch.get(), g.chunks.data(), sizeof(Chunk) * n, hipMemcpyHostToDevice));
+  chunk_bits_kernel<<<static_cast<uint32_t>(n), kBlock>>>(dch.get(), nullptr, g.payload.get(), bits.get());
+  MG_LAUNCH_CHECK();
+  std::vector<uint32_t> hb(n);
+  MG_HIP(hipMemcpy(hb.data(), bits.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    const Chunk& c = g.chunks[i];
+    cost[i] = 100ull * hb[i] + 35ull * (c.tile_end - c.tile_begin) + 2400ull * (c.strip_end - c.strip_begin);
+  }
+  return cost;
+}
 
 GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
   GraphPlan g;
@@ -1320,6 +1346,7 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
     }
     MG_HIP(hipDeviceSynchronize());
   }
+  g.chunk_cost = chunk_costs(g);
   return g;
 }
 
@@ -1469,6 +1496,7 @@ void build_engine(Engine* e) {
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
     std::vector<Chunk> ch, pch;
+    std::vector<uint64_t> cost;  // per entry of ch
     std::vector<StripItem> sp;
     for (uint32_t pi = 0; pi < P; ++pi) {
       Panel& p = e->panels[pi];
@@ -1498,25 +1526,39 @@ void build_engine(Engine* e) {
       L.alive_next = (k + 1 < p.hops) ? p.alive[k + 1] : 0;
       L.n_strips = p.n_strips;
       const bool pushed = k == 0 && pi < e->push0.size() && e->push0[pi];
-      for (Chunk c : g.chunks) {
+      for (size_t i = 0; i < g.chunks.size(); ++i) {
+        Chunk c = g.chunks[i];
         c.panel = pi;
-        (pushed ? pch : ch).push_back(c);
+        if (pushed) {
+          pch.push_back(c);
+        } else {
+          ch.push_back(c);
+          cost.push_back(g.chunk_cost[i]);
+        }
       }
       if (!pushed)
         for (uint32_t y : g.split) sp.push_back({pi, y});
     }
-    // Heaviest chunks first so the tail of the launch is made of light ones. Among equal
-    // sizes, the same chunk of every panel of one graph is adjacent, so the panels' reads of
-    // those A^T tiles follow each other closely and the repeats are served from cache.
-    std::sort(ch.begin(), ch.end(), [&](const Chunk& a, const Chunk& b) {
-      const uint32_t ta = a.tile_end - a.tile_begin, tb = b.tile_end - b.tile_begin;
-      if (ta != tb) return ta > tb;
-      const uint32_t ga = e->panels[a.panel].graph, gb = e->panels[b.panel].graph;
-      if (ga != gb) return ga < gb;
-      if (a.strip_begin != b.strip_begin) return a.strip_begin < b.strip_begin;
-      if (a.tile_begin != b.tile_begin) return a.tile_begin < b.tile_begin;
-      return a.panel < b.panel;
-    });
+    // Heaviest chunks first (chunk_costs) so the tail of the launch is made of light ones.
+    // Among equal costs, the same chunk of every panel of one graph is adjacent, so the
+    // panels' reads of those A^T tiles follow each other closely and the repeats are served
+    // from cache.
+    {
+      std::vector<uint32_t> idx(ch.size());
+      std::iota(idx.begin(), idx.end(), 0u);
+      std::sort(idx.begin(), idx.end(), [&](uint32_t i, uint32_t j) {
+        const Chunk &a = ch[i], &b = ch[j];
+        if (cost[i] != cost[j]) return cost[i] > cost[j];
+        const uint32_t ga = e->panels[a.panel].graph, gb = e->panels[b.panel].graph;
+        if (ga != gb) return ga < gb;
+        if (a.strip_begin != b.strip_begin) return a.strip_begin < b.strip_begin;
+        if (a.tile_begin != b.tile_begin) return a.tile_begin < b.tile_begin;
+        return a.panel < b.panel;
+      });
+      std::vector<Chunk> sorted(ch.size());
+      for (size_t i = 0; i < idx.size(); ++i) sorted[i] = ch[idx[i]];
+      ch.swap(sorted);
+    }
     e->lv[k].alloc(P);
     MG_HIP(hipMemcpy(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice));
     e->n_chunks[k] = static_cast<uint32_t>(ch.size());
@@ -2386,8 +2428,8 @@ extern "C" int mg_walk_batch_level_chunks(mg_context* ctx, const mg_walk_batch* 
     if (!out || ch.empty()) return;
     DeviceGuard g(ctx->device);
     DevBuf<uint32_t> bits(ch.size());
-    chunk_bits_kernel<<<static_cast<uint32_t>(ch.size()), kBlock, 0, ctx->stream>>>(e.chunks[level].get(),
-                                                                                    e.lv[level].get(), bits.get());
+    chunk_bits_kernel<<<static_cast<uint32_t>(ch.size()), kBlock, 0, ctx->stream>>>(
+        e.chunks[level].get(), e.lv[level].get(), nullptr, bits.get());
     MG_LAUNCH_CHECK();
     std::vector<uint32_t> hb(ch.size());
     std::vector<ChunkLog> hl(ch.size(), ChunkLog{0, 0});
