@@ -161,6 +161,34 @@ int mg_io_write_block(const char* dir, uint32_t n_strips, uint32_t n_nz_tile, ui
  * blocks live in dir/block{x*n_chunks+y}/. */
 int mg_io_read_grid_metadata(const char* dir, uint32_t* n_chunks, uint32_t* n_vertices, uint32_t* n_edges);
 int mg_io_write_grid_metadata(const char* dir, uint32_t n_chunks, uint32_t n_vertices, uint32_t n_edges);
+/* CSR-tiled blocks (host only), the input of the reference's enumerating walk:
+ * CSRTiledMatrixIO (core/io/csr_tiled_matrix_io.cu:33-172) with per-tile ImmutableCSR
+ * blobs (core/data_structures/immutable_csr.cu:79-101) as written by
+ * Edgelist2CSRTiledMatrix (core/util/format_converter.cuh:417-755):
+ * dir/meta.yaml (CSRTiledMatrixMetadata with its subgraphs list) +
+ * dir/meta_buf/{row_idx,col_idx,tile_offset_row,nz_tile_bm,csr_offset,data}.bin.
+ * The reader decodes every tile's edges back to global (src, dst) ids; it relies on the
+ * subgraphs list and csr_offset, not on tile order. n_edges = sum of num_outgoing_edges. */
+int mg_io_read_csr_tiled_block_info(const char* dir, uint32_t* n_strips, uint32_t* n_nz_tile, uint32_t* tile_size,
+                                    uint64_t* n_edges);
+/* Fills up to capacity edges (MG_ERR_IO if the block holds more); tile_row/tile_col (the
+ * tile of each edge) may be NULL, as may src/dst. */
+int mg_io_read_csr_tiled_block(const char* dir, uint32_t* src, uint32_t* dst, uint64_t capacity, uint32_t* tile_row,
+                               uint32_t* tile_col);
+/* One block's edges (global ids, all in one grid block) -> dir in the same layout:
+ * n_strips = ceil(block_scope / tile_size), tile_size being the tile's extent in vertices
+ * (the reference converter turns a tile_size argument below 128 into
+ * ceil(block_scope / tile_size) before this point). */
+int mg_io_write_csr_tiled_block(const char* dir, uint32_t block_scope, uint32_t tile_size, const uint32_t* src,
+                                const uint32_t* dst, uint64_t n_edges);
+/* Grid root of a CSR-tiled grid (core/io/grid_csr_tiled_matrix_io.cu; written by
+ * tools/graph_converter/converter/to_grid_csr_tiled_matrix.cuh:54-85): meta.yaml
+ * GridGraphMetadata {n_vertices, n_edges, n_chunks, max_vid} plus label.bin (u32 per
+ * vertex; the writer stores zeros, as the converter does). */
+int mg_io_read_csr_tiled_grid_metadata(const char* dir, uint32_t* n_chunks, uint32_t* n_vertices, uint32_t* n_edges,
+                                       uint32_t* max_vid);
+int mg_io_write_csr_tiled_grid_metadata(const char* dir, uint32_t n_chunks, uint32_t n_vertices, uint32_t n_edges,
+                                        uint32_t max_vid);
 int mg_tiled_matrix_metadata(const mg_tiled_matrix* m, uint32_t* n_strips, uint32_t* n_nz_tile,
                              uint32_t* tile_size);
 /* Copies any non-NULL array back to the host. nz_tile_bm receives

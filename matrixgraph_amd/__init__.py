@@ -518,6 +518,85 @@ def read_grid(path):
 
 
 # ----------------------------------------------------------------------------
+# CSR-tiled grids (host only), the input format of the reference's enumerating walk:
+# core/io/csr_tiled_matrix_io.cu:33-172, core/io/grid_csr_tiled_matrix_io.cu,
+# tools/graph_converter/converter/to_grid_csr_tiled_matrix.cuh:38-85
+# ----------------------------------------------------------------------------
+def csr_tile_extent(block_scope: int, tile_size: int) -> int:
+    """Tile extent (vertices per tile side) the reference converter uses for a tile_size
+    argument: below 128 the argument counts strips and the extent is ceil(block_scope /
+    tile_size) (core/util/format_converter.cuh:424-426; integer ceiling here, float there)."""
+    if block_scope <= 0 or tile_size <= 0:
+        raise MGError(1, "csr_tile_extent: block_scope and tile_size must be positive")
+    return -(-block_scope // tile_size) if tile_size < 128 else tile_size
+
+
+def write_csr_tiled_block(path, src, dst, block_scope: int, tile_extent: int) -> None:
+    """One grid block's edges (global ids) -> path in CSRTiledMatrixIO's layout."""
+    s = np.ascontiguousarray(src, np.uint32)
+    d = np.ascontiguousarray(dst, np.uint32)
+    if s.shape != d.shape:
+        raise MGError(1, "write_csr_tiled_block: src and dst differ in length")
+    _check(lib().mg_io_write_csr_tiled_block(_dir(path), block_scope, tile_extent, _ptr(s), _ptr(d), len(s)),
+           "mg_io_write_csr_tiled_block")
+
+
+def read_csr_tiled_block(path, with_tiles: bool = False):
+    """Edges of one CSR-tiled block as (src, dst, info) in global ids, tile by tile; with
+    with_tiles info also holds each edge's tile row and column."""
+    ns, nt, ts, ne = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+    _check(lib().mg_io_read_csr_tiled_block_info(_dir(path), C.byref(ns), C.byref(nt), C.byref(ts), C.byref(ne)),
+           "mg_io_read_csr_tiled_block_info")
+    n = ne.value
+    s, d = np.empty(n, np.uint32), np.empty(n, np.uint32)
+    tr = np.empty(n, np.uint32) if with_tiles else None
+    tc = np.empty(n, np.uint32) if with_tiles else None
+    _check(lib().mg_io_read_csr_tiled_block(_dir(path), _ptr(s), _ptr(d), n, _ptr(tr) if with_tiles else None,
+                                            _ptr(tc) if with_tiles else None), "mg_io_read_csr_tiled_block")
+    info = {"n_strips": ns.value, "n_nz_tile": nt.value, "tile_size": ts.value, "n_edges": n}
+    if with_tiles:
+        info["tile_row"], info["tile_col"] = tr, tc
+    return s, d, info
+
+
+def write_csr_tiled_grid(path, n_vertices: int, src, dst, n_chunks: int = 1, tile_size: int = 64,
+                         max_vid: int = None) -> None:
+    """Edge list -> CSR-tiled grid as the reference's partitioner + converter lay it out:
+    block scope S = ceil((max_vid + 1) / n_chunks) (tools/graph_partitioner/partitioner/
+    grid_cut.cu:88-89), edge (u, v) in block (u // S) * n_chunks + v // S, every block written
+    (empty ones as meta.yaml only), tile extent csr_tile_extent(S, tile_size)."""
+    s = np.ascontiguousarray(src, np.uint32)
+    d = np.ascontiguousarray(dst, np.uint32)
+    if max_vid is None:
+        max_vid = int(max(s.max(), d.max())) if len(s) else max(n_vertices - 1, 0)
+    if n_chunks <= 0:
+        raise MGError(1, "write_csr_tiled_grid: n_chunks must be positive")
+    scope = -(-(max_vid + 1) // n_chunks)
+    ext = csr_tile_extent(scope, tile_size)
+    bx, by = s // scope, d // scope
+    root = str(path).rstrip("/")
+    for x in range(n_chunks):
+        for y in range(n_chunks):
+            m = (bx == x) & (by == y)
+            write_csr_tiled_block(f"{root}/block{x * n_chunks + y}/", s[m], d[m], scope, ext)
+    _check(lib().mg_io_write_csr_tiled_grid_metadata(_dir(path), n_chunks, n_vertices, len(s), max_vid),
+           "mg_io_write_csr_tiled_grid_metadata")
+
+
+def read_csr_tiled_grid(path):
+    """All blocks of a CSR-tiled grid -> (src, dst, meta): the global edge list, ready for
+    EdgeList.upload(ctx, max(n_vertices, max_vid + 1), src, dst)."""
+    nc, nv, ne, mv = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().mg_io_read_csr_tiled_grid_metadata(_dir(path), C.byref(nc), C.byref(nv), C.byref(ne), C.byref(mv)),
+           "mg_io_read_csr_tiled_grid_metadata")
+    root = str(path).rstrip("/")
+    parts = [read_csr_tiled_block(f"{root}/block{g}/")[:2] for g in range(nc.value * nc.value)]
+    s = np.concatenate([p[0] for p in parts]) if parts else np.empty(0, np.uint32)
+    d = np.concatenate([p[1] for p in parts]) if parts else np.empty(0, np.uint32)
+    return s, d, {"n_chunks": nc.value, "n_vertices": nv.value, "n_edges": ne.value, "max_vid": mv.value}
+
+
+# ----------------------------------------------------------------------------
 # Batched walks
 # ----------------------------------------------------------------------------
 class WalkBatch:

@@ -89,6 +89,11 @@ SIGNATURES = {
     "mg_io_write_block": (i32, [C.c_char_p, u32, u32, u32, vp, vp, vp, vp]),
     "mg_io_read_grid_metadata": (i32, [C.c_char_p, P_u32, P_u32, P_u32]),
     "mg_io_write_grid_metadata": (i32, [C.c_char_p, u32, u32, u32]),
+    "mg_io_read_csr_tiled_block_info": (i32, [C.c_char_p, P_u32, P_u32, P_u32, P_u64]),
+    "mg_io_read_csr_tiled_block": (i32, [C.c_char_p, vp, vp, u64, vp, vp]),
+    "mg_io_write_csr_tiled_block": (i32, [C.c_char_p, u32, u32, vp, vp, u64]),
+    "mg_io_read_csr_tiled_grid_metadata": (i32, [C.c_char_p, P_u32, P_u32, P_u32, P_u32]),
+    "mg_io_write_csr_tiled_grid_metadata": (i32, [C.c_char_p, u32, u32, u32, u32]),
     "mg_multiply_symbolic": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(ProductStats)]),
     "mg_multiply_numeric": (i32, [vp, vp, vp, vp]),
     "mg_multiply_numeric_dense": (i32, [vp, vp, vp, vp, u32, P_u64]),

@@ -186,3 +186,23 @@ def test_two_hop_pairs_match_rules_and_oracle(ctx):
     assert pairs.info() == (n, len(ws))
     assert np.array_equal(gs, ws.astype(np.uint32)) and np.array_equal(gd, wd.astype(np.uint32))
     assert len(ws) == len(_two_hop_rules(n, s, d))
+
+
+@pytest.mark.parametrize("n_chunks,tile_size", [(1, 64), (2, 64), (2, 256)])
+def test_two_hop_from_csr_tiled_grid(ctx, tmp_path, n_chunks, tile_size):
+    """A CSR-tiled grid (the enumerating walk's input format) read back to edges and walked
+    gives the oracle's two-hop pairs of the graph it was written from."""
+    M = mg()
+    scale = 10
+    n = 1 << scale
+    s, d = O.rmat_edges(scale, 8, 0.57, 0.19, 0.19, 11)
+    root = tmp_path / "csr_grid"
+    M.write_csr_tiled_grid(root, n, s, d, n_chunks=n_chunks, tile_size=tile_size)
+    rs, rd, meta = M.read_csr_tiled_grid(root)
+    gs, gd = M.EdgeList.upload(ctx, max(meta["n_vertices"], meta["max_vid"] + 1), rs, rd).two_hop().download()
+    a = np.zeros((n, n), np.uint8)
+    a[s, d] = 1
+    c = O.unpack_rows(O.bitset_matmul(O.pack_rows(a), O.pack_rows(a), n, n, n), n)
+    np.fill_diagonal(c, 0)
+    ws, wd = np.nonzero(c)
+    assert np.array_equal(gs, ws.astype(np.uint32)) and np.array_equal(gd, wd.astype(np.uint32))
