@@ -17,6 +17,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+#ifndef MG_PAIR_BATCH
+#define MG_PAIR_BATCH 4
+#endif
+constexpr int kPairBatch = MG_PAIR_BATCH;  // tile pairs whose loads fill_tiles_kernel issues together
 
 __device__ inline uint64_t tile_product(uint64_t l_row, uint64_t rt_row) {
   uint64_t nz = ballot(rt_row != 0);
@@ -101,14 +105,15 @@ __device__ inline int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t l
 // OR the tile products, OR the result into data_c (FillTiles semantics,
 // core/task/gpu_task/kernel/matrix_operations.cu:318-366 with the intended
 // offsets of B at :356 and the intended 64x64 product of :212-233).
+template <bool kMfma>
 __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
     uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
     uint64_t* __restrict__ data_c, const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt,
     uint32_t min_bits, unsigned long long* __restrict__ dense_count) {
-  // bits_a / bits_bt (may be NULL: VALU only): tile pairs whose two tiles both have at least
-  // min_bits set bits take the matrix-core product.
+  // kMfma: tile pairs whose two tiles both have at least min_bits set bits (bits_a, bits_bt)
+  // take the matrix-core product; otherwise every pair is a VALU product.
   const uint32_t lane = lane_id();
   uint64_t dense = 0;
   const uint32_t n_waves = gridDim.x * kWavesPerBlock;
@@ -128,21 +133,49 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
       int64_t pos = -1;
       if (j < s1) pos = find_sorted(l_col, l0, l1, s_col[j]);
       uint64_t hit = ballot(pos >= 0);
-      while (hit) {
-        const int k = __ffsll(static_cast<unsigned long long>(hit)) - 1;
-        hit &= hit - 1;
-        const uint32_t js = base + k;
-        const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), k));
-        const uint64_t ia = a_short ? js : jl;
-        const uint64_t ib = a_short ? jl : js;
-        const uint64_t l_row = data_a[ia * kTile + lane];
-        const uint64_t rt_row = data_bt[ib * kTile + lane];
-        if (bits_a && bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits) {
-          acc |= tile_product_mfma(l_row, rt_row, lane);
-          ++dense;
-        } else {
-          acc |= tile_product(l_row, rt_row);
+      if constexpr (kMfma) {
+        while (hit) {
+          const int k = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+          hit &= hit - 1;
+          const uint32_t js = base + k;
+          const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), k));
+          const uint64_t ia = a_short ? js : jl;
+          const uint64_t ib = a_short ? jl : js;
+          const uint64_t l_row = data_a[ia * kTile + lane];
+          const uint64_t rt_row = data_bt[ib * kTile + lane];
+          if (bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits) {
+            acc |= tile_product_mfma(l_row, rt_row, lane);
+            ++dense;
+          } else {
+            acc |= tile_product(l_row, rt_row);
+          }
         }
+        continue;
+      }
+      // VALU only: kPairBatch matching pairs at a time, all their tile loads issued before the
+      // first product, so a wave waits once per batch instead of once per pair.
+      while (hit) {
+        uint64_t l_row[kPairBatch], rt_row[kPairBatch];
+        uint32_t n_pairs = 0;  // wave-uniform
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          l_row[q] = 0;
+          rt_row[q] = 0;
+          if (hit) {
+            const int k = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+            hit &= hit - 1;
+            const uint32_t js = base + k;
+            const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), k));
+            const uint64_t ia = a_short ? js : jl;
+            const uint64_t ib = a_short ? jl : js;
+            l_row[q] = data_a[ia * kTile + lane];
+            rt_row[q] = data_bt[ib * kTile + lane];
+            ++n_pairs;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q)
+          if (static_cast<uint32_t>(q) < n_pairs) acc |= tile_product(l_row[q], rt_row[q]);
       }
     }
     uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
@@ -366,9 +399,13 @@ void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const
                        uint64_t* data_c, const uint32_t* bits_a = nullptr, const uint32_t* bits_bt = nullptr,
                        uint32_t min_bits = 0, unsigned long long* dense_count = nullptr) {
   if (n_c == 0) return;
-  fill_tiles_kernel<<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
-      n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, bits_a, bits_bt, min_bits,
-      dense_count);
+  if (bits_a && bits_bt)
+    fill_tiles_kernel<true><<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+        n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, bits_a, bits_bt, min_bits,
+        dense_count);
+  else
+    fill_tiles_kernel<false><<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+        n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, nullptr, nullptr, 0, nullptr);
   MG_LAUNCH_CHECK();
 }
 
