@@ -98,4 +98,26 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Compact payload of a 64 x 64 bit tile with at most kPayloadMax set bits: entry i =
+// (row << 6) | column in bits 12i..12i+11, the entry count in bits 48..50. A tile with more
+// bits gets kPayloadDense and its bit count in the low 13 bits. Called by a whole wave with
+// lane r holding word r (row r) of the tile; returns the payload in every lane.
+constexpr uint32_t kPayloadMax = 4;
+constexpr uint64_t kPayloadDense = 1ull << 63;
+__device__ inline uint64_t tile_payload(uint64_t word) {
+  const uint64_t total = wave_sum(__popcll(word));
+  if (total > kPayloadMax) return kPayloadDense | total;
+  uint64_t pl = 0;
+  uint32_t n = 0;
+  for (uint64_t m = ballot(word != 0); m; m &= m - 1) {
+    const int r = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    for (uint64_t wr = read_lane(word, r); wr; wr &= wr - 1) {
+      const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wr)) - 1);
+      pl |= ((static_cast<uint64_t>(r) << 6) | c) << (12 * n);
+      ++n;
+    }
+  }
+  return pl | (static_cast<uint64_t>(n) << 48);
+}
+
 }  // namespace mgamd

@@ -144,6 +144,9 @@ struct mg_context {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   mgamd::DevBuf<unsigned long long> counters;  // small scratch for reductions (host-synchronised calls)
+  // Per-call device temporaries of asynchronous calls on `stream` (calls on one context run
+  // in stream order, so the next call may reuse them); grown by ctx_scratch.
+  mgamd::DevBuf<uint64_t> scratch;
 };
 
 struct mg_tiled_matrix {
@@ -165,6 +168,16 @@ struct mg_edge_list {
 };
 
 namespace mgamd {
+
+// At least n words of ctx->scratch; growing it first waits for the work queued on the
+// context's stream, which may still read the old buffer.
+inline uint64_t* ctx_scratch(mg_context* ctx, size_t n) {
+  if (ctx->scratch.size() < n) {
+    MG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->scratch.alloc(n);
+  }
+  return ctx->scratch.get();
+}
 
 // Helpers implemented in mg_matrix.hip / mg_graph.hip, shared across units.
 void alloc_tiled_matrix(mg_tiled_matrix* m, uint32_t n_strips, uint32_t n_nz_tile);

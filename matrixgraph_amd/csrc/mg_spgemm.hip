@@ -184,6 +184,125 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
   if (dense_count && lane == 0 && dense) atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(dense));
 }
 
+// 16-byte compact payload of a tile with at most kWideMax set bits: entry i = (row << 6) |
+// column in bits 12(i % 5).. of word i / 5, the entry count in bits 60..63 of word 1; count
+// kWideDense marks a denser tile.
+typedef uint64_t WidePayload __attribute__((ext_vector_type(2)));
+constexpr uint32_t kWideMax = 10;
+constexpr uint32_t kWideDense = 15;
+
+__device__ inline uint32_t wide_count(WidePayload p) { return static_cast<uint32_t>(p.y >> 60); }
+__device__ inline uint32_t wide_entry(WidePayload p, uint32_t i) {
+  return static_cast<uint32_t>((i < 5 ? p.x : p.y) >> (12 * (i < 5 ? i : i - 5))) & 0xFFFu;
+}
+
+// Wide payload of every tile, one wave per tile (lane r holds row r).
+__global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+                                                         WidePayload* __restrict__ payload) {
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
+    const uint64_t word = data[static_cast<uint64_t>(i) * kTile + lane_id()];
+    const uint64_t total = wave_sum(__popcll(word));
+    WidePayload pl = {0, static_cast<uint64_t>(kWideDense) << 60};
+    if (total <= kWideMax) {
+      uint64_t w[2] = {0, 0};
+      uint32_t n = 0;
+      for (uint64_t m = ballot(word != 0); m; m &= m - 1) {
+        const int r = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        for (uint64_t wr = read_lane(word, r); wr; wr &= wr - 1) {
+          const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wr)) - 1);
+          w[n / 5] |= ((static_cast<uint64_t>(r) << 6) | c) << (12 * (n % 5));
+          ++n;
+        }
+      }
+      pl.x = w[0];
+      pl.y = w[1] | (static_cast<uint64_t>(n) << 60);
+    }
+    if (lane_id() == 0) payload[i] = pl;
+  }
+}
+
+// fill_tiles_kernel for operands whose tiles are mostly sparse, from their wide payloads
+// (pl_a, pl_bt). The intersection gives each lane its own tile pair; a pair of two compact
+// tiles is multiplied by that lane alone: every entry (r, j) of the A tile meets every entry
+// (c, j) of the Bt tile with the same j and sets bit c of row r in the wave's LDS
+// accumulator. Pairs with a denser tile take the full-tile wave product, kPairBatch at a time.
+__global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
+    uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
+    const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
+    const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
+    const WidePayload* __restrict__ pl_a, const WidePayload* __restrict__ pl_bt, uint64_t* __restrict__ data_c) {
+  __shared__ unsigned long long s_acc[kWavesPerBlock][kTile];
+  const uint32_t lane = lane_id();
+  unsigned long long* const acc_l = s_acc[threadIdx.x / kWave];
+  acc_l[lane] = 0;
+  wave_lds_sync();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_c; i += n_waves) {
+    const uint32_t x = row_c[i];
+    const uint32_t y = col_c[i];
+    const uint32_t a0 = off_a[x], a1 = off_a[x + 1];
+    const uint32_t b0 = off_bt[y], b1 = off_bt[y + 1];
+    const bool a_short = (a1 - a0) <= (b1 - b0);
+    const uint32_t s0 = a_short ? a0 : b0, s1 = a_short ? a1 : b1;
+    const uint32_t l0 = a_short ? b0 : a0, l1 = a_short ? b1 : a1;
+    const uint32_t* s_col = a_short ? col_a : col_bt;
+    const uint32_t* l_col = a_short ? col_bt : col_a;
+    uint64_t acc = 0;
+    for (uint32_t base = s0; base < s1; base += kWave) {
+      const uint32_t j = base + lane;
+      int64_t pos = -1;
+      if (j < s1) pos = find_sorted(l_col, l0, l1, s_col[j]);
+      const bool has = pos >= 0;
+      const uint32_t ia = has ? (a_short ? j : static_cast<uint32_t>(pos)) : 0u;
+      const uint32_t ib = has ? (a_short ? static_cast<uint32_t>(pos) : j) : 0u;
+      WidePayload pa = {0, 0}, pb = {0, 0};
+      if (has) {
+        pa = pl_a[ia];
+        pb = pl_bt[ib];
+      }
+      const uint32_t na = wide_count(pa), nb = wide_count(pb);
+      const bool dense = has && (na == kWideDense || nb == kWideDense);
+      if (has && !dense) {
+        for (uint32_t u = 0; u < na; ++u) {
+          const uint32_t ea = wide_entry(pa, u);
+          for (uint32_t v = 0; v < nb; ++v) {
+            const uint32_t eb = wide_entry(pb, v);
+            if (((ea ^ eb) & 63u) == 0) atomicOr(acc_l + (ea >> 6), 1ull << (eb >> 6));
+          }
+        }
+      }
+      for (uint64_t dm = ballot(dense); dm;) {
+        uint64_t l_row[kPairBatch], rt_row[kPairBatch];
+        uint32_t n_pairs = 0;  // wave-uniform
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          l_row[q] = 0;
+          rt_row[q] = 0;
+          if (dm) {
+            const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+            dm &= dm - 1;
+            const uint64_t ka = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ia), k));
+            const uint64_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ib), k));
+            l_row[q] = data_a[ka * kTile + lane];
+            rt_row[q] = data_bt[kb * kTile + lane];
+            ++n_pairs;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q)
+          if (static_cast<uint32_t>(q) < n_pairs) acc |= tile_product(l_row[q], rt_row[q]);
+      }
+    }
+    wave_lds_sync();
+    acc |= acc_l[lane];
+    acc_l[lane] = 0;
+    wave_lds_sync();
+    uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
+    if (acc) *dst |= acc;
+  }
+}
+
 // len (<= 64) bits of a bit-packed array starting at bit `start`.
 __device__ inline uint64_t bit_window(const uint64_t* __restrict__ p, uint64_t start, uint32_t len) {
   const uint64_t w = start >> 6;
@@ -558,9 +677,27 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric: operands differ in n_strips");
     DeviceGuard g(ctx->device);
-    launch_fill_tiles(ctx->stream, c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(),
-                      c->col_idx.get(), a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(),
-                      c->data.get());
+    if (c->n_nz_tile == 0) return;
+    // Payloads of both operands (stream-ordered in the context's scratch), then the product.
+    hipStream_t st = ctx->stream;
+    WidePayload* pl = reinterpret_cast<WidePayload*>(
+        ctx_scratch(ctx, 2 * (static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1)));
+    WidePayload* pl_a = pl;
+    WidePayload* pl_bt = pl + a->n_nz_tile;
+    if (a->n_nz_tile) {
+      payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
+                                                                                       pl_a);
+      MG_LAUNCH_CHECK();
+    }
+    if (bt->n_nz_tile) {
+      payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(),
+                                                                                        bt->n_nz_tile, pl_bt);
+      MG_LAUNCH_CHECK();
+    }
+    fill_tiles_payload_kernel<<<grid_for(c->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(
+        c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
+        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get());
+    MG_LAUNCH_CHECK();
   });
 }
 

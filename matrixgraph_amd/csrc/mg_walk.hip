@@ -196,13 +196,9 @@ __global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __rest
   }
 }
 
-// Compact payload of an A^T tile with at most kPayloadMax set bits: entry i = (row << 6) |
-// column in bits 12i..12i+11 and the entry count in bits 48..50. A tile with more bits has
-// kPayloadDense set, its bit count in the low 13 bits and, once dense_compact_kernel has run,
-// the index of its copy in the dense-row array in bits 13..44; it is read row by row through
-// its row-occupancy mask.
-constexpr uint32_t kPayloadMax = 4;
-constexpr uint64_t kPayloadDense = 1ull << 63;
+// Compact payload of every A^T tile (tile_payload). A dense tile's payload also receives,
+// once dense_compact_kernel has run, the index of its copy in the dense-row array in bits
+// 13..44; it is read row by row through its row-occupancy mask.
 
 __global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
                                                               uint64_t* __restrict__ payload) {
@@ -210,21 +206,7 @@ __global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __
   const uint32_t n_waves = gridDim.x * kWaves;
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
     const uint64_t word = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
-    const uint64_t total = wave_sum(__popcll(word));
-    uint64_t pl = kPayloadDense | total;
-    if (total <= kPayloadMax) {
-      pl = 0;
-      uint32_t n = 0;
-      for (uint64_t m = ballot(word != 0); m; m &= m - 1) {
-        const int r = __ffsll(static_cast<unsigned long long>(m)) - 1;
-        for (uint64_t wr = read_lane(word, r); wr; wr &= wr - 1) {
-          const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(wr)) - 1);
-          pl |= ((static_cast<uint64_t>(r) << 6) | c) << (12 * n);
-          ++n;
-        }
-      }
-      pl |= static_cast<uint64_t>(n) << 48;
-    }
+    const uint64_t pl = tile_payload(word);
     if (lane == 0) payload[i] = pl;
   }
 }
