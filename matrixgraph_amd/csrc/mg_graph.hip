@@ -409,7 +409,6 @@ __global__ void __launch_bounds__(256) expand_rows_kernel(const PieceRef* __rest
     const PieceRef pc = pieces[i];
     const uint64_t slot = static_cast<uint64_t>(pc.count_base) + static_cast<uint64_t>(lane) * pc.n_pieces_in_strip +
                           pc.piece;
-    const uint64_t src = static_cast<uint64_t>(pc.strip) * 64 + lane;
     uint64_t pos = kWrite ? start[slot] : 0;
     uint32_t c_total = 0;
     for (uint32_t k = 0; k < pc.n_tiles; ++k) {
@@ -418,8 +417,20 @@ __global__ void __launch_bounds__(256) expand_rows_kernel(const PieceRef* __rest
       uint64_t w = data[static_cast<uint64_t>(t) * 64 + lane];
       if (drop_diagonal && y == pc.strip) w &= ~(1ull << lane);
       if (kWrite) {
-        for (; w; w &= w - 1)
-          out[pos++] = (src << 32) | (static_cast<uint64_t>(y) * 64 + (__ffsll(static_cast<unsigned long long>(w)) - 1));
+        // Row by row, the whole wave writes the row's pairs as one run (lane = column), so
+        // every store covers consecutive addresses.
+        for (uint64_t rows = ballot(w != 0); rows; rows &= rows - 1) {
+          const int r = __ffsll(static_cast<unsigned long long>(rows)) - 1;
+          const uint64_t m = read_lane(w, r);
+          const uint64_t p = read_lane(pos, r);
+          if ((m >> lane) & 1) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            out[p + below] = ((static_cast<uint64_t>(pc.strip) * 64 + static_cast<uint64_t>(r)) << 32) |
+                             (static_cast<uint64_t>(y) * 64 + lane);
+          }
+          if (lane == static_cast<uint32_t>(r)) pos += __popcll(m);
+        }
       } else {
         c_total += __popcll(w);
       }
