@@ -78,6 +78,31 @@ def test_dense_and_closed_form_tiles(ctx):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("seed", [7, 8])
+def test_tiles_around_compact_limit(ctx, seed):
+    """Tiles of 1..12 bits (the numeric pass multiplies tiles of up to 10 bits from their
+    compact entry lists, one lane per pair, and denser ones as whole tiles), 40-bit and full
+    tiles, with empty tiles between them: C = A * A equals a brute-force Boolean product."""
+    rng = np.random.default_rng(seed)
+    ns = 8
+    n = ns * 64
+    counts = [1, 2, 3, 9, 10, 11, 12, 40, 4096]
+    dense = np.zeros((n, n), np.uint8)
+    for x in range(ns):
+        for y in range(ns):
+            if rng.random() < 0.25:
+                continue
+            c = counts[int(rng.integers(len(counts)))]
+            cells = rng.choice(4096, size=c, replace=False)
+            dense[x * 64 + cells // 64, y * 64 + cells % 64] = 1
+    s, d = np.nonzero(dense)
+    a, at, c, st, _ = gpu_square(ctx, n, s.astype(np.uint32), d.astype(np.uint32))
+    got = host(c.n_strips, c.offset_row, c.row_idx, c.col_idx, c.data).to_dense(n)
+    assert np.array_equal(got, O.bool_matmul_bruteforce(dense, dense))
+    c_off, c_row, c_col, c_data, tp = oracle_square(a, at)
+    assert np.array_equal(c.data, c_data) and st["tile_products"] == tp
+
+
 def test_empty_operand(ctx):
     M = mg()
     s, d = O.rmat_edges(8, 16, 0.57, 0.19, 0.19, 1)
