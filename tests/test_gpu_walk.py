@@ -135,6 +135,51 @@ def test_walk_cycle_closed_form_and_empty_graph(ctx):
     assert not we.frontier(1).any() and we.hop_stats(1)["tile_products"] == 0
 
 
+def ragged_graph():
+    """1000 vertices (the last strip is partial): three DAG layers 0-99 -> 100-199 -> 200-299 ->
+    300-399 whose walks end after three hops, a complete 64 x 64 block on 640-703 (dense
+    tiles), self-loops, and edges into the last vertex."""
+    rng = np.random.default_rng(3)
+    src, dst = [], []
+    for layer in range(3):
+        src += rng.integers(layer * 100, layer * 100 + 100, 400).tolist()
+        dst += rng.integers(layer * 100 + 100, layer * 100 + 200, 400).tolist()
+    blk = np.arange(640, 704)
+    src += np.repeat(blk, 64).tolist()
+    dst += np.tile(blk, 64).tolist()
+    src += [999, 500, 640, 998, 997]
+    dst += [999, 500, 999, 999, 999]
+    return O.sort_unique_edges(np.array(src, np.uint32), np.array(dst, np.uint32))
+
+
+def test_walk_ragged_dag_dense_block_and_last_vertex(ctx):
+    """Every hop equals the oracle on the ragged graph; the DAG sources' walks are empty after
+    three hops; padding rows stay zero; digests match the oracle's over the padded rows."""
+    M = mg()
+    n = 1000
+    s, d = ragged_graph()
+    at = M.EdgeList.upload(ctx, n, s, d).to_tiled_matrix(transpose=True)
+    sources = np.array([0, 999, 5, 650, 320, 640, 998] + list(range(50, 90)), np.uint32)
+    hops = 6
+    w = M.Walk(ctx, at, len(sources), hops)
+    w.run(sources, hops)
+    W = w.words_per_vertex
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=W)
+    for k in range(1, hops + 1):
+        got = w.frontier(k)
+        assert np.array_equal(got[:n], xs[k]), f"hop {k}"
+        assert not got[n:].any()
+        padded = np.zeros_like(got)
+        padded[:n] = xs[k]
+        assert w.frontier_digest(k) == O.frontier_digest(padded, len(sources))
+        assert w.hop_stats(k)["popcount"] == popcount(xs[k])
+    dag = [i for i, v in enumerate(sources.tolist()) if v < 400 and v not in (320,)]
+    for k in range(4, hops + 1):
+        x = xs[k]
+        assert not any((int(x[v, i // 64]) >> (i % 64)) & 1 for i in dag for v in range(n))
+    assert xs[hops][999, 1 // 64] >> np.uint64(1) & np.uint64(1)  # the last vertex keeps its self-loop walk
+
+
 def test_walk_rerun_smaller_and_errors(ctx):
     M = mg()
     n = 1 << 12
