@@ -233,7 +233,9 @@ typedef struct mg_product_stats {
 int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                          mg_tiled_matrix** c, mg_product_stats* stats);
 /* Numeric pass into a C produced by mg_multiply_symbolic (OR-accumulates).
- * Replaces PPRQuery::FillTilesUnifiedMemory (ppr_query.cu:459-780). */
+ * Replaces PPRQuery::FillTilesUnifiedMemory (ppr_query.cu:459-780). Asynchronous on the
+ * context's stream; keeps 16 bytes per operand tile in the context's scratch, whose growth
+ * (first call, or larger operands) waits for the work already queued on that stream. */
 int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                         mg_tiled_matrix* c);
 /* The numeric pass with a per-tile-pair choice of engine: a pair whose two tiles both have
