@@ -63,7 +63,8 @@ const char* mg_last_error(void);
  * (replaces set_device_from_env, core/go_api/matrixgraph_go_api.cu:23-36). */
 int mg_context_create(int device, mg_context** out);
 int mg_context_destroy(mg_context* ctx);
-/* stream: a hipStream_t, or NULL for the context's own stream. */
+/* stream: a hipStream_t, or NULL for the context's own stream. Switching to another stream
+ * first waits for the work queued on the previous one. */
 int mg_context_set_stream(mg_context* ctx, void* stream);
 int mg_context_synchronize(mg_context* ctx);
 int mg_context_device(const mg_context* ctx, int* device, int* n_compute_units);

@@ -71,7 +71,14 @@ extern "C" int mg_context_destroy(mg_context* ctx) {
 extern "C" int mg_context_set_stream(mg_context* ctx, void* stream) {
   return guarded([&] {
     MG_REQUIRE(ctx != nullptr, "mg_context_set_stream: ctx is NULL");
-    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    const hipStream_t next = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    if (next != ctx->stream) {
+      // Work queued on the previous stream may still read the context's scratch, which the
+      // next call (ordered on the new stream only) may overwrite or regrow.
+      DeviceGuard g(ctx->device);
+      MG_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->stream = next;
   });
 }
 
