@@ -103,6 +103,52 @@ def test_tiles_around_compact_limit(ctx, seed):
     assert np.array_equal(c.data, c_data) and st["tile_products"] == tp
 
 
+def _hip_runtime():
+    import ctypes
+
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            continue
+    pytest.skip("HIP runtime library not found")
+
+
+def test_numeric_pass_on_a_caller_stream_then_back(ctx):
+    """The numeric pass queued on a caller's stream, then the context switched back to its own
+    stream (the switch waits for the caller's stream, whose pass reads the context's scratch)
+    and a second, smaller product run there: both equal the oracle."""
+    import ctypes
+
+    M = mg()
+    hip = _hip_runtime()
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        results = []
+        for k, (scale, seed) in enumerate([(12, 21), (11, 22)]):
+            s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, seed)
+            a = O.edges_to_tiles(1 << scale, s, d)
+            at = O.edges_to_tiles(1 << scale, s, d, transpose=True)
+            ctx.set_stream(stream.value if k == 0 else None)
+            ga = M.TiledMatrix.upload(ctx, host(*a))
+            gat = M.TiledMatrix.upload(ctx, host(*at))
+            c, _ = M.multiply_symbolic(ctx, ga, gat)
+            M.multiply_numeric(ctx, ga, gat, c)
+            results.append((a, at, c, ga, gat))
+        ctx.set_stream(None)
+        ctx.synchronize()
+        for a, at, c, _, _ in results:
+            c_off, c_row, c_col, c_data, _ = oracle_square(a, at)
+            got = c.download()
+            assert np.array_equal(got.row_idx, c_row) and np.array_equal(got.col_idx, c_col)
+            assert np.array_equal(got.data, c_data)
+    finally:
+        ctx.set_stream(None)
+        ctx.synchronize()
+        assert hip.hipStreamDestroy(stream) == 0
+
+
 def test_empty_operand(ctx):
     M = mg()
     s, d = O.rmat_edges(8, 16, 0.57, 0.19, 0.19, 1)
