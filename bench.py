@@ -107,10 +107,11 @@ def gather_floor(M, ctx, el, at, n_vertices, args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
-    """Time the oracle's tile algorithm (OpenMP) on every `stride`-th strip of each hop,
-    using the GPU's X_{k-1} as input, and scale each hop by total / sampled A^T bits.
-    Also checks the sampled strips against the GPU frontier (bit-exact)."""
+def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads, reps=1):
+    """Time the oracle's tile algorithm (OpenMP) on every `stride`-th strip of each hop
+    (stride 1: the whole walk), using the GPU's X_{k-1} as input, `reps` times, and scale each
+    hop by total / sampled A^T bits. Also checks the sampled strips against the GPU frontier
+    (bit-exact)."""
     ath = at.download()
     ns = ath.n_strips
     strips = np.arange(0, ns, stride)
@@ -120,11 +121,15 @@ def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
         x_in = w.frontier(k - 1)
         x_ref = w.frontier(k)
         x_out = np.zeros_like(x_in)
-        t0 = time.perf_counter()
-        _, visited = O.walk_tile_hop(ns, ath.offset_row, ath.col_idx, ath.data, x_in, 0, ns, threads, x_out=x_out,
-                                     stride=stride)
-        dt = time.perf_counter() - t0
+        dt = 0.0
+        for _ in range(reps):
+            x_out[:] = 0
+            t0 = time.perf_counter()
+            _, visited = O.walk_tile_hop(ns, ath.offset_row, ath.col_idx, ath.data, x_in, 0, ns, threads,
+                                         x_out=x_out, stride=stride)
+            dt += time.perf_counter() - t0
         sample_s += dt
+        dt /= reps
         rows = (strips[:, None] * 64 + np.arange(64)[None, :]).ravel()
         if not np.array_equal(x_out[rows], x_ref[rows]):
             raise SystemExit(f"cpu_baseline: CPU tile algorithm disagrees with the GPU at hop {k}")
@@ -135,9 +140,12 @@ def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads):
         "unit": "source-walks/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle tile algorithm (OpenMP, {threads} threads) on every {stride}-th output strip of each of "
-                  f"the {hops} hops ({len(strips)} strips/hop, {sample_s:.1f}s measured), scaled per hop by total/"
-                  f"sampled A^T bits; sampled strips checked bit-exact against the GPU",
+        "sample": (f"oracle tile algorithm (OpenMP, {threads} threads) on every output strip of each of the {hops} "
+                   f"hops (the whole walk), {reps} times, {sample_s:.1f}s measured; every output row checked bit-exact "
+                   f"against the GPU") if stride == 1 else
+                  (f"oracle tile algorithm (OpenMP, {threads} threads) on every {stride}-th output strip of each of "
+                   f"the {hops} hops ({len(strips)} strips/hop, {reps} times, {sample_s:.1f}s measured), scaled per "
+                   f"hop by total/sampled A^T bits; sampled strips checked bit-exact against the GPU"),
         "est_seconds_per_walk": est_total,
     }
 
@@ -367,7 +375,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--sources", type=int, default=4096)
     ap.add_argument("--hops", type=int, default=4)
-    ap.add_argument("--cpu-stride", type=int, default=64, help="CPU baseline samples every n-th strip")
+    ap.add_argument("--cpu-stride", type=int, default=1, help="CPU baseline samples every n-th strip (1: all)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="CPU baseline repetitions (timings averaged)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
@@ -515,7 +524,7 @@ def main():
 
         threads = O.max_threads()
         result["cpu_baseline"] = cpu_baseline(M, O, ctx, w, at, n_edges, args.hops, args.sources, args.cpu_stride,
-                                              threads)
+                                              threads, reps=max(1, args.cpu_reps))
     if not args.no_dense:
         result["dense"] = dense_section(M, ctx)
     if not args.no_batch:
