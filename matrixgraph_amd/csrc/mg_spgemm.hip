@@ -6,7 +6,9 @@
 // Tile product on a wave64: lane r holds L[r] (row r of the left tile), lane c holds
 // Rt[c] (row c of the transposed right tile); output row r bit c = (L[r] & Rt[c]) != 0.
 // The loop runs only over non-zero Rt rows (wave ballot), so sparse tiles cost a few
-// instructions; dense tiles (64 non-zero rows) take 64 iterations.
+// instructions; dense tiles (64 non-zero rows) take 64 iterations. The handle-based numeric
+// pass multiplies pairs of tiles with at most 10 bits from their entry lists instead, one
+// pair per lane (fill_tiles_payload_kernel), and keeps the wave product for denser tiles.
 #include <string>
 
 #include "mg_device.hpp"
