@@ -131,9 +131,10 @@ def sources_matrix(ctx, n_vertices: int, sources):
     return x
 
 
-def product_walk(ctx, at, n_vertices: int, sources, hops: int, min_bits: int):
+def product_walk(ctx, at, n_vertices: int, sources, hops: int, min_bits):
     """X_{k+1} = X_k * A by tile products (symbolic pass, then the numeric pass with
-    matrix-core products for tile pairs whose tiles both have >= min_bits bits).
+    matrix-core products for tile pairs whose tiles both have >= min_bits bits; min_bits None:
+    mg_multiply_numeric, compact tiles multiplied from their entry lists).
     Returns the list X_0..X_hops and per-hop stats (tile products, matrix-core products,
     symbolic and numeric milliseconds)."""
     import time
@@ -148,7 +149,12 @@ def product_walk(ctx, at, n_vertices: int, sources, hops: int, min_bits: int):
         c, st = M.multiply_symbolic(ctx, xs[-1], at)
         ctx.synchronize()
         t1 = time.perf_counter()
-        dense = M.multiply_numeric_dense(ctx, xs[-1], at, c, min_bits)
+        if min_bits is None:
+            M.multiply_numeric(ctx, xs[-1], at, c)
+            ctx.synchronize()
+            dense = 0
+        else:
+            dense = M.multiply_numeric_dense(ctx, xs[-1], at, c, min_bits)
         t2 = time.perf_counter()
         stats.append(dict(st, dense_products=dense, symbolic_ms=(t1 - t0) * 1e3, numeric_ms=(t2 - t1) * 1e3))
         xs.append(c)
