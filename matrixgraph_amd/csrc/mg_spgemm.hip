@@ -55,7 +55,9 @@ __device__ inline long bits_to_bytes(uint64_t x, uint32_t k0) {
 }
 
 // Same contract as tile_product: lane r holds L[r] and Rt[r]; returns output row `lane`.
-__device__ inline uint64_t tile_product_mfma(uint64_t l_row, uint64_t rt_row, uint32_t lane) {
+// Kept out of line: inlined into the pair loops its unrolled blocks would hold registers
+// across the whole kernel.
+__device__ __attribute__((noinline)) uint64_t tile_product_mfma(uint64_t l_row, uint64_t rt_row, uint32_t lane) {
   const uint32_t lo = lane & 31, hi = lane >> 5;
   uint64_t out = 0;
 #pragma unroll
@@ -106,18 +108,14 @@ __device__ inline int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t l
 // One wave per tile of C: intersect A.row(x) with Bt.row(y) on the tile column,
 // OR the tile products, OR the result into data_c (FillTiles semantics,
 // core/task/gpu_task/kernel/matrix_operations.cu:318-366 with the intended
-// offsets of B at :356 and the intended 64x64 product of :212-233).
-template <bool kMfma>
+// offsets of B at :356 and the intended 64x64 product of :212-233). Whole-tile products on
+// the vector ALUs (the layer-1 entry point, whose operands are raw tile arrays).
 __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
     uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
-    uint64_t* __restrict__ data_c, const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt,
-    uint32_t min_bits, unsigned long long* __restrict__ dense_count) {
-  // kMfma: tile pairs whose two tiles both have at least min_bits set bits (bits_a, bits_bt)
-  // take the matrix-core product; otherwise every pair is a VALU product.
+    uint64_t* __restrict__ data_c) {
   const uint32_t lane = lane_id();
-  uint64_t dense = 0;
   const uint32_t n_waves = gridDim.x * kWavesPerBlock;
   for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_c; i += n_waves) {
     const uint32_t x = row_c[i];
@@ -135,27 +133,8 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
       int64_t pos = -1;
       if (j < s1) pos = find_sorted(l_col, l0, l1, s_col[j]);
       uint64_t hit = ballot(pos >= 0);
-      if constexpr (kMfma) {
-        while (hit) {
-          const int k = __ffsll(static_cast<unsigned long long>(hit)) - 1;
-          hit &= hit - 1;
-          const uint32_t js = base + k;
-          const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pos), k));
-          const uint64_t ia = a_short ? js : jl;
-          const uint64_t ib = a_short ? jl : js;
-          const uint64_t l_row = data_a[ia * kTile + lane];
-          const uint64_t rt_row = data_bt[ib * kTile + lane];
-          if (bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits) {
-            acc |= tile_product_mfma(l_row, rt_row, lane);
-            ++dense;
-          } else {
-            acc |= tile_product(l_row, rt_row);
-          }
-        }
-        continue;
-      }
-      // VALU only: kPairBatch matching pairs at a time, all their tile loads issued before the
-      // first product, so a wave waits once per batch instead of once per pair.
+      // kPairBatch matching pairs at a time, all their tile loads issued before the first
+      // product, so a wave waits once per batch instead of once per pair.
       while (hit) {
         uint64_t l_row[kPairBatch], rt_row[kPairBatch];
         uint32_t n_pairs = 0;  // wave-uniform
@@ -183,7 +162,6 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
     uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
     if (acc) *dst |= acc;
   }
-  if (dense_count && lane == 0 && dense) atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(dense));
 }
 
 // 16-byte compact payload of a tile with at most kWideMax set bits: entry i = (row << 6) |
@@ -229,13 +207,19 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
 // tiles is multiplied by that lane alone: every entry (r, j) of the A tile meets every entry
 // (c, j) of the Bt tile with the same j and sets bit c of row r in the wave's LDS
 // accumulator. Pairs with a denser tile take the full-tile wave product, kPairBatch at a time.
+// kMfma: a pair whose two tiles both have at least min_bits set bits (bits_a, bits_bt) is
+// taken first, by the matrix cores, one pair at a time; dense_count counts those pairs.
+template <bool kMfma>
 __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
     uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
-    const WidePayload* __restrict__ pl_a, const WidePayload* __restrict__ pl_bt, uint64_t* __restrict__ data_c) {
+    const WidePayload* __restrict__ pl_a, const WidePayload* __restrict__ pl_bt, uint64_t* __restrict__ data_c,
+    const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt, uint32_t min_bits,
+    unsigned long long* __restrict__ dense_count) {
   __shared__ unsigned long long s_acc[kWavesPerBlock][kTile];
   const uint32_t lane = lane_id();
+  uint64_t n_mfma = 0;
   unsigned long long* const acc_l = s_acc[threadIdx.x / kWave];
   acc_l[lane] = 0;
   wave_lds_sync();
@@ -264,14 +248,25 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
         pb = pl_bt[ib];
       }
       const uint32_t na = wide_count(pa), nb = wide_count(pb);
-      const bool dense = has && (na == kWideDense || nb == kWideDense);
-      if (has && !dense) {
+      bool mfma = false;
+      if constexpr (kMfma) mfma = has && bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits;
+      const bool dense = has && !mfma && (na == kWideDense || nb == kWideDense);
+      if (has && !mfma && !dense) {
         for (uint32_t u = 0; u < na; ++u) {
           const uint32_t ea = wide_entry(pa, u);
           for (uint32_t v = 0; v < nb; ++v) {
             const uint32_t eb = wide_entry(pb, v);
             if (((ea ^ eb) & 63u) == 0) atomicOr(acc_l + (ea >> 6), 1ull << (eb >> 6));
           }
+        }
+      }
+      if constexpr (kMfma) {
+        for (uint64_t mm = ballot(mfma); mm; mm &= mm - 1) {
+          const int k = __ffsll(static_cast<unsigned long long>(mm)) - 1;
+          const uint64_t ka = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ia), k));
+          const uint64_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ib), k));
+          acc |= tile_product_mfma(data_a[ka * kTile + lane], data_bt[kb * kTile + lane], lane);
+          ++n_mfma;
         }
       }
       for (uint64_t dm = ballot(dense); dm;) {
@@ -303,6 +298,8 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
     uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
     if (acc) *dst |= acc;
   }
+  if (kMfma && dense_count && lane == 0 && n_mfma)
+    atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(n_mfma));
 }
 
 // len (<= 64) bits of a bit-packed array starting at bit `start`.
@@ -517,16 +514,46 @@ constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
 void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const uint32_t* off_bt,
                        const uint32_t* row_c, const uint32_t* col_c, const uint32_t* col_a,
                        const uint32_t* col_bt, const uint64_t* data_a, const uint64_t* data_bt,
-                       uint64_t* data_c, const uint32_t* bits_a = nullptr, const uint32_t* bits_bt = nullptr,
-                       uint32_t min_bits = 0, unsigned long long* dense_count = nullptr) {
+                       uint64_t* data_c) {
   if (n_c == 0) return;
+  fill_tiles_kernel<<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(n_c, off_a, off_bt, row_c, col_c,
+                                                                            col_a, col_bt, data_a, data_bt, data_c);
+  MG_LAUNCH_CHECK();
+}
+
+// Numeric pass of the handle-level product: payloads of both operands (stream-ordered in the
+// context's scratch), then fill_tiles_payload_kernel; with bits_a/bits_bt, pairs of tiles with
+// at least min_bits bits each go to the matrix cores and are counted into dense_count.
+void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                            mg_tiled_matrix* c, const uint32_t* bits_a, const uint32_t* bits_bt, uint32_t min_bits,
+                            unsigned long long* dense_count) {
+  if (c->n_nz_tile == 0) return;
+  hipStream_t st = ctx->stream;
+  WidePayload* pl =
+      reinterpret_cast<WidePayload*>(ctx_scratch(ctx, 2 * (static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1)));
+  WidePayload* pl_a = pl;
+  WidePayload* pl_bt = pl + a->n_nz_tile;
+  if (a->n_nz_tile) {
+    payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
+                                                                                     pl_a);
+    MG_LAUNCH_CHECK();
+  }
+  if (bt->n_nz_tile) {
+    payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(), bt->n_nz_tile,
+                                                                                      pl_bt);
+    MG_LAUNCH_CHECK();
+  }
+  const uint32_t grid = grid_for(c->n_nz_tile, kWavesPerBlock, 65536);
   if (bits_a && bits_bt)
-    fill_tiles_kernel<true><<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
-        n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, bits_a, bits_bt, min_bits,
-        dense_count);
+    fill_tiles_payload_kernel<true><<<grid, kBlock, 0, st>>>(
+        c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
+        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), bits_a,
+        bits_bt, min_bits, dense_count);
   else
-    fill_tiles_kernel<false><<<grid_for(n_c, kWavesPerBlock, 65536), kBlock, 0, s>>>(
-        n_c, off_a, off_bt, row_c, col_c, col_a, col_bt, data_a, data_bt, data_c, nullptr, nullptr, 0, nullptr);
+    fill_tiles_payload_kernel<false><<<grid, kBlock, 0, st>>>(
+        c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
+        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), nullptr,
+        nullptr, 0, nullptr);
   MG_LAUNCH_CHECK();
 }
 
@@ -679,27 +706,7 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric: operands differ in n_strips");
     DeviceGuard g(ctx->device);
-    if (c->n_nz_tile == 0) return;
-    // Payloads of both operands (stream-ordered in the context's scratch), then the product.
-    hipStream_t st = ctx->stream;
-    WidePayload* pl = reinterpret_cast<WidePayload*>(
-        ctx_scratch(ctx, 2 * (static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1)));
-    WidePayload* pl_a = pl;
-    WidePayload* pl_bt = pl + a->n_nz_tile;
-    if (a->n_nz_tile) {
-      payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
-                                                                                       pl_a);
-      MG_LAUNCH_CHECK();
-    }
-    if (bt->n_nz_tile) {
-      payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(),
-                                                                                        bt->n_nz_tile, pl_bt);
-      MG_LAUNCH_CHECK();
-    }
-    fill_tiles_payload_kernel<<<grid_for(c->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(
-        c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
-        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get());
-    MG_LAUNCH_CHECK();
+    launch_payload_product(ctx, a, bt, c, nullptr, nullptr, 0, nullptr);
   });
 }
 
@@ -724,9 +731,7 @@ extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix*
     }
     unsigned long long* cnt = ctx->counters.get();  // 64 slots
     MG_HIP(hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned long long), st));
-    launch_fill_tiles(st, c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(),
-                      c->col_idx.get(), a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(),
-                      c->data.get(), ba.get(), bb.get(), min_bits, cnt);
+    launch_payload_product(ctx, a, bt, c, ba.get(), bb.get(), min_bits, cnt);
     unsigned long long h[64];
     MG_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
