@@ -269,7 +269,70 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
           ++n_mfma;
         }
       }
-      for (uint64_t dm = ballot(dense); dm;) {
+      // Pairs with one dense tile: the dense tile's rows (lane = row) against the other tile's
+      // entries, kPairBatch pairs' loads at a time. An A entry (r, j) ORs column j of the Bt
+      // tile (a ballot over its rows) into row r; a Bt entry (c, j) sets bit c of every row r
+      // whose A row has bit j.
+      const bool a_dense = na == kWideDense, b_dense = nb == kWideDense;
+      for (uint64_t dm = ballot(dense && !a_dense); dm;) {
+        uint64_t rt_row[kPairBatch];
+        WidePayload ent[kPairBatch];
+        uint32_t n_pairs = 0;  // wave-uniform
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          rt_row[q] = 0;
+          ent[q] = WidePayload{0, 0};
+          if (dm) {
+            const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+            dm &= dm - 1;
+            const uint64_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ib), k));
+            rt_row[q] = data_bt[kb * kTile + lane];
+            ent[q].x = read_lane(pa.x, k);
+            ent[q].y = read_lane(pa.y, k);
+            ++n_pairs;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          if (static_cast<uint32_t>(q) >= n_pairs) continue;
+          const uint32_t n = wide_count(ent[q]);
+          for (uint32_t u = 0; u < n; ++u) {
+            const uint32_t e = wide_entry(ent[q], u);
+            const uint64_t col = ballot((rt_row[q] >> (e & 63u)) & 1);
+            if (lane == (e >> 6)) acc |= col;
+          }
+        }
+      }
+      for (uint64_t dm = ballot(dense && a_dense && !b_dense); dm;) {
+        uint64_t l_row[kPairBatch];
+        WidePayload ent[kPairBatch];
+        uint32_t n_pairs = 0;  // wave-uniform
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          l_row[q] = 0;
+          ent[q] = WidePayload{0, 0};
+          if (dm) {
+            const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+            dm &= dm - 1;
+            const uint64_t ka = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ia), k));
+            l_row[q] = data_a[ka * kTile + lane];
+            ent[q].x = read_lane(pb.x, k);
+            ent[q].y = read_lane(pb.y, k);
+            ++n_pairs;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          if (static_cast<uint32_t>(q) >= n_pairs) continue;
+          const uint32_t n = wide_count(ent[q]);
+          for (uint32_t v = 0; v < n; ++v) {
+            const uint32_t e = wide_entry(ent[q], v);
+            acc |= ((l_row[q] >> (e & 63u)) & 1ull) << (e >> 6);
+          }
+        }
+      }
+      // Both tiles dense: the full-tile wave product.
+      for (uint64_t dm = ballot(dense && a_dense && b_dense); dm;) {
         uint64_t l_row[kPairBatch], rt_row[kPairBatch];
         uint32_t n_pairs = 0;  // wave-uniform
 #pragma unroll
