@@ -150,7 +150,7 @@ def order_by_cost(cost, buckets_per_octave: int = 2):
 
 
 def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_ratio=2.0):
-    """Orders every level by predicted cost, runs once with logging to judge the predictions,
+    """Orders every level by predicted cost, runs once with logging to check the predictions,
     re-orders mispredicted levels by their exact costs, and returns the metrics."""
     levels = batch.info()[1]
     cache = {}
