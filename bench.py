@@ -65,48 +65,6 @@ def two_hop(M, ctx, scale):
                          "pairs_equal": cpu_pairs == n_pairs}}}), flush=True)
 
 
-def gather_floor(M, ctx, el, at, n_vertices, args):
-    """Gathers the source row of every A^T bit (strip order: target strip, then source) with no
-    other work, and times the walk's sweep hops beside it in the same process."""
-    src, dst = el.download()
-    el.close()
-    key = ((dst.astype(np.int64) >> 6) << 32) | src.astype(np.int64)
-    rows = np.ascontiguousarray(src[np.argsort(key, kind="stable")].astype(np.uint32))
-    del key, src, dst
-    rows_d = M.DeviceBuffer.from_array(ctx, rows)
-    x = M.DeviceBuffer(ctx, (n_vertices + 1) * 512)
-    floor = [M.probe_gather_rows(ctx, x, rows_d, len(rows), grid=1024, reps=3) for _ in range(3)]
-    x.close()
-    rows_d.close()
-    sources = M.sample_sources(n_vertices, args.sources, args.seed)
-    w = M.Walk(ctx, at, args.sources, args.hops)
-    w.run(sources, args.hops)
-    ctx.synchronize()
-    w.set_timing(True)
-    for _ in range(3):
-        w.run(sources, args.hops)
-    ctx.synchronize()
-    w.set_timing(False)
-    hops = []
-    for k in range(1, args.hops + 1):
-        _, num_ms, n_runs = w.kernel_times(k)
-        st = w.hop_stats(k)
-        hops.append({"hop": k, "kernel": "sweep" if st["kernel"] == M.HOP_SWEEP else "push",
-                     "ms": round(num_ms / max(n_runs, 1), 4), "gathers": st["edges_visited"]})
-    fl = min(floor)
-    steady = [h for h in hops if h["kernel"] == "sweep" and h["gathers"] > 0.9 * len(rows)]
-    out = {
-        "gather_floor": {
-            "rows": len(rows), "order": "A^T bits in strip order (target strip, then source)",
-            "ms": [round(v, 4) for v in floor], "GBps_requested": round(len(rows) * 512 / (fl * 1e-3) / 1e9, 1),
-            "grid": 1024,
-        },
-        "walk_hops": hops,
-        "steady_hop_vs_floor": [round(fl / h["ms"], 3) for h in steady],
-    }
-    print(json.dumps(out), flush=True)
-
-
 def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads, reps=1):
     """Time the oracle's tile algorithm (OpenMP) on every `stride`-th strip of each hop
     (stride 1: the whole walk), using the GPU's X_{k-1} as input, `reps` times, and scale each
@@ -394,12 +352,6 @@ def main():
     ap.add_argument("--two-hop", type=int, default=0, metavar="SCALE",
                     help="only time the two-hop pair enumeration (mg_edges_two_hop) on an R-MAT graph of this "
                          "scale (edge factor 16, seed 2), beside the oracle's bitset product on the host")
-    ap.add_argument("--gather-floor", action="store_true",
-                    help="only measure the hop kernel's gather floor: the 512-byte frontier row of every A^T bit "
-                         "gathered in strip order with no other work, beside the walk's steady hops")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run device-to-device copies of MIB MiB (5 times): the known-size reference for "
-                         "FETCH_SIZE / WRITE_SIZE counter scaling")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -409,14 +361,6 @@ def main():
     import matrixgraph_amd as M
 
     ctx = M.Context(0)
-    if args.copy_calibration:
-        nbytes = args.copy_calibration << 20
-        src, dst = M.DeviceBuffer(ctx, nbytes), M.DeviceBuffer(ctx, nbytes)
-        for _ in range(5):
-            M._check(M.lib().mg_buffer_copy(ctx.h, dst.ptr, src.ptr, nbytes), "mg_buffer_copy")
-        ctx.synchronize()
-        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
-        return
     if args.two_hop:
         two_hop(M, ctx, args.two_hop)
         return
@@ -424,9 +368,6 @@ def main():
     el = M.EdgeList.rmat(ctx, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
     n_vertices, n_edges = el.info()
     at = el.to_tiled_matrix(transpose=True)
-    if args.gather_floor:
-        gather_floor(M, ctx, el, at, n_vertices, args)
-        return
     el.close()
     ns, n_tiles, _ = at.metadata()
     log(f"graph: 2^{args.scale} vertices, {n_edges} edges, {n_tiles} A^T tiles, setup {time.perf_counter() - t0:.1f}s")

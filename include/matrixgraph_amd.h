@@ -385,14 +385,6 @@ int matrixgraph_bit_matmult(const uint64_t* A, const uint64_t* Bt, uint64_t* C, 
 int matrixgraph_bit_walk(const uint32_t* src, const uint32_t* dst, int n_edges, int n_vertices,
                          const uint32_t* sources, int n_sources, int hops, uint64_t* x_out, uint64_t* popcount);
 
-/* Measurement helper (no counterpart in the reference): the gather floor of the hop kernel.
- * Gathers the 512-byte rows x[rows[i]] (64 words each; x holds n_x_rows rows) exactly as the
- * hop kernel gathers frontier rows, with no other work, over `grid` workgroups of 256
- * threads that each take a contiguous slice of rows; reports device milliseconds per pass
- * (mean of reps passes after one warm-up). Row indices are checked on the host first. */
-int mg_probe_gather_rows(mg_context* ctx, const uint64_t* x, uint64_t n_x_rows, const uint32_t* rows,
-                         uint64_t n_rows, uint32_t grid, uint32_t reps, double* ms_per_pass);
-
 /* ---------------------------------------------------------------------------
  * Batched walks: many queries of uneven source-set size and hop count over several
  * graphs in one launch sequence, captured once as a HIP graph and replayed by

@@ -377,16 +377,6 @@ def sample_sources(n_vertices: int, count: int, seed: int) -> np.ndarray:
     return out
 
 
-def probe_gather_rows(ctx: Context, x: "DeviceBuffer", rows: "DeviceBuffer", n_rows: int, grid: int = 1024,
-                      reps: int = 3) -> float:
-    """Device ms per pass of gathering the 512-byte rows x[rows[i]] with no other work (the
-    hop kernel's gather floor; see mg_probe_gather_rows)."""
-    ms = C.c_double()
-    _check(lib().mg_probe_gather_rows(ctx.h, C.c_void_p(x.ptr), x.nbytes // 512, C.c_void_p(rows.ptr), n_rows, grid,
-                                      reps, C.byref(ms)), "mg_probe_gather_rows")
-    return ms.value
-
-
 WALK_PRUNE = 1
 WALK_SWEEP_FIRST_HOP = 2
 HOP_SWEEP = 0

@@ -124,7 +124,6 @@ SIGNATURES = {
     "mg_walk_frontier_digest": (i32, [vp, vp, u32, P_u64]),
     "mg_walk_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
     "mg_walk_destroy": (i32, [vp]),
-    "mg_probe_gather_rows": (i32, [vp, vp, u64, vp, u64, u32, u32, C.POINTER(f64)]),
     "matrixgraph_bit_matmult": (i32, [vp, vp, vp, i32, i32, i32]),
     "matrixgraph_bit_walk": (i32, [vp, vp, i32, i32, vp, i32, i32, vp, vp]),
     # batched walks

@@ -214,7 +214,8 @@ std::unique_ptr<mg_edge_list> finish_edges(mg_context* ctx, DevBuf<uint64_t>&& k
   uint64_t u = sort_unique_u64(ctx, keys, n, 64);
   if (drop_sentinel && u > 0) {
     uint64_t last = 0;
-    MG_HIP(hipMemcpy(&last, keys.get() + (u - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    MG_HIP(hipMemcpyAsync(&last, keys.get() + (u - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));
     if (last == kSentinel) --u;
   }
   el->n_edges = u;
@@ -549,6 +550,11 @@ std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_mat
   expand_rows_kernel<false><<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
                                                   drop_diagonal ? 1 : 0, count.get(), nullptr, nullptr);
   MG_LAUNCH_CHECK();
+  // The scan below is 32-bit: bound the 64-bit total first (the other edge-list paths stop at
+  // 2^31 edges too).
+  const uint64_t total64 = device_sum_u32(ctx, count.get(), n_slots);
+  MG_REQUIRE(total64 < (1ull << 31), "tile -> edge expansion: " + std::to_string(total64) +
+                                         " edges, more than 2^31 - 1");
   exclusive_scan_u32(ctx, count.get(), start.get(), n_slots);
   uint32_t total = 0;
   MG_HIP(hipMemcpyAsync(&total, start.get() + n_slots, sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -182,6 +182,8 @@ inline uint64_t* ctx_scratch(mg_context* ctx, size_t n) {
 // Helpers implemented in mg_matrix.hip / mg_graph.hip, shared across units.
 void alloc_tiled_matrix(mg_tiled_matrix* m, uint32_t n_strips, uint32_t n_nz_tile);
 uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words);
+// Sum of n u32 values, in 64 bits (for totals that a u32 scan would wrap).
+uint64_t device_sum_u32(mg_context* ctx, const uint32_t* data, uint64_t n);
 // Exclusive scan of n u32 counts into out[0..n] (out[n] = total), on ctx->stream.
 void exclusive_scan_u32(mg_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n);
 void exclusive_scan_u32_on(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n);

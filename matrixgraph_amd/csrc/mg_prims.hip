@@ -19,6 +19,15 @@ __global__ void __launch_bounds__(256) popcount_kernel(const uint64_t* __restric
   if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
 }
 
+__global__ void __launch_bounds__(256) sum_u32_kernel(const uint32_t* __restrict__ data, uint64_t n,
+                                                      unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) local += data[i];
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+}
+
 __global__ void mark_dups_kernel(const uint64_t* __restrict__ keys, uint64_t n,
                                  uint32_t* __restrict__ keep) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) keep[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
@@ -31,6 +40,19 @@ uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words
   MG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
   if (n_words) {
     popcount_kernel<<<grid_for(n_words, 256 * 16, 8192), 256, 0, ctx->stream>>>(data, n_words, d);
+    MG_LAUNCH_CHECK();
+  }
+  unsigned long long h = 0;
+  MG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  MG_HIP(hipStreamSynchronize(ctx->stream));
+  return h;
+}
+
+uint64_t device_sum_u32(mg_context* ctx, const uint32_t* data, uint64_t n) {
+  unsigned long long* d = ctx->counters.get();
+  MG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
+  if (n) {
+    sum_u32_kernel<<<grid_for(n, 256 * 16, 8192), 256, 0, ctx->stream>>>(data, n, d);
     MG_LAUNCH_CHECK();
   }
   unsigned long long h = 0;

@@ -723,6 +723,11 @@ extern "C" int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, c
                                                          col_b.get(), cnt.get(), tp.get(), nullptr, nullptr,
                                                          nullptr);
     MG_LAUNCH_CHECK();
+    // C's tile index is 32-bit (the reference's n_nz_tile is u32): bound the 64-bit total
+    // before the 32-bit scan.
+    const uint64_t n_c64 = device_sum_u32(ctx, cnt.get(), n);
+    MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, "mg_multiply_symbolic: " + std::to_string(n_c64) +
+                                           " output tiles, more than a u32 tile index holds");
     exclusive_scan_u32(ctx, cnt.get(), off_c.get(), n);
     sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(tp.get(), n, sums.get());
     MG_LAUNCH_CHECK();

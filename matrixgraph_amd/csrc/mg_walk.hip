@@ -1280,17 +1280,18 @@ struct GraphPlan {
 // reads, strips the write-out; heaviest first leaves light chunks for the tail of a launch.
 // Within one allocation this order beat ordering by tiles alone by 2-4 % on hops 2-4 of the
 // 4096-source walk on R-MAT 22 (tiles alone beat strip order by 4 % and reversed by 35 %).
-std::vector<uint64_t> chunk_costs(const GraphPlan& g) {
+std::vector<uint64_t> chunk_costs(const GraphPlan& g, hipStream_t st) {
   const size_t n = g.chunks.size();
   std::vector<uint64_t> cost(n, 0);
   if (!n || !g.at->n_nz_tile) return cost;
   DevBuf<Chunk> dch(n);
   DevBuf<uint32_t> bits(n);
-  MG_HIP(hipMemcpy(dch.get(), g.chunks.data(), sizeof(Chunk) * n, hipMemcpyHostToDevice));
-  chunk_bits_kernel<<<static_cast<uint32_t>(n), kBlock>>>(dch.get(), nullptr, g.payload.get(), bits.get());
+  MG_HIP(hipMemcpyAsync(dch.get(), g.chunks.data(), sizeof(Chunk) * n, hipMemcpyHostToDevice, st));
+  chunk_bits_kernel<<<static_cast<uint32_t>(n), kBlock, 0, st>>>(dch.get(), nullptr, g.payload.get(), bits.get());
   MG_LAUNCH_CHECK();
   std::vector<uint32_t> hb(n);
-  MG_HIP(hipMemcpy(hb.data(), bits.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  MG_HIP(hipMemcpyAsync(hb.data(), bits.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
   for (size_t i = 0; i < n; ++i) {
     const Chunk& c = g.chunks[i];
     cost[i] = 100ull * hb[i] + 35ull * (c.tile_end - c.tile_begin) + 2400ull * (c.strip_end - c.strip_begin);
@@ -1298,37 +1299,43 @@ std::vector<uint64_t> chunk_costs(const GraphPlan& g) {
   return cost;
 }
 
-GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles) {
+// Every launch and copy is ordered on `st`, the caller's context stream, so a matrix still
+// being filled by earlier asynchronous work on that stream is complete before it is read.
+GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_t st) {
   GraphPlan g;
   g.at = at;
   g.n_strips = at->n_strips;
   std::vector<uint32_t> off(at->n_strips + 1ull);
-  MG_HIP(hipMemcpy(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost));
+  MG_HIP(hipMemcpyAsync(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
   g.chunks = build_chunks(off, at->n_strips, chunk_tiles, &g.split);
   g.rmask.alloc(std::max<uint32_t>(1, at->n_nz_tile));
   g.payload.alloc(std::max<uint32_t>(1, at->n_nz_tile));
   if (at->n_nz_tile) {
-    row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile, g.rmask.get());
+    row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), at->n_nz_tile,
+                                                                               g.rmask.get());
     MG_LAUNCH_CHECK();
-    tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock>>>(at->data.get(), at->n_nz_tile,
-                                                                            g.payload.get());
+    tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), at->n_nz_tile,
+                                                                                   g.payload.get());
     MG_LAUNCH_CHECK();
     const uint32_t n = at->n_nz_tile;
     DevBuf<uint32_t> flag(n), pos(n + 1ull);
-    dense_flag_kernel<<<grid_for(n, kBlock), kBlock>>>(g.payload.get(), n, flag.get());
+    dense_flag_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.payload.get(), n, flag.get());
     MG_LAUNCH_CHECK();
-    exclusive_scan_u32_on(nullptr, flag.get(), pos.get(), n);
+    exclusive_scan_u32_on(st, flag.get(), pos.get(), n);
     uint32_t n_dense = 0;
-    MG_HIP(hipMemcpy(&n_dense, pos.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    MG_HIP(hipMemcpyAsync(&n_dense, pos.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
     g.dense.alloc(std::max<uint64_t>(1, static_cast<uint64_t>(n_dense) * kTile));
     if (n_dense) {
-      dense_compact_kernel<<<grid_for(n, kWaves, 65536), kBlock>>>(at->data.get(), n, pos.get(), g.payload.get(),
-                                                                    g.dense.get());
+      dense_compact_kernel<<<grid_for(n, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), n, pos.get(),
+                                                                          g.payload.get(), g.dense.get());
       MG_LAUNCH_CHECK();
     }
-    MG_HIP(hipDeviceSynchronize());
+    // flag and pos are freed on return: their last readers must have finished.
+    MG_HIP(hipStreamSynchronize(st));
   }
-  g.chunk_cost = chunk_costs(g);
+  g.chunk_cost = chunk_costs(g, st);
   return g;
 }
 
@@ -1359,32 +1366,36 @@ __global__ void __launch_bounds__(kBlock) column_copy_kernel(const uint32_t* __r
 }
 
 // Counting sort of the A^T tiles by column (16 bytes per tile on the device).
-void build_column_index(GraphPlan& g) {
+void build_column_index(GraphPlan& g, hipStream_t st) {
   const mg_tiled_matrix* at = g.at;
   const uint32_t ns = at->n_strips, n = at->n_nz_tile;
   DevBuf<uint32_t> cursor;
   cursor.alloc(std::max<uint32_t>(1, ns));
-  MG_HIP(hipMemset(cursor.get(), 0, sizeof(uint32_t) * std::max<uint32_t>(1, ns)));
+  MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * std::max<uint32_t>(1, ns), st));
   if (n) {
-    column_count_kernel<<<grid_for(n, kBlock), kBlock>>>(at->col_idx.get(), n, cursor.get());
+    column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(at->col_idx.get(), n, cursor.get());
     MG_LAUNCH_CHECK();
   }
   std::vector<uint32_t> count(ns);
-  if (ns) MG_HIP(hipMemcpy(count.data(), cursor.get(), sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+  if (ns) MG_HIP(hipMemcpyAsync(count.data(), cursor.get(), sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, st));
+  MG_HIP(hipStreamSynchronize(st));
   g.host_csc_off.assign(ns + 1ull, 0);
   for (uint32_t t = 0; t < ns; ++t) g.host_csc_off[t + 1] = g.host_csc_off[t] + count[t];
-  if (ns) MG_HIP(hipMemcpy(cursor.get(), g.host_csc_off.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+  if (ns)
+    MG_HIP(hipMemcpyAsync(cursor.get(), g.host_csc_off.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, st));
   g.csc_tile.alloc(std::max<uint32_t>(1, n));
   g.csc_row.alloc(std::max<uint32_t>(1, n));
   g.csc_payload.alloc(std::max<uint32_t>(1, n));
   if (n) {
-    column_scatter_kernel<<<grid_for(n, kBlock), kBlock>>>(at->col_idx.get(), n, cursor.get(), g.csc_tile.get());
+    column_scatter_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(at->col_idx.get(), n, cursor.get(),
+                                                                  g.csc_tile.get());
     MG_LAUNCH_CHECK();
-    column_copy_kernel<<<grid_for(n, kBlock), kBlock>>>(g.csc_tile.get(), n, at->row_idx.get(), g.payload.get(),
-                                                        g.csc_row.get(), g.csc_payload.get());
+    column_copy_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.csc_tile.get(), n, at->row_idx.get(),
+                                                               g.payload.get(), g.csc_row.get(), g.csc_payload.get());
     MG_LAUNCH_CHECK();
   }
-  MG_HIP(hipDeviceSynchronize());
+  // cursor is freed on return and host_csc_off's upload reads host memory.
+  MG_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -1442,7 +1453,9 @@ struct Engine {
 namespace {
 
 // Allocate frontiers, build the per-level descriptor tables, chunk lists and items.
-void build_engine(Engine* e) {
+// All copies and fills are ordered on `st` (the context stream); host sources stay alive
+// until the stream has consumed them.
+void build_engine(Engine* e, hipStream_t st) {
   const uint32_t P = static_cast<uint32_t>(e->panels.size());
   e->max_hops = 0;
   for (auto& p : e->panels) e->max_hops = std::max(e->max_hops, p.hops);
@@ -1461,11 +1474,11 @@ void build_engine(Engine* e) {
       p.z[k].alloc(p.n_strips);
       // All strips start zero and known to be zero (Z and N masks 0); later runs rely on
       // that to skip stores of words that stay zero.
-      MG_HIP(hipMemset(p.x[k].get(), 0, p.x[k].bytes()));
-      MG_HIP(hipMemset(p.n[k].get(), 0, p.n[k].bytes()));
-      MG_HIP(hipMemset(p.s[k].get(), 0, p.s[k].bytes()));
-      MG_HIP(hipMemset(p.v[k].get(), 0, p.v[k].bytes()));
-      MG_HIP(hipMemset(p.z[k].get(), 0, p.z[k].bytes()));
+      MG_HIP(hipMemsetAsync(p.x[k].get(), 0, p.x[k].bytes(), st));
+      MG_HIP(hipMemsetAsync(p.n[k].get(), 0, p.n[k].bytes(), st));
+      MG_HIP(hipMemsetAsync(p.s[k].get(), 0, p.s[k].bytes(), st));
+      MG_HIP(hipMemsetAsync(p.v[k].get(), 0, p.v[k].bytes(), st));
+      MG_HIP(hipMemsetAsync(p.z[k].get(), 0, p.z[k].bytes(), st));
     }
   }
   e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kCounterStride);
@@ -1542,20 +1555,21 @@ void build_engine(Engine* e) {
       ch.swap(sorted);
     }
     e->lv[k].alloc(P);
-    MG_HIP(hipMemcpy(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice, st));
     e->n_chunks[k] = static_cast<uint32_t>(ch.size());
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
-    if (!ch.empty()) MG_HIP(hipMemcpy(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    if (!ch.empty()) MG_HIP(hipMemcpyAsync(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, st));
     if (k == 0) {
       e->n_push_chunks = static_cast<uint32_t>(pch.size());
       e->push_chunks.alloc(std::max<size_t>(1, pch.size()));
       if (!pch.empty())
-        MG_HIP(hipMemcpy(e->push_chunks.get(), pch.data(), sizeof(Chunk) * pch.size(), hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpyAsync(e->push_chunks.get(), pch.data(), sizeof(Chunk) * pch.size(), hipMemcpyHostToDevice, st));
     }
     e->n_split[k] = static_cast<uint32_t>(sp.size());
     e->split[k].alloc(std::max<size_t>(1, sp.size()));
-    if (!sp.empty()) MG_HIP(hipMemcpy(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice));
+    if (!sp.empty()) MG_HIP(hipMemcpyAsync(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice, st));
+    MG_HIP(hipStreamSynchronize(st));  // lv, ch, pch and sp are about to go out of scope
   }
   // X_0 fills: frontier and masks of every panel, plus all counters.
   std::vector<PanelInit> inits(P);
@@ -1568,12 +1582,13 @@ void build_engine(Engine* e) {
   }
   fills.push_back({reinterpret_cast<uint64_t*>(e->counters.get()), e->counters.size()});
   e->inits.alloc(P);
-  MG_HIP(hipMemcpy(e->inits.get(), inits.data(), sizeof(PanelInit) * P, hipMemcpyHostToDevice));
+  MG_HIP(hipMemcpyAsync(e->inits.get(), inits.data(), sizeof(PanelInit) * P, hipMemcpyHostToDevice, st));
   e->panel_strips.alloc(P);
-  MG_HIP(hipMemcpy(e->panel_strips.get(), strips.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice));
+  MG_HIP(hipMemcpyAsync(e->panel_strips.get(), strips.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, st));
   e->n_fills = static_cast<uint32_t>(fills.size());
   e->fills.alloc(fills.size());
-  MG_HIP(hipMemcpy(e->fills.get(), fills.data(), sizeof(FillRange) * fills.size(), hipMemcpyHostToDevice));
+  MG_HIP(hipMemcpyAsync(e->fills.get(), fills.data(), sizeof(FillRange) * fills.size(), hipMemcpyHostToDevice, st));
+  MG_HIP(hipStreamSynchronize(st));
 }
 
 template <int W>
@@ -1803,7 +1818,7 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     Engine& e = w->e;
     e.W = words_for(max_sources);
     e.prune = (flags & MG_WALK_PRUNE) ? 1 : 0;
-    e.graphs.push_back(plan_graph(at, kDefaultChunkTiles));
+    e.graphs.push_back(plan_graph(at, kDefaultChunkTiles, ctx->stream));
     Panel p;
     p.graph = 0;
     p.hops = max_hops;
@@ -1812,8 +1827,8 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     p.alive.assign(max_hops, words_mask(0, e.W));
     e.panels.push_back(std::move(p));
     // The column index first: the level tables built next point at it.
-    if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) build_column_index(e.graphs[0]);
-    build_engine(&e);
+    if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) build_column_index(e.graphs[0], ctx->stream);
+    build_engine(&e, ctx->stream);
     e.sources.alloc(max_sources);
     w->staging.alloc(max_sources);
     if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) {
@@ -1823,7 +1838,8 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
       w->push_staging.alloc(w->max_push_items);
       const uint32_t zero = 0;
       e.push_panels.alloc(1);
-      MG_HIP(hipMemcpy(e.push_panels.get(), &zero, sizeof(zero), hipMemcpyHostToDevice));
+      MG_HIP(hipMemcpyAsync(e.push_panels.get(), &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+      MG_HIP(hipStreamSynchronize(ctx->stream));
     }
     MG_HIP(hipEventCreateWithFlags(&w->staging_read, hipEventDisableTiming));
     *out = w.release();
@@ -2109,7 +2125,7 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     e.W = 64;
     for (uint32_t gi = 0; gi < n_graphs; ++gi) {
       MG_REQUIRE(graphs_at[gi] != nullptr, "mg_walk_batch_create: NULL graph");
-      e.graphs.push_back(plan_graph(graphs_at[gi], chunk_tiles));
+      e.graphs.push_back(plan_graph(graphs_at[gi], chunk_tiles, ctx->stream));
     }
     // Validate queries.
     b->queries.resize(n_queries);
@@ -2198,7 +2214,7 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
         const uint32_t gi = e.panels[pi].graph;
         GraphPlan& g = e.graphs[gi];
         if (!indexed[gi]) {
-          build_column_index(g);
+          build_column_index(g, ctx->stream);
           indexed[gi] = 1;
         }
         if (2 * push_items_for(g, pi, st, nullptr) <= g.at->n_nz_tile) {
@@ -2209,17 +2225,17 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
         }
       }
     }
-    build_engine(&e);
+    build_engine(&e, ctx->stream);
     e.n_push_items = static_cast<uint32_t>(push_items.size());
     e.push_items.alloc(std::max<size_t>(1, push_items.size()));
     if (!push_items.empty())
-      MG_HIP(hipMemcpy(e.push_items.get(), push_items.data(), sizeof(PushItem) * push_items.size(),
-                       hipMemcpyHostToDevice));
+      MG_HIP(hipMemcpyAsync(e.push_items.get(), push_items.data(), sizeof(PushItem) * push_items.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
     e.n_push_panels = static_cast<uint32_t>(push_panels.size());
     e.push_panels.alloc(std::max<size_t>(1, push_panels.size()));
     if (!push_panels.empty())
-      MG_HIP(hipMemcpy(e.push_panels.get(), push_panels.data(), sizeof(uint32_t) * push_panels.size(),
-                       hipMemcpyHostToDevice));
+      MG_HIP(hipMemcpyAsync(e.push_panels.get(), push_panels.data(), sizeof(uint32_t) * push_panels.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
     // Sources: all queries' bits, uploaded once.
     std::vector<SourceItem> items;
     for (uint32_t q = 0; q < n_queries; ++q) {
@@ -2228,7 +2244,9 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     }
     e.n_sources = static_cast<uint32_t>(items.size());
     e.sources.alloc(items.size());
-    MG_HIP(hipMemcpy(e.sources.get(), items.data(), sizeof(SourceItem) * items.size(), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpyAsync(e.sources.get(), items.data(), sizeof(SourceItem) * items.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    MG_HIP(hipStreamSynchronize(ctx->stream));  // push_items, push_panels and items are host locals
     *out = b.release();
   });
 }
@@ -2393,6 +2411,7 @@ extern "C" int mg_walk_batch_set_logging(mg_walk_batch* b, int enable) {
         e.logs[k].alloc(std::max<uint32_t>(1, e.n_chunks[k]));
         MG_HIP(hipMemset(e.logs[k].get(), 0, e.logs[k].bytes()));
       }
+      MG_HIP(hipDeviceSynchronize());  // the fills run on the null stream; runs use the context's
     }
     if (on != e.logging) drop_graph(b);  // kernel arguments change: capture again
     e.logging = on;
@@ -2460,5 +2479,6 @@ extern "C" int mg_walk_batch_set_level_order(mg_walk_batch* b, uint32_t level, c
     if (!ch.empty())
       MG_HIP(hipMemcpy(e.chunks[level].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
     if (!e.logs.empty()) MG_HIP(hipMemset(e.logs[level].get(), 0, e.logs[level].bytes()));
+    MG_HIP(hipDeviceSynchronize());  // null-stream fill; runs use the context's stream
   });
 }

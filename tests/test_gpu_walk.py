@@ -260,13 +260,3 @@ def test_walk_rmat22_full_scale_digests(ctx):
         assert w.frontier_digest(k) == O.frontier_digest(x, n_src), f"hop {k}"
         assert w.hop_stats(k)["popcount"] == popcount(x)
 
-
-def test_probe_gather_rows_bounds(ctx):
-    """The gather-floor probe runs over valid rows and rejects a row index past x."""
-    M = mg()
-    x = M.DeviceBuffer(ctx, 10 * 512)
-    rows = M.DeviceBuffer.from_array(ctx, np.arange(1000, dtype=np.uint32) % 10)
-    assert M.probe_gather_rows(ctx, x, rows, 1000, grid=4, reps=2) >= 0.0
-    bad = M.DeviceBuffer.from_array(ctx, np.array([3, 10], np.uint32))
-    with pytest.raises(M.MGError):
-        M.probe_gather_rows(ctx, x, bad, 2, grid=1, reps=1)
