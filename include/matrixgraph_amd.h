@@ -318,7 +318,8 @@ enum {
 };
 enum {
   MG_HOP_SWEEP = 0u, /* the hop swept every strip of A^T (walk_numeric kernels) */
-  MG_HOP_PUSH = 1u   /* first hop pushed from the source strips' A^T columns */
+  MG_HOP_PUSH = 1u,  /* first hop pushed from the source strips' A^T columns */
+  MG_HOP_JOBS = 2u   /* the hop read A^T's job list by row segments (mg_walk_set_hop_kernel) */
 };
 
 typedef struct mg_hop_stats {
@@ -332,8 +333,8 @@ typedef struct mg_hop_stats {
   uint64_t edges_visited;     /* set bits of A read by the numeric pass */
   double symbolic_ms;         /* 0: the structural pass runs inside the numeric kernel */
   double numeric_ms;          /* hop kernels (structural + numeric) of the last logged run, else 0 */
-  uint32_t kernel;            /* MG_HOP_SWEEP or MG_HOP_PUSH */
-  uint32_t reserved;
+  uint32_t kernel;            /* MG_HOP_SWEEP, MG_HOP_PUSH or MG_HOP_JOBS */
+  uint32_t segments;          /* MG_HOP_JOBS: row segments of the job-list kernel, else 0 */
 } mg_hop_stats;
 
 /* Allocates X_0..X_max_hops (W = ceil(max_sources/64) rounded up to a power of two,
@@ -344,6 +345,17 @@ typedef struct mg_hop_stats {
  * most half of its tiles; results and statistics are the same either way. */
 int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sources,
                    uint32_t max_hops, uint32_t flags, mg_walk** out);
+/* Hop kernel of the 4096-source (64-word) levels from hop 2 on. segments = 0: the
+ * tile-decoding sweep (every hop decodes the A^T tiles' payloads into gathers of whole
+ * 512-byte source rows); 1, 2, 4 or 8: the job-list kernel, which reads a list of at's set
+ * bits built once (4 bytes per set bit plus 4 per tile, on first use) and splits each
+ * source row into that many segments, one per workgroup. The default is 4 (environment
+ * MG_WALK_SEGMENTS overrides it at creation). Walks of fewer than 2049 sources, and graphs
+ * above 2^26 vertices or 2^32 - 1 set bits, keep the sweep. Results and statistics are the
+ * same for every choice. Synchronises the context's stream. */
+int mg_walk_set_hop_kernel(mg_context* ctx, mg_walk* w, uint32_t segments);
+/* The segments in effect (0: sweep). */
+int mg_walk_hop_kernel(const mg_walk* w, uint32_t* segments);
 /* Runs hops 1..hops from the given distinct sources (async; stats after sync). */
 int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, const uint32_t* sources, uint32_t hops);
 /* enable != 0 starts (and clears) a log of HIP events around the hop kernel of every hop
@@ -454,6 +466,9 @@ typedef struct mg_chunk_record {
   uint32_t out_tiles;               /* logged: numerically non-empty output tiles (unsplit strips) */
 } mg_chunk_record;
 int mg_walk_batch_set_logging(mg_walk_batch* b, int enable);
+/* Hop kernel of the batch's levels after the first (see mg_walk_set_hop_kernel); levels
+ * run while chunk logging is on use the sweep, which logs. Drops the captured graph. */
+int mg_walk_batch_set_hop_kernel(mg_context* ctx, mg_walk_batch* b, uint32_t segments);
 /* level in [0, max_hops): the X_level -> X_level+1 launch. out == NULL only sets *n.
  * Synchronises. */
 int mg_walk_batch_level_chunks(mg_context* ctx, const mg_walk_batch* b, uint32_t level, uint64_t* n,

@@ -381,6 +381,7 @@ WALK_PRUNE = 1
 WALK_SWEEP_FIRST_HOP = 2
 HOP_SWEEP = 0
 HOP_PUSH = 1
+HOP_JOBS = 2
 BATCH_SWEEP_FIRST_HOP = 1
 
 
@@ -398,6 +399,18 @@ class Walk:
 
     def set_timing(self, on: bool):
         _check(lib().mg_walk_set_timing(self.h, 1 if on else 0), "mg_walk_set_timing")
+
+    def set_hop_kernel(self, segments: int) -> int:
+        """Hop kernel of the 4096-source levels after the first: 0 = tile-decoding sweep,
+        1/2/4/8 = job-list kernel with that many row segments (mg_walk_set_hop_kernel).
+        Returns the choice in effect (0 where the job list does not apply)."""
+        _check(lib().mg_walk_set_hop_kernel(self.ctx.h, self.h, segments), "mg_walk_set_hop_kernel")
+        return self.hop_kernel()
+
+    def hop_kernel(self) -> int:
+        g = C.c_uint32()
+        _check(lib().mg_walk_hop_kernel(self.h, C.byref(g)), "mg_walk_hop_kernel")
+        return g.value
 
     def run(self, sources, hops: int) -> None:
         src = np.ascontiguousarray(sources, np.uint32)
@@ -648,6 +661,10 @@ class WalkBatch:
     # --- scheduler control (cost models) ---------------------------------------------
     def set_logging(self, on: bool):
         _check(lib().mg_walk_batch_set_logging(self.h, 1 if on else 0), "mg_walk_batch_set_logging")
+
+    def set_hop_kernel(self, segments: int):
+        """Hop kernel of the levels after the first (see Walk.set_hop_kernel)."""
+        _check(lib().mg_walk_batch_set_hop_kernel(self.ctx.h, self.h, segments), "mg_walk_batch_set_hop_kernel")
 
     def level_chunks(self, level: int) -> np.ndarray:
         """Chunks of hop level `level` in launch order (structured array of mg_chunk_record)."""

@@ -38,7 +38,7 @@ class HopStats(C.Structure):
         ("symbolic_ms", f64),
         ("numeric_ms", f64),
         ("kernel", u32),
-        ("reserved", u32),
+        ("segments", u32),
     ]
 
 
@@ -117,6 +117,8 @@ SIGNATURES = {
     "mg_walk_create": (i32, [vp, vp, u32, u32, u32, C.POINTER(vp)]),
     "mg_walk_run": (i32, [vp, vp, u32, vp, u32]),
     "mg_walk_set_timing": (i32, [vp, i32]),
+    "mg_walk_set_hop_kernel": (i32, [vp, vp, u32]),
+    "mg_walk_hop_kernel": (i32, [vp, C.POINTER(u32)]),
     "mg_walk_hop_stats": (i32, [vp, vp, u32, C.POINTER(HopStats)]),
     "mg_walk_kernel_times": (i32, [vp, vp, u32, C.POINTER(f64), C.POINTER(f64), P_u32]),
     "mg_walk_words_per_vertex": (i32, [vp, P_u32]),
@@ -137,6 +139,7 @@ SIGNATURES = {
     "mg_walk_batch_query_export": (i32, [vp, vp, u32, C.POINTER(vp)]),
     "mg_walk_batch_destroy": (i32, [vp]),
     "mg_walk_batch_set_logging": (i32, [vp, i32]),
+    "mg_walk_batch_set_hop_kernel": (i32, [vp, vp, u32]),
     "mg_walk_batch_level_chunks": (i32, [vp, vp, u32, P_u64, vp]),
     "mg_walk_batch_panel_info": (i32, [vp, u32, P_u32, P_u32, P_u32, vp]),
     "mg_walk_batch_set_level_order": (i32, [vp, u32, vp, u64]),

@@ -22,6 +22,7 @@
 // once per graph, predicates the 64 row loads of the tile.
 // A finalize kernel completes the masks of strips split across chunks.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -93,6 +94,9 @@ struct PanelLevel {
   uint64_t* z_out;  // per strip: rows of X_{k+1} that may be non-zero in memory
   uint64_t* s_out;
   unsigned long long* counters;
+  const uint32_t* jobs;     // per A^T set bit: (source vertex << 6) | target row, tile order
+  const uint32_t* job_off;  // per A^T tile: first position in jobs (n_tiles + 1 entries)
+  uint64_t* part;           // segmented hop kernel: per (segment, strip) partial N, V, Z masks
   uint64_t alive_in;    // words of X_k that propagate into X_{k+1}
   uint64_t alive_next;  // words of X_{k+1} that propagate further (they decide V_{k+1})
   uint32_t n_strips;
@@ -818,6 +822,292 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD)
   flush_block_counters(cnt, s_cnt, as_global(L.counters), log);
 }
 
+// --- numeric, 64 words per vertex, from the job list, by row segments ------------
+// The job list holds one u32 per set bit of A^T, (source vertex << 6) | target row, in tile
+// order, built once per graph from the tile payloads. A workgroup of this kernel handles one
+// scheduler chunk and one of G segments of the 4096-source row (64/G words): it gathers only
+// that segment of every source row, so each workgroup moves 512/G bytes per set bit and keeps
+// a 64 x (64/G)-word strip accumulator in LDS. Workgroup b takes chunk b / G and segment
+// b % G; workgroups are placed on the eight XCDs round-robin by block index, so with G = 4
+// an XCD only ever gathers one segment and its L2 holds four times as many hot source rows
+// as when every workgroup gathers whole rows. No tile is decoded during the hop.
+//
+// Per strip: the gathers (lanes grouped by (64/G)/2, each lane 16 bytes of the segment,
+// kSegUnroll jobs in flight per group, source rows whose V bit is clear skipped), the
+// structural sweep of the strip's tiles (only in the chunk's owner segment, chunk % G, which
+// also counts tile products and edges), one barrier, then the write-out of the segment with
+// the N, V, Z bits it can see. The accumulator is double-buffered, so the write-out of one
+// strip overlaps the next strip's gathers; the mask slots are triple-buffered and flushed one
+// strip later, after the next barrier. Unsplit strips store their per-segment masks in
+// `part`, which seg_combine_kernel ORs into N, V, Z after the launch; pieces of split strips
+// OR into zeroed memory and finalize_strips_kernel completes them, as for the sweep kernel.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+constexpr int kSegWaves = 4;
+constexpr int kSegBlock = kSegWaves * kWave;
+#ifndef MG_SEG_UNROLL
+#define MG_SEG_UNROLL 4
+#endif
+constexpr int kSegUnroll = MG_SEG_UNROLL;  // jobs in flight per lane group
+constexpr uint32_t kNoJob = 0xFFFFFFFFu;
+// Largest vertex count a u32 job can address (26 bits of source vertex).
+constexpr uint64_t kJobMaxVertices = 1ull << 26;
+
+// Set bits per A^T tile, from its payload (the compact entry count, or a dense tile's count).
+__global__ void __launch_bounds__(kBlock) tile_bits_kernel(const uint64_t* __restrict__ payload, uint32_t n,
+                                                           uint32_t* __restrict__ bits) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t pl = payload[i];
+    bits[i] = (pl & kPayloadDense) ? static_cast<uint32_t>(pl & 0x1FFFu) : static_cast<uint32_t>((pl >> 48) & 7u);
+  }
+}
+
+// Writes the jobs of every A^T tile at job_off[i]: lane = tile for the compact payloads; a
+// dense tile is expanded by the whole wave (lane = row) from its compacted rows.
+__global__ void __launch_bounds__(kBlock) job_fill_kernel(const uint64_t* __restrict__ payload,
+                                                          const uint32_t* __restrict__ col,
+                                                          const uint64_t* __restrict__ dense,
+                                                          const uint32_t* __restrict__ job_off, uint32_t n,
+                                                          uint32_t* __restrict__ jobs) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWaves;
+  for (uint32_t g0 = read_first_lane((blockIdx.x * kWaves + threadIdx.x / kWave) * 64u); g0 < n;
+       g0 += n_waves * 64u) {
+    const uint32_t i = g0 + lane;
+    const bool valid = i < n;
+    const uint64_t pl = valid ? payload[i] : 0ull;
+    const uint32_t t = valid ? col[i] : 0u;
+    const uint32_t at = valid ? job_off[i] : 0u;
+    const bool is_dense = valid && (pl & kPayloadDense) != 0;
+    if (valid && !is_dense) {
+      const uint32_t n_ent = static_cast<uint32_t>(pl >> 48) & 7u;
+      for (uint32_t k = 0; k < n_ent; ++k) {
+        const uint32_t e = static_cast<uint32_t>(pl >> (12 * k)) & 0xFFFu;  // (row << 6) | column
+        jobs[at + k] = ((t * 64u + (e & 63u)) << 6) | (e >> 6);
+      }
+    }
+    for (uint64_t dm = ballot(is_dense); dm; dm &= dm - 1) {
+      const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+      const uint64_t slot = (read_lane(pl, k) >> 13) & 0xFFFFFFFFull;
+      const uint32_t tk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t), k));
+      const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(at), k));
+      uint64_t word = dense[slot * kTile + lane];
+      // Exclusive prefix of the rows' bit counts across the wave.
+      uint32_t pos = __popcll(word);
+#pragma unroll
+      for (int m = 1; m < kWave; m <<= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(pos), m, kWave));
+        if (lane >= static_cast<uint32_t>(m)) pos += o;
+      }
+      pos -= __popcll(word);
+      for (; word; word &= word - 1) {
+        const uint32_t c = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
+        jobs[base + pos++] = ((tk * 64u + c) << 6) | lane;
+      }
+    }
+  }
+}
+
+template <int G>
+__global__ void __launch_bounds__(kSegBlock) walk_seg_kernel(const Chunk* __restrict__ chunks,
+                                                             const PanelLevel* __restrict__ lv) {
+  constexpr int SW = 64 / G;         // words of a row in one segment
+  constexpr int LJ = SW / 2;         // lanes per job (two words each)
+  constexpr int J = kWave / LJ;      // jobs per wave step
+  constexpr int NG = kSegWaves * J;  // lane groups per workgroup
+  constexpr int kAcc = 64 * SW;      // words of one strip accumulator
+  static_assert(SW >= 2 && kAcc % kSegBlock == 0, "segment shape");
+  __shared__ unsigned long long acc[2][kAcc];
+  __shared__ unsigned long long s_mask[3][4];  // N, V, Z, S of the strip in each slot
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t cid = blockIdx.x / G;
+  const uint32_t seg = blockIdx.x % G;
+  const Chunk ch = chunks[cid];
+  const PanelLevel& L = lv[ch.panel];
+  const bool owner = (cid % G) == seg;  // structural sweep, tile counts and edges
+  const gptr<const uint32_t> __restrict__ off = as_global(L.off);
+  const gptr<const uint32_t> __restrict__ col = as_global(L.col);
+  const gptr<const uint32_t> __restrict__ jobs = as_global(L.jobs);
+  const gptr<const uint32_t> __restrict__ job_off = as_global(L.job_off);
+  const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
+  const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
+  const gptr<uint64_t> x_out = as_global(L.x_out);
+  const uint64_t alive_in = L.alive_in;
+  const uint64_t alive_next = L.alive_next;
+  const uint32_t sub = lane % LJ;
+  const uint32_t gi = wave * J + lane / LJ;
+  const uint32_t w0 = seg * SW + 2 * sub;  // first of this lane's two words of a row
+  const uint64_t keep0 = ((alive_in >> w0) & 1) ? ~0ull : 0ull;
+  const uint64_t keep1 = ((alive_in >> (w0 + 1)) & 1) ? ~0ull : 0ull;
+  const gptr<const char> xb = reinterpret_cast<gptr<const char>>(as_global(L.x_in)) + 8u * w0;
+  // Per-strip values of the chunk, one strip per lane: row pointers, the job offsets of the
+  // strip boundaries (clamped to the chunk's tile range), previous Z and N masks, S_k & alive.
+  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
+  const uint32_t ys = ch.strip_begin + lane;
+  const uint32_t off_l = lane <= n_cs ? off[ys] : 0u;
+  const uint32_t cl_l = off_l < ch.tile_begin ? ch.tile_begin : (off_l > ch.tile_end ? ch.tile_end : off_l);
+  const uint32_t jo_l = lane <= n_cs ? job_off[cl_l] : 0u;
+  const uint64_t zp_l = lane < n_cs ? as_global(L.z_out)[ys] : 0ull;
+  const uint64_t np_l = lane < n_cs ? as_global(L.n_out)[ys] : 0ull;
+  const uint64_t left_l = (owner && lane < n_cs) ? (s_in[ys] & alive_in) : 0ull;
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 12) (&s_mask[0][0])[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < 2 * kAcc; i += kSegBlock) (&acc[0][0])[i] = 0;
+  __syncthreads();
+
+  // Stores strip i's masks (slot i % 3), once every wave has contributed; thread 0 only.
+  auto flush_masks = [&](uint32_t i) {
+    const uint32_t y = ch.strip_begin + i;
+    const uint32_t slot = i % 3;
+    const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i)));
+    const uint32_t r1 =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i + 1)));
+    const bool split = r0 < ch.tile_begin || r1 > ch.tile_end;
+    const uint64_t sym = s_mask[slot][3];
+    const gptr<uint64_t> part = as_global(L.part) + (static_cast<uint64_t>(seg) * L.n_strips + y) * 3;
+    if (!split) {
+      part[0] = s_mask[slot][0];
+      part[1] = s_mask[slot][1];
+      part[2] = s_mask[slot][2];
+      cnt[bNumeric] += __popcll(s_mask[slot][0]);
+      if (owner) {
+        as_global(L.s_out)[y] = sym;
+        cnt[bOut] += __popcll(sym);
+        cnt[bLeft] += __popcll(read_lane(left_l, static_cast<int>(i)));
+      }
+    } else {
+      part[0] = 0;
+      part[1] = 0;
+      part[2] = 0;
+      if (owner && sym) atomicOr((unsigned long long*)(as_global(L.s_out) + y), static_cast<unsigned long long>(sym));
+    }
+    s_mask[slot][0] = 0;
+    s_mask[slot][1] = 0;
+    s_mask[slot][2] = 0;
+    s_mask[slot][3] = 0;
+  };
+
+  for (uint32_t i = 0; i < n_cs; ++i) {
+    const uint32_t b = i & 1;
+    const uint32_t slot = i % 3;
+    const uint32_t y = ch.strip_begin + i;
+    const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i)));
+    const uint32_t r1 =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i + 1)));
+    const uint32_t t0 = r0 > ch.tile_begin ? r0 : ch.tile_begin;
+    const uint32_t t1 = r1 < ch.tile_end ? r1 : ch.tile_end;
+    const bool split = (t0 != r0) || (t1 != r1);
+    const uint32_t ja = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jo_l), static_cast<int>(i)));
+    const uint32_t jb =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jo_l), static_cast<int>(i + 1)));
+    char* const accb = reinterpret_cast<char*>(acc[b]) + 16u * sub;
+    // Gathers: group gi takes jobs ja + gi, ja + gi + NG, ...
+    for (uint32_t base = ja + gi; base < jb; base += NG * kSegUnroll) {
+      uint32_t jv[kSegUnroll];
+#pragma unroll
+      for (int k = 0; k < kSegUnroll; ++k) {
+        const uint32_t j = base + NG * k;
+        jv[k] = j < jb ? __builtin_nontemporal_load(jobs + j) : kNoJob;
+      }
+      uint64_t vw[kSegUnroll];
+#pragma unroll
+      for (int k = 0; k < kSegUnroll; ++k) vw[k] = jv[k] != kNoJob ? v_in[jv[k] >> 12] : 0ull;
+      u64x2 val[kSegUnroll];
+#pragma unroll
+      for (int k = 0; k < kSegUnroll; ++k) {
+        const uint32_t u = jv[k] >> 6;
+        const bool ok = (vw[k] >> (u & 63u)) & 1;
+        if (owner && sub == 0) cnt[bEdges] += ok;
+        val[k] = ok ? *reinterpret_cast<gptr<const u64x2>>(xb + static_cast<uint64_t>(u) * 512u) : u64x2{0ull, 0ull};
+      }
+#pragma unroll
+      for (int k = 0; k < kSegUnroll; ++k) {
+        const uint64_t v0 = val[k].x & keep0, v1 = val[k].y & keep1;
+        if (v0 | v1) {
+          unsigned long long* a = reinterpret_cast<unsigned long long*>(accb + (jv[k] & 63u) * (8u * SW));
+          atomicOr(a, v0);
+          atomicOr(a + 1, v1);
+        }
+      }
+    }
+    // Structural sweep of the strip's tiles (owner segment only).
+    if (owner) {
+      uint64_t sym = 0;
+      for (uint32_t p = t0 + threadIdx.x; p < t1; p += kSegBlock) {
+        const uint64_t sm = s_in[__builtin_nontemporal_load(col + p)] & alive_in;
+        sym |= sm;
+        cnt[bTP] += __popcll(sm);
+        cnt[bUsed] += sm != 0;
+      }
+      sym = wave_or(sym);
+      if (lane == 0 && sym) atomicOr(&s_mask[slot][3], static_cast<unsigned long long>(sym));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && i > 0) flush_masks(i - 1);
+    // Write-out of this segment of the strip; the accumulator is cleared for strip i + 2.
+    const uint64_t zp = read_lane(zp_l, static_cast<int>(i));
+    const uint64_t np = read_lane(np_l, static_cast<int>(i));
+    const gptr<uint64_t> dst = x_out + static_cast<uint64_t>(y) * 64u * 64u + seg * SW;
+    uint64_t nmask = 0, vmask = 0, zmask = 0;
+    uint32_t pc = 0;
+#pragma unroll
+    for (int m = 0; m < kAcc / kSegBlock; ++m) {
+      const uint32_t idx = threadIdx.x + m * kSegBlock;
+      const uint32_t r = idx / SW, w = seg * SW + idx % SW;
+      const uint64_t val = acc[b][idx];
+      acc[b][idx] = 0;
+      if (!split) {
+        if (val) {
+          nmask |= 1ull << w;
+          zmask |= 1ull << r;
+          if ((alive_next >> w) & 1) vmask |= 1ull << r;
+          pc += __popcll(val);
+        }
+        if (val != 0 || (((zp >> r) & 1) && ((np >> w) & 1))) __builtin_nontemporal_store(val, dst + r * 64u + (idx % SW));
+      } else if (val) {
+        atomicOr((unsigned long long*)(dst + r * 64u + (idx % SW)), static_cast<unsigned long long>(val));
+      }
+    }
+    if (!split) {
+      cnt[bPop] += pc;
+      nmask = wave_or(nmask);
+      vmask = wave_or(vmask);
+      zmask = wave_or(zmask);
+      if (lane == 0) {
+        if (nmask) atomicOr(&s_mask[slot][0], static_cast<unsigned long long>(nmask));
+        if (vmask) atomicOr(&s_mask[slot][1], static_cast<unsigned long long>(vmask));
+        if (zmask) atomicOr(&s_mask[slot][2], static_cast<unsigned long long>(zmask));
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && n_cs > 0) flush_masks(n_cs - 1);
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
+}
+
+// N, V, Z of every unsplit strip from the G segments' partial masks (split strips get zeros
+// here and their masks from finalize_strips_kernel, which runs after this).
+__global__ void __launch_bounds__(kBlock) seg_combine_kernel(const PanelLevel* __restrict__ lv, uint32_t n_segs) {
+  const PanelLevel& L = lv[blockIdx.y];
+  const uint32_t ns = L.n_strips;
+  for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x; y < ns; y += gridDim.x * blockDim.x) {
+    uint64_t n = 0, v = 0, z = 0;
+    for (uint32_t sg = 0; sg < n_segs; ++sg) {
+      const uint64_t* p = L.part + (static_cast<uint64_t>(sg) * ns + y) * 3;
+      n |= p[0];
+      v |= p[1];
+      z |= p[2];
+    }
+    L.n_out[y] = n;
+    L.v_out[y] = v;
+    L.z_out[y] = z;
+  }
+}
+
 // Zero the strips that are written by several chunks (atomic OR targets) and their
 // structural masks.
 template <int W>
@@ -1273,6 +1563,11 @@ struct GraphPlan {
   DevBuf<uint32_t> csc_row;
   DevBuf<uint64_t> csc_payload;
   std::vector<uint32_t> host_csc_off;
+  uint64_t total_bits = 0;   // set bits of A^T (chunk_costs)
+  // Job list of A^T for the segmented hop kernel (build_job_list): empty unless built.
+  DevBuf<uint32_t> jobs;
+  DevBuf<uint32_t> job_off;
+  bool has_jobs = false;
 };
 
 // Launch-order key of each chunk, in hundredths: its set bits + 0.35 per tile + 24 per
@@ -1280,7 +1575,9 @@ struct GraphPlan {
 // reads, strips the write-out; heaviest first leaves light chunks for the tail of a launch.
 // Within one allocation this order beat ordering by tiles alone by 2-4 % on hops 2-4 of the
 // 4096-source walk on R-MAT 22 (tiles alone beat strip order by 4 % and reversed by 35 %).
-std::vector<uint64_t> chunk_costs(const GraphPlan& g, hipStream_t st) {
+// *total_bits receives the set bits of all chunks (every tile once).
+std::vector<uint64_t> chunk_costs(const GraphPlan& g, hipStream_t st, uint64_t* total_bits) {
+  *total_bits = 0;
   const size_t n = g.chunks.size();
   std::vector<uint64_t> cost(n, 0);
   if (!n || !g.at->n_nz_tile) return cost;
@@ -1295,8 +1592,41 @@ std::vector<uint64_t> chunk_costs(const GraphPlan& g, hipStream_t st) {
   for (size_t i = 0; i < n; ++i) {
     const Chunk& c = g.chunks[i];
     cost[i] = 100ull * hb[i] + 35ull * (c.tile_end - c.tile_begin) + 2400ull * (c.strip_end - c.strip_begin);
+    *total_bits += hb[i];
   }
   return cost;
+}
+
+// Whether the segmented hop kernel can run on this graph: every job must fit in a u32
+// (source vertex below 2^26) and every job position, plus a workgroup's look-ahead past the
+// end of a range, in a u32.
+bool job_list_fits(const GraphPlan& g) {
+  return static_cast<uint64_t>(g.n_strips) * 64 <= kJobMaxVertices && g.total_bits <= 0xFFFF0000ull;
+}
+
+// Builds the job list of A^T (once per graph): per-tile bit counts from the payloads, their
+// exclusive scan as job_off, then job_fill_kernel. 4 bytes per set bit plus 4 per tile.
+void build_job_list(GraphPlan& g, hipStream_t st) {
+  if (g.has_jobs) return;
+  MG_REQUIRE(job_list_fits(g), "walk: graph too large for the job-list hop kernel");
+  const uint32_t n = g.at->n_nz_tile;
+  g.job_off.alloc(n + 1ull);
+  g.jobs.alloc(std::max<uint64_t>(1, g.total_bits));
+  if (n) {
+    DevBuf<uint32_t> bits(n);
+    tile_bits_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.payload.get(), n, bits.get());
+    MG_LAUNCH_CHECK();
+    exclusive_scan_u32_on(st, bits.get(), g.job_off.get(), n);
+    job_fill_kernel<<<grid_for(n, kWaves * 64, 65536), kBlock, 0, st>>>(g.payload.get(), g.at->col_idx.get(),
+                                                                        g.dense.get(), g.job_off.get(), n,
+                                                                        g.jobs.get());
+    MG_LAUNCH_CHECK();
+    MG_HIP(hipStreamSynchronize(st));  // bits is freed on return
+  } else {
+    MG_HIP(hipMemsetAsync(g.job_off.get(), 0, sizeof(uint32_t), st));
+    MG_HIP(hipStreamSynchronize(st));
+  }
+  g.has_jobs = true;
 }
 
 // Every launch and copy is ordered on `st`, the caller's context stream, so a matrix still
@@ -1335,7 +1665,7 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_
     // flag and pos are freed on return: their last readers must have finished.
     MG_HIP(hipStreamSynchronize(st));
   }
-  g.chunk_cost = chunk_costs(g, st);
+  g.chunk_cost = chunk_costs(g, st, &g.total_bits);
   return g;
 }
 
@@ -1408,11 +1738,16 @@ struct Panel {
   uint64_t words_used = 0;  // words of X holding sources
   std::vector<uint64_t> alive;  // alive[k]: words that propagate from X_k to X_{k+1}
   std::vector<DevBuf<uint64_t>> x, s, n, v, z;
+  DevBuf<uint64_t> part;  // segmented hop kernel: partial masks, Engine::seg x 3 per strip
 };
 
 // Device launch plan for a set of panels over levels 0..max_hops-1.
 struct Engine {
   uint32_t W = 64;
+  // Hop kernel of the 64-word levels from level seg_from on: 0 = tile-decoding sweep
+  // (walk_numeric64_kernel), G = 1, 2, 4 or 8 = job-list kernel with G row segments.
+  uint32_t seg = 0;
+  uint32_t seg_from = 1;
   std::vector<GraphPlan> graphs;
   std::vector<Panel> panels;
   uint32_t max_hops = 0;
@@ -1422,6 +1757,7 @@ struct Engine {
   std::vector<DevBuf<ChunkLog>> logs;           // per-chunk outcomes (when logging)
   bool logging = false;
   std::vector<DevBuf<PanelLevel>> lv;
+  std::vector<std::vector<PanelLevel>> host_lv;  // host copies of lv
   std::vector<DevBuf<Chunk>> chunks;
   std::vector<uint32_t> n_chunks;
   std::vector<DevBuf<StripItem>> split;
@@ -1483,6 +1819,7 @@ void build_engine(Engine* e, hipStream_t st) {
   }
   e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kCounterStride);
   e->lv.resize(e->max_hops);
+  e->host_lv.clear();
   e->chunks.resize(e->max_hops);
   e->host_chunks.assign(e->max_hops, {});
   e->n_chunks.assign(e->max_hops, 0);
@@ -1556,6 +1893,7 @@ void build_engine(Engine* e, hipStream_t st) {
     }
     e->lv[k].alloc(P);
     MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice, st));
+    e->host_lv.push_back(lv);
     e->n_chunks[k] = static_cast<uint32_t>(ch.size());
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
@@ -1591,6 +1929,51 @@ void build_engine(Engine* e, hipStream_t st) {
   MG_HIP(hipStreamSynchronize(st));
 }
 
+// Selects the hop kernel of the 64-word levels (Engine::seg): builds the job lists of the
+// graphs and the panels' partial-mask buffers on first use and points the level tables at
+// them. G = 0 keeps the tile-decoding sweep. Not called while a run may be in flight.
+void apply_seg(Engine* e, uint32_t G, hipStream_t st) {
+  MG_REQUIRE(G == 0 || G == 1 || G == 2 || G == 4 || G == 8, "walk: row segments must be 0, 1, 2, 4 or 8");
+  if (G && e->W != 64) G = 0;  // narrower panels keep walk_numeric_kernel<W>
+  if (G)
+    for (auto& g : e->graphs)
+      if (!job_list_fits(g)) G = 0;
+  if (G) {
+    for (auto& g : e->graphs) build_job_list(g, st);
+    for (auto& p : e->panels) {
+      const uint64_t need = static_cast<uint64_t>(G) * 3 * std::max<uint32_t>(1, p.n_strips);
+      if (p.part.size() < need) {
+        p.part.release();
+        p.part.alloc(need);
+      }
+    }
+  }
+  e->seg = G;
+  for (uint32_t k = 0; k < e->host_lv.size(); ++k) {
+    auto& lv = e->host_lv[k];
+    for (uint32_t pi = 0; pi < lv.size(); ++pi) {
+      if (!lv[pi].x_in) continue;  // panel not active at this level
+      const GraphPlan& g = e->graphs[e->panels[pi].graph];
+      lv[pi].jobs = g.has_jobs ? g.jobs.get() : nullptr;
+      lv[pi].job_off = g.has_jobs ? g.job_off.get() : nullptr;
+      lv[pi].part = e->panels[pi].part.get();
+    }
+    MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * lv.size(), hipMemcpyHostToDevice, st));
+  }
+  MG_HIP(hipStreamSynchronize(st));
+}
+
+// Default hop kernel of 64-word levels (kDefaultSegments); MG_WALK_SEGMENTS (0, 1, 2, 4 or 8)
+// overrides it.
+#ifndef MG_DEFAULT_SEGMENTS
+#define MG_DEFAULT_SEGMENTS 0
+#endif
+constexpr uint32_t kDefaultSegments = MG_DEFAULT_SEGMENTS;
+uint32_t default_segments() {
+  const char* env = std::getenv("MG_WALK_SEGMENTS");
+  return env ? static_cast<uint32_t>(std::atoi(env)) : kDefaultSegments;
+}
+
 template <int W>
 void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   // The structural pass runs inside the numeric kernel: its event pair brackets nothing.
@@ -1601,15 +1984,32 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[2], st));
+  const uint32_t G = (W == 64 && e->seg && !e->logging && k >= e->seg_from) ? e->seg : 0;
   if (e->n_chunks[k]) {
     ChunkLog* const log = e->logging ? e->logs[k].get() : nullptr;
-    if constexpr (W == 64)
-      walk_numeric64_kernel<<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log);
-    else
+    if constexpr (W == 64) {
+      const uint32_t nb = e->n_chunks[k];
+      switch (G) {
+        case 0: walk_numeric64_kernel<<<nb, kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log); break;
+        case 1: walk_seg_kernel<1><<<nb, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        case 2: walk_seg_kernel<2><<<nb * 2, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        case 4: walk_seg_kernel<4><<<nb * 4, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        case 8: walk_seg_kernel<8><<<nb * 8, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        default: fail(MG_ERR_INTERNAL, "walk: bad segment count");
+      }
+    } else {
       walk_numeric_kernel<W><<<e->n_chunks[k], kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log);
+    }
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
+  if (G) {
+    const uint32_t n_panels = static_cast<uint32_t>(e->panels.size());
+    uint32_t max_strips = 1;
+    for (const auto& p : e->panels) max_strips = std::max(max_strips, p.n_strips);
+    seg_combine_kernel<<<dim3(grid_for(max_strips, kBlock, 1024), n_panels), kBlock, 0, st>>>(e->lv[k].get(), G);
+    MG_LAUNCH_CHECK();
+  }
   if (e->n_split[k]) {
     finalize_strips_kernel<W><<<e->n_split[k], kBlock, 0, st>>>(e->split[k].get(), e->lv[k].get());
     MG_LAUNCH_CHECK();
@@ -1829,6 +2229,7 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     // The column index first: the level tables built next point at it.
     if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) build_column_index(e.graphs[0], ctx->stream);
     build_engine(&e, ctx->stream);
+    apply_seg(&e, default_segments(), ctx->stream);
     e.sources.alloc(max_sources);
     w->staging.alloc(max_sources);
     if (!(flags & MG_WALK_SWEEP_FIRST_HOP)) {
@@ -1852,6 +2253,23 @@ extern "C" int mg_walk_set_timing(mg_walk* w, int enable) {
     const bool on = enable != 0;
     if (on && !w->timing) w->reset_log();
     w->timing = on;
+  });
+}
+
+extern "C" int mg_walk_set_hop_kernel(mg_context* ctx, mg_walk* w, uint32_t segments) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && w, "mg_walk_set_hop_kernel: NULL argument");
+    MG_REQUIRE(ctx->device == w->device, "mg_walk_set_hop_kernel: walk belongs to another device");
+    DeviceGuard g(ctx->device);
+    MG_HIP(hipStreamSynchronize(ctx->stream));  // no run may still read the level tables
+    apply_seg(&w->e, segments, ctx->stream);
+  });
+}
+
+extern "C" int mg_walk_hop_kernel(const mg_walk* w, uint32_t* segments) {
+  return guarded([&] {
+    MG_REQUIRE(w && segments, "mg_walk_hop_kernel: NULL argument");
+    *segments = w->e.seg;
   });
 }
 
@@ -1932,7 +2350,14 @@ extern "C" int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_h
     MG_HIP(hipMemcpyAsync(c, w->e.ctr(0, hop), sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
     hop_stats_from(c, (w->n_sources + 63) / 64, out);
-    out->kernel = (hop == 1 && w->last_push) ? MG_HOP_PUSH : MG_HOP_SWEEP;
+    if (hop == 1 && w->last_push) {
+      out->kernel = MG_HOP_PUSH;
+    } else if (w->e.W == 64 && w->e.seg && hop - 1 >= w->e.seg_from) {
+      out->kernel = MG_HOP_JOBS;
+      out->segments = w->e.seg;
+    } else {
+      out->kernel = MG_HOP_SWEEP;
+    }
     if (!w->run_events.empty() && w->run_events.back().size() >= 4ull * hop) {
       const auto& ev = w->run_events.back();
       float a = 0, b = 0;
@@ -2226,6 +2651,7 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
       }
     }
     build_engine(&e, ctx->stream);
+    apply_seg(&e, default_segments(), ctx->stream);
     e.n_push_items = static_cast<uint32_t>(push_items.size());
     e.push_items.alloc(std::max<size_t>(1, push_items.size()));
     if (!push_items.empty())
@@ -2397,6 +2823,17 @@ void drop_graph(mg_walk_batch* b) {
   b->graph = nullptr;
 }
 }  // namespace
+
+extern "C" int mg_walk_batch_set_hop_kernel(mg_context* ctx, mg_walk_batch* b, uint32_t segments) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && b, "mg_walk_batch_set_hop_kernel: NULL argument");
+    MG_REQUIRE(ctx->device == b->device, "mg_walk_batch_set_hop_kernel: batch belongs to another device");
+    DeviceGuard g(ctx->device);
+    MG_HIP(hipDeviceSynchronize());  // no run of this batch may still read its tables
+    drop_graph(b);                   // kernel choice changes: capture again
+    apply_seg(&b->e, segments, ctx->stream);
+  });
+}
 
 extern "C" int mg_walk_batch_set_logging(mg_walk_batch* b, int enable) {
   return guarded([&] {

@@ -99,3 +99,31 @@ def test_batch_first_hop_push_equals_sweep(ctx, packing, n_queries, max_sources)
         xs = O.walk_frontiers(n, edges[q.graph][0], edges[q.graph][1], q.sources, q.hops,
                               W=(len(q.sources) + 63) // 64)
         assert np.array_equal(bp.query_frontier(i), xs[q.hops]), i
+
+
+@pytest.mark.parametrize("segments", [2, 8])
+def test_batch_job_list_kernel_equals_sweep(ctx, segments):
+    """Batch levels after the first by the job-list kernel (queries packed in panels, finished
+    queries' words masked) give the sweep kernel's per-query frontiers and batch totals, on a
+    captured graph replayed twice, then again after switching kernels."""
+    M = mg()
+    specs, graphs, qs = small_workload(ctx, 30, 17)
+    bj = M.WalkBatch(ctx, graphs, qs)
+    bs = M.WalkBatch(ctx, graphs, qs)
+    bj.set_hop_kernel(segments)
+    bs.set_hop_kernel(0)
+    for b in (bj, bs):
+        b.run()
+        b.run()
+    ctx.synchronize()
+    assert bj.totals() == bs.totals()
+    for i in range(len(qs)):
+        assert bj.query_digest(i) == bs.query_digest(i), i
+    bj.set_hop_kernel(0)
+    bs.set_hop_kernel(segments)
+    for b in (bj, bs):
+        b.run()
+    ctx.synchronize()
+    assert bj.totals() == bs.totals()
+    for i in range(0, len(qs), 3):
+        assert bj.query_digest(i) == bs.query_digest(i), i

@@ -227,9 +227,9 @@ def test_walk_first_hop_push_equals_sweep(ctx, scale, seed, sizes, flags):
             assert np.array_equal(wp.frontier(k), ws.frontier(k)), f"run {r} hop {k}"
             a, b = wp.hop_stats(k), ws.hop_stats(k)
             assert {f: a[f] for f in _STAT_KEYS} == {f: b[f] for f in _STAT_KEYS}, f"run {r} hop {k}"
-            assert b["kernel"] == M.HOP_SWEEP
+            assert b["kernel"] == (M.HOP_SWEEP if k == 1 else a["kernel"])
             if k > 1:
-                assert a["kernel"] == M.HOP_SWEEP
+                assert a["kernel"] in (M.HOP_SWEEP, M.HOP_JOBS)
         pushed += wp.hop_stats(1)["kernel"] == M.HOP_PUSH
     assert pushed >= 1
     xs = O.walk_frontiers(n, s, d, sources, hops, W=wp.words_per_vertex)
@@ -260,3 +260,67 @@ def test_walk_rmat22_full_scale_digests(ctx):
         assert w.frontier_digest(k) == O.frontier_digest(x, n_src), f"hop {k}"
         assert w.hop_stats(k)["popcount"] == popcount(x)
 
+
+
+def _stats(w, k):
+    st = w.hop_stats(k)
+    return {f: st[f] for f in _STAT_KEYS}
+
+
+@pytest.mark.parametrize("segments", [1, 2, 4, 8])
+def test_job_list_kernel_equals_sweep_and_oracle(ctx, segments):
+    """R-MAT 18 (hub strips are split across chunks), 4096 sources: the job-list hop kernel with
+    1, 2, 4 or 8 row segments gives the sweep kernel's frontiers and statistics at every hop,
+    over runs whose source sets shrink and grow again (stores skipped outside the previous
+    masks), and the oracle's frontiers on the last run."""
+    M = mg()
+    scale, seed, hops = 18, 9, 4
+    n = 1 << scale
+    el, at, s, d = setup_graph(ctx, scale, seed)
+    wj = M.Walk(ctx, at, 4096, hops)
+    ws = M.Walk(ctx, at, 4096, hops)
+    assert wj.set_hop_kernel(segments) == segments
+    assert ws.set_hop_kernel(0) == 0
+    for r, size in enumerate((4096, 700, 4096)):
+        sources = O.sample_sources(n, size, 40 + r)
+        wj.run(sources, hops)
+        ws.run(sources, hops)
+        for k in range(1, hops + 1):
+            assert np.array_equal(wj.frontier(k), ws.frontier(k)), f"run {r} hop {k}"
+            assert _stats(wj, k) == _stats(ws, k), f"run {r} hop {k}"
+            if k > 1:
+                assert wj.hop_stats(k)["kernel"] == M.HOP_JOBS and wj.hop_stats(k)["segments"] == segments
+                assert ws.hop_stats(k)["kernel"] == M.HOP_SWEEP
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=64)
+    for k in range(1, hops + 1):
+        assert np.array_equal(wj.frontier(k), xs[k]), f"oracle hop {k}"
+
+
+def test_job_list_kernel_ragged_graph(ctx):
+    """The job-list kernel on a 1000-vertex graph (partial last strip, dense tiles, walks that
+    die out, self-loops) with a 4096-source walk: every hop equals the oracle."""
+    M = mg()
+    n = 1000
+    s, d = ragged_graph()
+    at = M.EdgeList.upload(ctx, n, s, d).to_tiled_matrix(transpose=True)
+    sources = np.array([0, 999, 5, 650, 320, 640, 998] + list(range(50, 90)), np.uint32)
+    hops = 6
+    w = M.Walk(ctx, at, 4096, hops)
+    assert w.set_hop_kernel(4) == 4
+    w.run(sources, hops)
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=64)
+    for k in range(1, hops + 1):
+        got = w.frontier(k)
+        assert np.array_equal(got[:n], xs[k]), f"hop {k}"
+        assert not got[n:].any()
+        assert w.hop_stats(k)["popcount"] == popcount(xs[k])
+
+
+def test_hop_kernel_choice_errors(ctx):
+    M = mg()
+    el, at, s, d = setup_graph(ctx, 10, 1)
+    w = M.Walk(ctx, at, 4096, 2)
+    with pytest.raises(M.MGError):
+        w.set_hop_kernel(3)
+    narrow = M.Walk(ctx, at, 100, 2)
+    assert narrow.set_hop_kernel(4) == 0  # fewer than 2049 sources keep the sweep
