@@ -247,6 +247,18 @@ int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tile
  * No counterpart in the reference (its product is VALU only). */
 int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                               mg_tiled_matrix* c, uint32_t min_bits, uint64_t* dense_products);
+/* Allocation-free numeric pass: same result as mg_multiply_numeric, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Makes no hipMalloc/hipFree call: the
+ * operand payloads go to a static per-device array of MG_NOALLOC_SCRATCH_BYTES (held from
+ * library load until the process ends), so A and Bt may hold at most MG_NOALLOC_MAX_TILES
+ * tiles together; larger operands give MG_ERR_INVALID. Asynchronous on the context's stream.
+ * Every call on one device shares that array: calls that overlap on one device (other streams
+ * or contexts) race. */
+#define MG_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define MG_NOALLOC_MAX_TILES ((MG_NOALLOC_SCRATCH_BYTES / 16u) - 1u)
+int mg_multiply_numeric_noalloc(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                mg_tiled_matrix* c);
 
 /* ---------------------------------------------------------------------------
  * Edge lists and the edge-list -> bit-tile converter (input production).

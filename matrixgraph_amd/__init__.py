@@ -278,6 +278,15 @@ def multiply_numeric_dense(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: Til
     return n.value
 
 
+def multiply_numeric_noalloc(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: TiledMatrix) -> None:
+    """The numeric pass without device allocation (static per-device payload scratch; A and
+    Bt together at most NOALLOC_MAX_TILES tiles). Asynchronous on the context's stream."""
+    _check(lib().mg_multiply_numeric_noalloc(ctx.h, a.h, bt.h, c.h), "mg_multiply_numeric_noalloc")
+
+
+NOALLOC_MAX_TILES = (32 << 20) // 16 - 1
+
+
 def multiply(ctx: Context, a: TiledMatrix, bt: TiledMatrix):
     """C = A * B (B supplied transposed): symbolic then numeric."""
     c, st = multiply_symbolic(ctx, a, bt)

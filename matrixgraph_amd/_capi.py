@@ -97,6 +97,7 @@ SIGNATURES = {
     "mg_multiply_symbolic": (i32, [vp, vp, vp, C.POINTER(vp), C.POINTER(ProductStats)]),
     "mg_multiply_numeric": (i32, [vp, vp, vp, vp]),
     "mg_multiply_numeric_dense": (i32, [vp, vp, vp, vp, u32, P_u64]),
+    "mg_multiply_numeric_noalloc": (i32, [vp, vp, vp, vp]),
     # edges
     "mg_edges_rmat": (i32, [vp, u32, u32, f64, f64, f64, u64, C.POINTER(vp)]),
     "mg_edges_er": (i32, [vp, u32, u64, u32, u64, C.POINTER(vp)]),

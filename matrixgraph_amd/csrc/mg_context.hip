@@ -35,6 +35,7 @@ extern "C" int mg_context_create(int device, mg_context** out) {
       invalid("mg_context_create: device " + std::to_string(device) + " not present (" +
               std::to_string(n_dev) + " devices)");
     DeviceGuard g(device);
+    noalloc_scratch_resolve(device);
     hipDeviceProp_t prop;
     MG_HIP(hipGetDeviceProperties(&prop, device));
     auto* ctx = new mg_context();

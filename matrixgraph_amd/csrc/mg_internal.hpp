@@ -179,6 +179,16 @@ inline uint64_t* ctx_scratch(mg_context* ctx, size_t n) {
   return ctx->scratch.get();
 }
 
+// Static payload scratch of mg_multiply_numeric_noalloc (mg_spgemm.hip): 32 MiB per device,
+// 16 bytes per tile of both operands plus 16 (kNoAllocMaxTiles tiles in A and Bt together).
+constexpr size_t kNoAllocScratchBytes = 32ull << 20;
+constexpr uint64_t kNoAllocMaxTiles = kNoAllocScratchBytes / 16 - 1;
+constexpr int kMaxDevices = 64;
+// Looks up this device's address of the static scratch (device current); once per device.
+void noalloc_scratch_resolve(int device);
+// The address looked up by noalloc_scratch_resolve for `device`.
+void* noalloc_scratch(int device);
+
 // Helpers implemented in mg_matrix.hip / mg_graph.hip, shared across units.
 void alloc_tiled_matrix(mg_tiled_matrix* m, uint32_t n_strips, uint32_t n_nz_tile);
 uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words);

@@ -9,6 +9,7 @@
 // instructions; dense tiles (64 non-zero rows) take 64 iterations. The handle-based numeric
 // pass multiplies pairs of tiles with at most 10 bits from their entry lists instead, one
 // pair per lane (fill_tiles_payload_kernel), and keeps the wave product for denser tiles.
+#include <mutex>
 #include <string>
 
 #include "mg_device.hpp"
@@ -587,13 +588,17 @@ void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const
 // Numeric pass of the handle-level product: payloads of both operands (stream-ordered in the
 // context's scratch), then fill_tiles_payload_kernel; with bits_a/bits_bt, pairs of tiles with
 // at least min_bits bits each go to the matrix cores and are counted into dense_count.
+// `pl` holds payload_words(a, bt) words of device scratch (the context's, or the static array
+// of the allocation-free entry point).
+size_t payload_words(const mg_tiled_matrix* a, const mg_tiled_matrix* bt) {
+  return 2 * (static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1);
+}
+
 void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
-                            mg_tiled_matrix* c, const uint32_t* bits_a, const uint32_t* bits_bt, uint32_t min_bits,
-                            unsigned long long* dense_count) {
+                            mg_tiled_matrix* c, WidePayload* pl, const uint32_t* bits_a, const uint32_t* bits_bt,
+                            uint32_t min_bits, unsigned long long* dense_count) {
   if (c->n_nz_tile == 0) return;
   hipStream_t st = ctx->stream;
-  WidePayload* pl =
-      reinterpret_cast<WidePayload*>(ctx_scratch(ctx, 2 * (static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1)));
   WidePayload* pl_a = pl;
   WidePayload* pl_bt = pl + a->n_nz_tile;
   if (a->n_nz_tile) {
@@ -621,6 +626,37 @@ void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_
 }
 
 }  // namespace
+
+// Payload scratch of mg_multiply_numeric_noalloc: one copy per device, held from the moment
+// the code object is loaded until the process ends. Host code never passes the name to a
+// kernel: it stands for a host-side shadow, not a device address. noalloc_scratch_resolve
+// (called by mg_context_create with the device current) looks up this device's address once;
+// every later use goes through that pointer. All calls on one device share the array, so
+// calls that overlap on one device (different streams or contexts) race.
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / sizeof(uint64_t)];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_scratch_resolve(int device) {
+  MG_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  MG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  MG_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned NULL for the static scratch");
+  g_scratch_ptr[device] = ptr;
+}
+
+void* noalloc_scratch(int device) {
+  MG_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  MG_REQUIRE(g_scratch_ptr[device] != nullptr, "static scratch not resolved for this device");
+  return g_scratch_ptr[device];
+}
+
 }  // namespace mgamd
 
 using namespace mgamd;
@@ -774,7 +810,30 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric: operands differ in n_strips");
     DeviceGuard g(ctx->device);
-    launch_payload_product(ctx, a, bt, c, nullptr, nullptr, 0, nullptr);
+    if (c->n_nz_tile == 0) return;
+    WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
+    launch_payload_product(ctx, a, bt, c, pl, nullptr, nullptr, 0, nullptr);
+  });
+}
+
+extern "C" int mg_multiply_numeric_noalloc(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                           mg_tiled_matrix* c) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && c, "mg_multiply_numeric_noalloc: NULL argument");
+    MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
+               "mg_multiply_numeric_noalloc: operands differ in n_strips");
+    const size_t words = payload_words(a, bt);
+    if (words * sizeof(uint64_t) > kNoAllocScratchBytes)
+      invalid("mg_multiply_numeric_noalloc: " + std::to_string(a->n_nz_tile) + " + " +
+              std::to_string(bt->n_nz_tile) + " operand tiles need " + std::to_string(words * sizeof(uint64_t)) +
+              " bytes of payload scratch, more than the static " + std::to_string(kNoAllocScratchBytes) +
+              " (at most " + std::to_string(kNoAllocMaxTiles) + " tiles in A and Bt together)");
+    DeviceGuard g(ctx->device);
+    if (c->n_nz_tile == 0) return;
+    // The device address of this device's copy of `scratch`, looked up when the context was
+    // created; the kernels write every payload word they read, so nothing is cleared.
+    WidePayload* pl = reinterpret_cast<WidePayload*>(noalloc_scratch(ctx->device));
+    launch_payload_product(ctx, a, bt, c, pl, nullptr, nullptr, 0, nullptr);
   });
 }
 
@@ -799,7 +858,8 @@ extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix*
     }
     unsigned long long* cnt = ctx->counters.get();  // 64 slots
     MG_HIP(hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned long long), st));
-    launch_payload_product(ctx, a, bt, c, ba.get(), bb.get(), min_bits, cnt);
+    WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
+    launch_payload_product(ctx, a, bt, c, pl, ba.get(), bb.get(), min_bits, cnt);
     unsigned long long h[64];
     MG_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
     MG_HIP(hipStreamSynchronize(st));
