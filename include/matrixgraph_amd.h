@@ -325,8 +325,11 @@ int mg_sample_sources(uint32_t n_vertices, uint32_t count, uint64_t seed, uint32
 enum {
   MG_WALK_PRUNE = 1u,          /* structural tile set of X_{k+1} built from numerically
                                   non-empty X_k tiles instead of structural ones */
-  MG_WALK_SWEEP_FIRST_HOP = 2u /* never push the first hop: every hop sweeps all strips
-                                  (no column index of A^T is built) */
+  MG_WALK_SWEEP_FIRST_HOP = 2u, /* never push the first hop: every hop sweeps all strips
+                                   (no column index of A^T is built) */
+  MG_WALK_VERTEX_ORDER = 4u     /* keep the frontiers' strips in vertex order; by default the
+                                   engine stores and walks them in a fixed permuted order
+                                   (results, digests, downloads and exports are the same) */
 };
 enum {
   MG_HOP_SWEEP = 0u, /* the hop swept every strip of A^T (walk_numeric kernels) */
@@ -435,8 +438,9 @@ typedef struct mg_batch_options {
   uint32_t flags;       /* MG_BATCH_* */
 } mg_batch_options;
 enum {
-  MG_BATCH_SWEEP_FIRST_HOP = 1u /* never push a panel's first hop: every level-0 chunk is swept
-                                   (and logged); no column index of A^T is built */
+  MG_BATCH_SWEEP_FIRST_HOP = 1u, /* never push a panel's first hop: every level-0 chunk is swept
+                                    (and logged); no column index of A^T is built */
+  MG_BATCH_VERTEX_ORDER = 2u     /* as MG_WALK_VERTEX_ORDER, for every graph of the batch */
 };
 
 /* graphs_at[g] is graph g supplied transposed (A^T); they must outlive the batch and keep

@@ -388,10 +388,12 @@ def sample_sources(n_vertices: int, count: int, seed: int) -> np.ndarray:
 
 WALK_PRUNE = 1
 WALK_SWEEP_FIRST_HOP = 2
+WALK_VERTEX_ORDER = 4  # keep frontier strips in vertex order (no internal strip permutation)
 HOP_SWEEP = 0
 HOP_PUSH = 1
 HOP_JOBS = 2
 BATCH_SWEEP_FIRST_HOP = 1
+BATCH_VERTEX_ORDER = 2
 
 
 class Walk:
@@ -616,7 +618,7 @@ class WalkBatch:
     run as one captured HIP graph (mg_walk_batch_*)."""
 
     def __init__(self, ctx: Context, graphs_at, queries, packing: int = 0, chunk_tiles: int = 0,
-                 use_graph: bool = True, sweep_first_hop: bool = False):
+                 use_graph: bool = True, sweep_first_hop: bool = False, vertex_order: bool = False):
         self.ctx = ctx
         self.graphs = list(graphs_at)
         self.queries = list(queries)
@@ -629,7 +631,8 @@ class WalkBatch:
             arr[i].hops = q.hops
         gp = (C.c_void_p * len(self.graphs))(*[g.h.value if hasattr(g.h, "value") else g.h for g in self.graphs])
         opt = _capi.BatchOptions(packing, chunk_tiles, 1 if use_graph else 0,
-                                 BATCH_SWEEP_FIRST_HOP if sweep_first_hop else 0)
+                                 (BATCH_SWEEP_FIRST_HOP if sweep_first_hop else 0)
+                                 | (BATCH_VERTEX_ORDER if vertex_order else 0))
         h = C.c_void_p()
         _check(lib().mg_walk_batch_create(ctx.h, gp, len(self.graphs), arr, len(self.queries), C.byref(opt),
                                           C.byref(h)), "mg_walk_batch_create")

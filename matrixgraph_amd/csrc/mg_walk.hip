@@ -28,6 +28,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mg_device.hpp"
 #include "mg_internal.hpp"
 
@@ -78,6 +80,7 @@ struct PanelLevel {
   const uint32_t* off;
   const uint32_t* col;
   const uint32_t* row;      // per A^T tile: its strip (row index)
+  const uint32_t* orig;     // per tile of the engine's index: its position in `data` (null: same)
   const uint64_t* data;
   const uint64_t* rmask;    // per A^T tile: bit r = row r non-zero
   const uint64_t* payload;  // per A^T tile: compact bit list (see tile_payload_kernel)
@@ -190,12 +193,18 @@ __global__ void init_sources_kernel(const SourceItem* __restrict__ items, uint32
 }
 
 // Row-occupancy mask of every A^T tile: bit r set when word r (target row r) is non-zero.
-__global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+// Tile i of the engine's index is tile orig[i] of the matrix (orig null: the same tile).
+__device__ inline uint64_t data_tile(const uint32_t* __restrict__ orig, uint32_t i) {
+  return static_cast<uint64_t>(orig ? orig[i] : i) * kTile;
+}
+
+__global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __restrict__ data,
+                                                          const uint32_t* __restrict__ orig, uint32_t n_tiles,
                                                           uint64_t* __restrict__ mask) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * kWaves;
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
-    const uint64_t m = ballot(__builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane) != 0);
+    const uint64_t m = ballot(__builtin_nontemporal_load(data + data_tile(orig, i) + lane) != 0);
     if (lane == 0) mask[i] = m;
   }
 }
@@ -204,12 +213,13 @@ __global__ void __launch_bounds__(kBlock) row_mask_kernel(const uint64_t* __rest
 // once dense_compact_kernel has run, the index of its copy in the dense-row array in bits
 // 13..44; it is read row by row through its row-occupancy mask.
 
-__global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+__global__ void __launch_bounds__(kBlock) tile_payload_kernel(const uint64_t* __restrict__ data,
+                                                              const uint32_t* __restrict__ orig, uint32_t n_tiles,
                                                               uint64_t* __restrict__ payload) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * kWaves;
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
-    const uint64_t word = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
+    const uint64_t word = __builtin_nontemporal_load(data + data_tile(orig, i) + lane);
     const uint64_t pl = tile_payload(word);
     if (lane == 0) payload[i] = pl;
   }
@@ -224,7 +234,8 @@ __global__ void __launch_bounds__(kBlock) dense_flag_kernel(const uint64_t* __re
 // Copies the rows of every dense tile to its slot pos[i] of a compact array (64 words per
 // tile) and records the slot in the payload: the hop kernels then read dense rows from a
 // few hundred MB instead of the whole tile store.
-__global__ void __launch_bounds__(kBlock) dense_compact_kernel(const uint64_t* __restrict__ data, uint32_t n,
+__global__ void __launch_bounds__(kBlock) dense_compact_kernel(const uint64_t* __restrict__ data,
+                                                               const uint32_t* __restrict__ orig, uint32_t n,
                                                                const uint32_t* __restrict__ pos,
                                                                uint64_t* __restrict__ payload,
                                                                uint64_t* __restrict__ dense) {
@@ -234,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) dense_compact_kernel(const uint64_t* _
     const uint64_t pl = payload[i];
     if (!(pl & kPayloadDense)) continue;
     const uint64_t slot = pos[i];
-    dense[slot * kTile + lane] = __builtin_nontemporal_load(data + static_cast<uint64_t>(i) * kTile + lane);
+    dense[slot * kTile + lane] = __builtin_nontemporal_load(data + data_tile(orig, i) + lane);
     if (lane == 0) payload[i] = (pl & ~(0xFFFFFFFFull << 13)) | (slot << 13);
   }
 }
@@ -457,6 +468,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
   const gptr<const uint32_t> __restrict__ off = as_global(L.off);
   const gptr<const uint32_t> __restrict__ col = as_global(L.col);
   const gptr<const uint64_t> __restrict__ data = as_global(L.data);
+  const uint32_t* const orig = L.orig;
   const gptr<const uint64_t> __restrict__ rmask = as_global(L.rmask);
   const gptr<const uint64_t> __restrict__ x_in = as_global(L.x_in);
   const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
@@ -493,6 +505,7 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
       const uint32_t p = g0 + kNumWaves * lane;
       const bool valid = p < seg1;
       const uint32_t t_l = valid ? col[p] : 0;
+      const uint32_t o_l = valid ? (orig ? as_global(orig)[p] : p) : 0;
       const uint64_t rm_l = valid ? rmask[p] : 0;
       const uint64_t vm_l = valid ? v_in[t_l] : 0;
       const uint64_t sm_l = valid ? (s_in[t_l] & alive_in) : 0;
@@ -521,9 +534,8 @@ __global__ void __launch_bounds__(kNumBlock, MG_NUMERIC_MIN_WAVES_PER_SIMD) walk
           b[j] = 0;
           if (j < cnt_b) {
             const uint64_t rm = read_lane(rm_l, ks[j]);
-            if ((rm >> lane) & 1)
-              b[j] = __builtin_nontemporal_load(data + static_cast<uint64_t>(g0 + kNumWaves * static_cast<uint32_t>(ks[j])) *
-                                                           kTile + lane);
+            const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(o_l), ks[j]));
+            if ((rm >> lane) & 1) b[j] = __builtin_nontemporal_load(data + static_cast<uint64_t>(o) * kTile + lane);
           }
         }
 #pragma unroll
@@ -1449,8 +1461,15 @@ __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel*
 }
 
 // Digest of words [w0, w0+n_words) of a vertex-major frontier (word index relative to w0).
+// Row of X that holds vertex v: strips are stored in the engine's order (map: vertex strip ->
+// engine strip; null keeps the vertex order).
+__device__ inline uint64_t engine_row(const uint32_t* __restrict__ map, uint64_t v) {
+  return map ? static_cast<uint64_t>(map[v >> 6]) * 64 + (v & 63) : v;
+}
+
 __global__ void __launch_bounds__(kBlock) frontier_digest_kernel(const uint64_t* __restrict__ x, uint64_t n_vertices,
                                                                  uint32_t W, uint32_t w0, uint32_t n_words,
+                                                                 const uint32_t* __restrict__ map,
                                                                  unsigned long long* __restrict__ out) {
   uint64_t acc = 0;
   const uint64_t total = n_vertices * n_words;
@@ -1458,21 +1477,23 @@ __global__ void __launch_bounds__(kBlock) frontier_digest_kernel(const uint64_t*
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
     const uint64_t v = i / n_words;
     const uint32_t w = static_cast<uint32_t>(i % n_words);
-    const uint64_t val = x[v * W + w0 + w];
+    const uint64_t val = x[engine_row(map, v) * W + w0 + w];
     if (val) acc += digest_term(v * 64 + w, val);
   }
   acc = wave_sum(acc);
   if (lane_id() == 0 && acc) atomicAdd(out, static_cast<unsigned long long>(acc));
 }
 
-// Copy words [w0, w0+n_words) of every vertex into a compact (n_vertices x n_words) array.
+// Copy words [w0, w0+n_words) of every vertex, in vertex order, into a compact
+// (n_vertices x n_words) array.
 __global__ void __launch_bounds__(kBlock) frontier_slice_kernel(const uint64_t* __restrict__ x, uint64_t n_vertices,
                                                                 uint32_t W, uint32_t w0, uint32_t n_words,
+                                                                const uint32_t* __restrict__ map,
                                                                 uint64_t* __restrict__ out) {
   const uint64_t total = n_vertices * n_words;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride)
-    out[i] = x[(i / n_words) * W + w0 + (i % n_words)];
+    out[i] = x[engine_row(map, i / n_words) * W + w0 + (i % n_words)];
 }
 
 // Set bits of each chunk's A^T tiles (from the tile payloads): those of the chunk's panel
@@ -1502,12 +1523,13 @@ __global__ void __launch_bounds__(kBlock) chunk_bits_kernel(const Chunk* __restr
 __global__ void __launch_bounds__(kBlock) export_tiles_kernel(const uint32_t* __restrict__ row_idx,
                                                               const uint32_t* __restrict__ col_idx, uint32_t n_tiles,
                                                               const uint64_t* __restrict__ x, uint32_t W, uint32_t w0,
+                                                              const uint32_t* __restrict__ map,
                                                               uint64_t* __restrict__ out) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * kWaves;
   for (uint32_t i = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
     const uint32_t xr = row_idx[i], y = col_idx[i];
-    const uint64_t w = x[(static_cast<uint64_t>(y) * 64 + lane) * W + w0 + xr];
+    const uint64_t w = x[engine_row(map, static_cast<uint64_t>(y) * 64 + lane) * W + w0 + xr];
     uint64_t mine = 0;
     for (int c = 0; c < 64; ++c) {
       const uint64_t m = ballot((w >> c) & 1);
@@ -1547,9 +1569,30 @@ std::vector<Chunk> build_chunks(const std::vector<uint32_t>& off, uint32_t n_str
 }
 
 // Per-graph scheduler data, built once from the A^T row pointer.
+//
+// Engine order of the vertices: the walk stores strip y of every frontier at position
+// map[y] (a fixed pseudo-random permutation of the strips; vertex v = 64y + r lives in row
+// 64 map[y] + r) and walks A^T with its strips and tile columns relabelled the same way, so
+// the hop kernels see an isomorphic graph. R-MAT vertex ids are biased bit by bit (hubs have
+// few one bits), and the frontier rows of hubs, gathered most, would otherwise crowd onto
+// the same memory channels: gathering the R-MAT-22 hop's 65 M rows takes 4.79 ms in vertex
+// order and 4.10 ms with strips permuted (3.64 ms at four times the grid). Sources are
+// mapped on the way in, frontiers, digests and exports on the way out; counts are unchanged.
 struct GraphPlan {
   const mg_tiled_matrix* at = nullptr;
   uint32_t n_strips = 0;
+  // map[y] = engine strip of vertex strip y; empty when the vertex order is kept.
+  std::vector<uint32_t> map;
+  DevBuf<uint32_t> d_map;
+  // A^T's index in engine order (r_*: owned copies when relabelled); orig[i] = position of
+  // engine tile i in at->data (null when the order is kept).
+  DevBuf<uint32_t> r_off, r_col, r_row, r_orig;
+  const uint32_t* off = nullptr;
+  const uint32_t* col = nullptr;
+  const uint32_t* row = nullptr;
+  const uint32_t* orig = nullptr;
+  uint32_t engine_vertex(uint32_t v) const { return map.empty() ? v : map[v >> 6] * 64u + (v & 63u); }
+  const uint32_t* device_map() const { return map.empty() ? nullptr : d_map.get(); }
   std::vector<Chunk> chunks;
   std::vector<uint64_t> chunk_cost;  // launch-order key per chunk (chunk_cost_of)
   std::vector<uint32_t> split;
@@ -1617,7 +1660,7 @@ void build_job_list(GraphPlan& g, hipStream_t st) {
     tile_bits_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.payload.get(), n, bits.get());
     MG_LAUNCH_CHECK();
     exclusive_scan_u32_on(st, bits.get(), g.job_off.get(), n);
-    job_fill_kernel<<<grid_for(n, kWaves * 64, 65536), kBlock, 0, st>>>(g.payload.get(), g.at->col_idx.get(),
+    job_fill_kernel<<<grid_for(n, kWaves * 64, 65536), kBlock, 0, st>>>(g.payload.get(), g.col,
                                                                         g.dense.get(), g.job_off.get(), n,
                                                                         g.jobs.get());
     MG_LAUNCH_CHECK();
@@ -1629,24 +1672,110 @@ void build_job_list(GraphPlan& g, hipStream_t st) {
   g.has_jobs = true;
 }
 
+__global__ void __launch_bounds__(kBlock) relabel_keys_kernel(const uint32_t* __restrict__ row,
+                                                              const uint32_t* __restrict__ col,
+                                                              const uint32_t* __restrict__ map, uint32_t n,
+                                                              uint64_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ vals) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    keys[i] = (static_cast<uint64_t>(map[row[i]]) << 32) | map[col[i]];
+    vals[i] = i;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) split_keys_kernel(const uint64_t* __restrict__ keys, uint32_t n,
+                                                            uint32_t* __restrict__ row, uint32_t* __restrict__ col) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    row[i] = static_cast<uint32_t>(keys[i] >> 32);
+    col[i] = static_cast<uint32_t>(keys[i]);
+  }
+}
+
+// The engine's strip permutation for n strips: a Fisher-Yates shuffle driven by SplitMix64
+// with a fixed key, so a graph always gets the same order.
+std::vector<uint32_t> strip_permutation(uint32_t n) {
+  std::vector<uint32_t> p(n);
+  std::iota(p.begin(), p.end(), 0u);
+  const uint64_t key = stream_key(0x5354524950ull /* "STRIP" */, n);
+  for (uint32_t i = n; i > 1; --i) {
+    const uint32_t j = static_cast<uint32_t>((sm64_at(key, i) >> 11) % i);
+    std::swap(p[i - 1], p[j]);
+  }
+  return p;
+}
+
+// A^T's index in engine order: tile (y, t) becomes (map[y], map[t]); the tiles are sorted by
+// the new (row, column) (radix sort of 64-bit keys with the old positions as values). The
+// row pointer follows from the old one on the host. The tile data is not moved.
+void relabel_index(GraphPlan& g, const std::vector<uint32_t>& off, std::vector<uint32_t>* r_off_host,
+                   hipStream_t st) {
+  const mg_tiled_matrix* at = g.at;
+  const uint32_t ns = at->n_strips, n = at->n_nz_tile;
+  g.map = strip_permutation(ns);
+  std::vector<uint32_t> inv(ns);
+  for (uint32_t y = 0; y < ns; ++y) inv[g.map[y]] = y;
+  r_off_host->assign(ns + 1ull, 0);
+  for (uint32_t yp = 0; yp < ns; ++yp) (*r_off_host)[yp + 1] = (*r_off_host)[yp] + (off[inv[yp] + 1] - off[inv[yp]]);
+  g.d_map.alloc(std::max<uint32_t>(1, ns));
+  if (ns) MG_HIP(hipMemcpyAsync(g.d_map.get(), g.map.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, st));
+  g.r_off.alloc(ns + 1ull);
+  MG_HIP(hipMemcpyAsync(g.r_off.get(), r_off_host->data(), sizeof(uint32_t) * (ns + 1ull), hipMemcpyHostToDevice,
+                        st));
+  g.r_col.alloc(std::max<uint32_t>(1, n));
+  g.r_row.alloc(std::max<uint32_t>(1, n));
+  g.r_orig.alloc(std::max<uint32_t>(1, n));
+  if (n) {
+    MG_REQUIRE(n < (1u << 31), "walk: more than 2^31 tiles");
+    DevBuf<uint64_t> keys(n), keys_out(n);
+    DevBuf<uint32_t> vals(n);
+    relabel_keys_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(at->row_idx.get(), at->col_idx.get(), g.d_map.get(), n,
+                                                              keys.get(), vals.get());
+    MG_LAUNCH_CHECK();
+    int bits = 1;
+    while ((1ull << bits) < ns) ++bits;
+    size_t temp_bytes = 0;
+    MG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.get(), keys_out.get(), vals.get(),
+                                              g.r_orig.get(), static_cast<int>(n), 0, 32 + bits, st));
+    DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
+    MG_HIP(hipcub::DeviceRadixSort::SortPairs(temp.get(), temp_bytes, keys.get(), keys_out.get(), vals.get(),
+                                              g.r_orig.get(), static_cast<int>(n), 0, 32 + bits, st));
+    split_keys_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(keys_out.get(), n, g.r_row.get(), g.r_col.get());
+    MG_LAUNCH_CHECK();
+    MG_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+  }
+  g.off = g.r_off.get();
+  g.col = g.r_col.get();
+  g.row = g.r_row.get();
+  g.orig = g.r_orig.get();
+}
+
 // Every launch and copy is ordered on `st`, the caller's context stream, so a matrix still
 // being filled by earlier asynchronous work on that stream is complete before it is read.
-GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_t st) {
+// relabel: walk the graph in the engine's strip order (see GraphPlan).
+GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_t st, bool relabel) {
   GraphPlan g;
   g.at = at;
   g.n_strips = at->n_strips;
   std::vector<uint32_t> off(at->n_strips + 1ull);
   MG_HIP(hipMemcpyAsync(off.data(), at->offset_row.get(), sizeof(uint32_t) * off.size(), hipMemcpyDeviceToHost, st));
   MG_HIP(hipStreamSynchronize(st));
+  g.off = at->offset_row.get();
+  g.col = at->col_idx.get();
+  g.row = at->row_idx.get();
+  if (relabel && at->n_strips > 1) {
+    std::vector<uint32_t> r_off;
+    relabel_index(g, off, &r_off, st);
+    off.swap(r_off);
+  }
   g.chunks = build_chunks(off, at->n_strips, chunk_tiles, &g.split);
   g.rmask.alloc(std::max<uint32_t>(1, at->n_nz_tile));
   g.payload.alloc(std::max<uint32_t>(1, at->n_nz_tile));
   if (at->n_nz_tile) {
-    row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), at->n_nz_tile,
-                                                                               g.rmask.get());
+    row_mask_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), g.orig,
+                                                                               at->n_nz_tile, g.rmask.get());
     MG_LAUNCH_CHECK();
-    tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), at->n_nz_tile,
-                                                                                   g.payload.get());
+    tile_payload_kernel<<<grid_for(at->n_nz_tile, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), g.orig,
+                                                                                   at->n_nz_tile, g.payload.get());
     MG_LAUNCH_CHECK();
     const uint32_t n = at->n_nz_tile;
     DevBuf<uint32_t> flag(n), pos(n + 1ull);
@@ -1658,7 +1787,7 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_
     MG_HIP(hipStreamSynchronize(st));
     g.dense.alloc(std::max<uint64_t>(1, static_cast<uint64_t>(n_dense) * kTile));
     if (n_dense) {
-      dense_compact_kernel<<<grid_for(n, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), n, pos.get(),
+      dense_compact_kernel<<<grid_for(n, kWaves, 65536), kBlock, 0, st>>>(at->data.get(), g.orig, n, pos.get(),
                                                                           g.payload.get(), g.dense.get());
       MG_LAUNCH_CHECK();
     }
@@ -1703,7 +1832,7 @@ void build_column_index(GraphPlan& g, hipStream_t st) {
   cursor.alloc(std::max<uint32_t>(1, ns));
   MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * std::max<uint32_t>(1, ns), st));
   if (n) {
-    column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(at->col_idx.get(), n, cursor.get());
+    column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.col, n, cursor.get());
     MG_LAUNCH_CHECK();
   }
   std::vector<uint32_t> count(ns);
@@ -1717,10 +1846,9 @@ void build_column_index(GraphPlan& g, hipStream_t st) {
   g.csc_row.alloc(std::max<uint32_t>(1, n));
   g.csc_payload.alloc(std::max<uint32_t>(1, n));
   if (n) {
-    column_scatter_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(at->col_idx.get(), n, cursor.get(),
-                                                                  g.csc_tile.get());
+    column_scatter_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.col, n, cursor.get(), g.csc_tile.get());
     MG_LAUNCH_CHECK();
-    column_copy_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.csc_tile.get(), n, at->row_idx.get(),
+    column_copy_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.csc_tile.get(), n, g.row,
                                                                g.payload.get(), g.csc_row.get(), g.csc_payload.get());
     MG_LAUNCH_CHECK();
   }
@@ -1835,9 +1963,10 @@ void build_engine(Engine* e, hipStream_t st) {
       if (k >= p.hops) continue;
       const GraphPlan& g = e->graphs[p.graph];
       PanelLevel& L = lv[pi];
-      L.off = g.at->offset_row.get();
-      L.col = g.at->col_idx.get();
-      L.row = g.at->row_idx.get();
+      L.off = g.off;
+      L.col = g.col;
+      L.row = g.row;
+      L.orig = g.orig;
       L.data = g.at->data.get();
       L.rmask = g.rmask.get();
       L.payload = g.payload.get();
@@ -1961,6 +2090,14 @@ void apply_seg(Engine* e, uint32_t G, hipStream_t st) {
     MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * lv.size(), hipMemcpyHostToDevice, st));
   }
   MG_HIP(hipStreamSynchronize(st));
+}
+
+// Whether walks and batches use the engine's strip order (GraphPlan): yes unless the caller's
+// flag asks for the vertex order, or MG_WALK_RELABEL=0 is set in the environment.
+bool use_relabel(bool identity_flag) {
+  if (identity_flag) return false;
+  const char* env = std::getenv("MG_WALK_RELABEL");
+  return !(env && std::atoi(env) == 0);
 }
 
 // Default hop kernel of 64-word levels (kDefaultSegments); MG_WALK_SEGMENTS (0, 1, 2, 4 or 8)
@@ -2209,7 +2346,8 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     *out = nullptr;
     MG_REQUIRE(max_sources >= 1 && max_sources <= 4096, "mg_walk_create: max_sources must be in [1, 4096]");
     MG_REQUIRE(max_hops >= 1 && max_hops <= 64, "mg_walk_create: max_hops must be in [1, 64]");
-    MG_REQUIRE((flags & ~(MG_WALK_PRUNE | MG_WALK_SWEEP_FIRST_HOP)) == 0, "mg_walk_create: unknown flags");
+    MG_REQUIRE((flags & ~(MG_WALK_PRUNE | MG_WALK_SWEEP_FIRST_HOP | MG_WALK_VERTEX_ORDER)) == 0,
+               "mg_walk_create: unknown flags");
     DeviceGuard g(ctx->device);
     auto w = std::make_unique<mg_walk>();
     w->device = ctx->device;
@@ -2218,7 +2356,7 @@ extern "C" int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32
     Engine& e = w->e;
     e.W = words_for(max_sources);
     e.prune = (flags & MG_WALK_PRUNE) ? 1 : 0;
-    e.graphs.push_back(plan_graph(at, kDefaultChunkTiles, ctx->stream));
+    e.graphs.push_back(plan_graph(at, kDefaultChunkTiles, ctx->stream, use_relabel(flags & MG_WALK_VERTEX_ORDER)));
     Panel p;
     p.graph = 0;
     p.hops = max_hops;
@@ -2295,7 +2433,8 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     // The previous run's uploads read the staging buffers asynchronously.
     MG_HIP(hipEventSynchronize(w->staging_read));
     SourceItem* stg = w->staging.get();
-    for (uint32_t i = 0; i < n_sources; ++i) stg[i] = {0, i, sources[i]};
+    const GraphPlan& gp = w->e.graphs[0];
+    for (uint32_t i = 0; i < n_sources; ++i) stg[i] = {0, i, gp.engine_vertex(sources[i])};
     MG_HIP(hipMemcpyAsync(w->e.sources.get(), stg, sizeof(SourceItem) * n_sources, hipMemcpyHostToDevice, st));
     // First hop: push from the source strips when their A^T columns hold at most half of the
     // tiles (the sweep reads every tile once; the push reads the source columns).
@@ -2303,7 +2442,7 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     bool push = false;
     if (w->max_push_items) {
       std::vector<uint32_t> strips(n_sources);
-      for (uint32_t i = 0; i < n_sources; ++i) strips[i] = sources[i] / 64;
+      for (uint32_t i = 0; i < n_sources; ++i) strips[i] = gp.engine_vertex(sources[i]) / 64;
       std::sort(strips.begin(), strips.end());
       strips.erase(std::unique(strips.begin(), strips.end()), strips.end());
       const GraphPlan& g = w->e.graphs[0];
@@ -2406,19 +2545,30 @@ extern "C" int mg_walk_frontier_download(mg_context* ctx, const mg_walk* w, uint
     MG_REQUIRE(hop <= w->hops, "mg_walk_frontier_download: hop out of range");
     DeviceGuard g(ctx->device);
     const auto& x = w->e.panels[0].x[hop];
-    const uint64_t words = static_cast<uint64_t>(w->e.panels[0].n_strips) * 64 * w->e.W;
-    MG_HIP(hipMemcpyAsync(xt, x.get(), words * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    const uint64_t nv = static_cast<uint64_t>(w->e.panels[0].n_strips) * 64;
+    const uint64_t words = nv * w->e.W;
+    const uint32_t* map = w->e.graphs[0].device_map();
+    if (!map) {
+      MG_HIP(hipMemcpyAsync(xt, x.get(), words * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+      DevBuf<uint64_t> tmp(words);  // X_hop in vertex order
+      frontier_slice_kernel<<<grid_for(words, kBlock * 8, 16384), kBlock, 0, ctx->stream>>>(x.get(), nv, w->e.W, 0,
+                                                                                           w->e.W, map, tmp.get());
+      MG_LAUNCH_CHECK();
+      MG_HIP(hipMemcpyAsync(xt, tmp.get(), words * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    }
     MG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 
 namespace {
-uint64_t frontier_digest(mg_context* ctx, const uint64_t* x, uint64_t nv, uint32_t W, uint32_t w0, uint32_t n_words) {
+uint64_t frontier_digest(mg_context* ctx, const uint64_t* x, uint64_t nv, uint32_t W, uint32_t w0, uint32_t n_words,
+                         const uint32_t* map) {
   unsigned long long* d = ctx->counters.get();
   MG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream));
   if (n_words) {
     frontier_digest_kernel<<<grid_for(nv * n_words, kBlock * 16, 16384), kBlock, 0, ctx->stream>>>(x, nv, W, w0,
-                                                                                                   n_words, d);
+                                                                                                   n_words, map, d);
     MG_LAUNCH_CHECK();
   }
   unsigned long long h = 0;
@@ -2427,11 +2577,13 @@ uint64_t frontier_digest(mg_context* ctx, const uint64_t* x, uint64_t nv, uint32
   return h;
 }
 
+// g: the plan of the frontier's graph (its strip order).
 mg_tiled_matrix* export_frontier(mg_context* ctx, const uint64_t* x, const uint64_t* s_dev, uint32_t ns, uint32_t W,
-                                 uint32_t w0, uint32_t rows) {
-  std::vector<uint64_t> s(ns);
-  MG_HIP(hipMemcpyAsync(s.data(), s_dev, sizeof(uint64_t) * ns, hipMemcpyDeviceToHost, ctx->stream));
+                                 uint32_t w0, uint32_t rows, const GraphPlan& g) {
+  std::vector<uint64_t> s_eng(ns), s(ns);
+  MG_HIP(hipMemcpyAsync(s_eng.data(), s_dev, sizeof(uint64_t) * ns, hipMemcpyDeviceToHost, ctx->stream));
   MG_HIP(hipStreamSynchronize(ctx->stream));
+  for (uint32_t y = 0; y < ns; ++y) s[y] = s_eng[g.map.empty() ? y : g.map[y]];  // vertex strip order
   // Tile index in row-major (x, y) order over the padded-square grid.
   std::vector<uint32_t> off(ns + 1ull, 0), ri, ci;
   for (uint32_t xr = 0; xr < rows; ++xr) {
@@ -2453,7 +2605,8 @@ mg_tiled_matrix* export_frontier(mg_context* ctx, const uint64_t* x, const uint6
     MG_HIP(hipMemcpyAsync(m->row_idx.get(), ri.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
     MG_HIP(hipMemcpyAsync(m->col_idx.get(), ci.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice, ctx->stream));
     export_tiles_kernel<<<grid_for(nt, kWaves, 65536), kBlock, 0, ctx->stream>>>(m->row_idx.get(), m->col_idx.get(),
-                                                                                 nt, x, W, w0, m->data.get());
+                                                                                 nt, x, W, w0, g.device_map(),
+                                                                                 m->data.get());
     MG_LAUNCH_CHECK();
   }
   MG_HIP(hipStreamSynchronize(ctx->stream));
@@ -2468,7 +2621,7 @@ extern "C" int mg_walk_frontier_digest(mg_context* ctx, const mg_walk* w, uint32
     DeviceGuard g(ctx->device);
     const Panel& p = w->e.panels[0];
     *digest = frontier_digest(ctx, p.x[hop].get(), static_cast<uint64_t>(p.n_strips) * 64, w->e.W, 0,
-                              (w->n_sources + 63) / 64);
+                              (w->n_sources + 63) / 64, w->e.graphs[0].device_map());
   });
 }
 
@@ -2479,7 +2632,8 @@ extern "C" int mg_walk_export(mg_context* ctx, const mg_walk* w, uint32_t hop, m
     MG_REQUIRE(hop <= w->hops, "mg_walk_export: hop out of range");
     DeviceGuard g(ctx->device);
     const Panel& p = w->e.panels[0];
-    *out = export_frontier(ctx, p.x[hop].get(), p.s[hop].get(), p.n_strips, w->e.W, 0, (w->n_sources + 63) / 64);
+    *out = export_frontier(ctx, p.x[hop].get(), p.s[hop].get(), p.n_strips, w->e.W, 0, (w->n_sources + 63) / 64,
+                           w->e.graphs[0]);
   });
 }
 
@@ -2541,7 +2695,8 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     const uint32_t packing = opt ? opt->packing : MG_PACK_BY_HOPS;
     const uint32_t bflags = opt ? opt->flags : 0u;
     MG_REQUIRE(packing <= MG_PACK_ONE_PER_PANEL, "mg_walk_batch_create: unknown packing");
-    MG_REQUIRE((bflags & ~MG_BATCH_SWEEP_FIRST_HOP) == 0, "mg_walk_batch_create: unknown flags");
+    MG_REQUIRE((bflags & ~(MG_BATCH_SWEEP_FIRST_HOP | MG_BATCH_VERTEX_ORDER)) == 0,
+               "mg_walk_batch_create: unknown flags");
     DeviceGuard g(ctx->device);
     auto b = std::make_unique<mg_walk_batch>();
     b->device = ctx->device;
@@ -2550,7 +2705,7 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     e.W = 64;
     for (uint32_t gi = 0; gi < n_graphs; ++gi) {
       MG_REQUIRE(graphs_at[gi] != nullptr, "mg_walk_batch_create: NULL graph");
-      e.graphs.push_back(plan_graph(graphs_at[gi], chunk_tiles, ctx->stream));
+      e.graphs.push_back(plan_graph(graphs_at[gi], chunk_tiles, ctx->stream, use_relabel(bflags & MG_BATCH_VERTEX_ORDER)));
     }
     // Validate queries.
     b->queries.resize(n_queries);
@@ -2628,7 +2783,8 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
       std::vector<std::vector<uint32_t>> strips(e.panels.size());
       for (uint32_t q = 0; q < n_queries; ++q) {
         const QueryPlace& P = b->queries[q];
-        for (uint32_t i = 0; i < P.n_sources; ++i) strips[P.panel].push_back(queries[q].sources[i] / 64);
+        const GraphPlan& gq = e.graphs[P.graph];
+        for (uint32_t i = 0; i < P.n_sources; ++i) strips[P.panel].push_back(gq.engine_vertex(queries[q].sources[i]) / 64);
       }
       std::vector<uint8_t> indexed(e.graphs.size(), 0);
       e.push0.assign(e.panels.size(), 0);
@@ -2666,7 +2822,9 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
     std::vector<SourceItem> items;
     for (uint32_t q = 0; q < n_queries; ++q) {
       const QueryPlace& P = b->queries[q];
-      for (uint32_t i = 0; i < P.n_sources; ++i) items.push_back({P.panel, P.w0 * 64 + i, queries[q].sources[i]});
+      const GraphPlan& gq = e.graphs[P.graph];
+      for (uint32_t i = 0; i < P.n_sources; ++i)
+        items.push_back({P.panel, P.w0 * 64 + i, gq.engine_vertex(queries[q].sources[i])});
     }
     e.n_sources = static_cast<uint32_t>(items.size());
     e.sources.alloc(items.size());
@@ -2771,7 +2929,8 @@ extern "C" int mg_walk_batch_query_digest(mg_context* ctx, const mg_walk_batch* 
     DeviceGuard g(ctx->device);
     const QueryPlace& P = b->queries[q];
     const Panel& p = b->e.panels[P.panel];
-    *digest = frontier_digest(ctx, p.x[P.hops].get(), static_cast<uint64_t>(p.n_strips) * 64, b->e.W, P.w0, P.n_words);
+    *digest = frontier_digest(ctx, p.x[P.hops].get(), static_cast<uint64_t>(p.n_strips) * 64, b->e.W, P.w0, P.n_words,
+                              b->e.graphs[P.graph].device_map());
   });
 }
 
@@ -2785,7 +2944,7 @@ extern "C" int mg_walk_batch_query_download(mg_context* ctx, const mg_walk_batch
     const uint64_t nv = static_cast<uint64_t>(p.n_strips) * 64;
     DevBuf<uint64_t> tmp(nv * P.n_words);
     frontier_slice_kernel<<<grid_for(nv * P.n_words, kBlock * 8, 16384), kBlock, 0, ctx->stream>>>(
-        p.x[P.hops].get(), nv, b->e.W, P.w0, P.n_words, tmp.get());
+        p.x[P.hops].get(), nv, b->e.W, P.w0, P.n_words, b->e.graphs[P.graph].device_map(), tmp.get());
     MG_LAUNCH_CHECK();
     MG_HIP(hipMemcpyAsync(out, tmp.get(), tmp.bytes(), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
@@ -2800,7 +2959,8 @@ extern "C" int mg_walk_batch_query_export(mg_context* ctx, const mg_walk_batch* 
     DeviceGuard g(ctx->device);
     const QueryPlace& P = b->queries[q];
     const Panel& p = b->e.panels[P.panel];
-    *out = export_frontier(ctx, p.x[P.hops].get(), p.s[P.hops].get(), p.n_strips, b->e.W, P.w0, P.n_words);
+    *out = export_frontier(ctx, p.x[P.hops].get(), p.s[P.hops].get(), p.n_strips, b->e.W, P.w0, P.n_words,
+                           b->e.graphs[P.graph]);
   });
 }
 

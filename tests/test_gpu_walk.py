@@ -324,3 +324,52 @@ def test_hop_kernel_choice_errors(ctx):
         w.set_hop_kernel(3)
     narrow = M.Walk(ctx, at, 100, 2)
     assert narrow.set_hop_kernel(4) == 0  # fewer than 2049 sources keep the sweep
+
+
+@pytest.mark.parametrize("scale,seed,n_src,hops,flags", [(14, 3, 4096, 4, 0), (13, 8, 300, 3, 0),
+                                                         (14, 5, 4096, 3, 2), (10, 2, 1000, 4, 0)])
+def test_engine_strip_order_equals_vertex_order(ctx, scale, seed, n_src, hops, flags):
+    """The engine's permuted strip order (default) and the vertex order give the same
+    frontiers (downloads), digests, exports and statistics at every hop, for full and narrow
+    frontiers, pushed and swept first hops; the permuted walk also equals the oracle."""
+    M = mg()
+    n = 1 << scale
+    el, at, s, d = setup_graph(ctx, scale, seed)
+    sources = O.sample_sources(n, n_src, seed)
+    wp = M.Walk(ctx, at, n_src, hops, flags)
+    wv = M.Walk(ctx, at, n_src, hops, flags | M.WALK_VERTEX_ORDER)
+    for w in (wp, wv):
+        w.run(sources, hops)
+    xs = O.walk_frontiers(n, s, d, sources, hops, W=wp.words_per_vertex)
+    for k in range(1, hops + 1):
+        fp = wp.frontier(k)
+        assert np.array_equal(fp, wv.frontier(k)), f"hop {k}"
+        assert np.array_equal(fp, xs[k]), f"hop {k} vs oracle"
+        assert wp.frontier_digest(k) == wv.frontier_digest(k) == O.frontier_digest(xs[k], n_src)
+        assert _stats(wp, k) == _stats(wv, k), f"hop {k}"
+        assert wp.hop_stats(k)["kernel"] == wv.hop_stats(k)["kernel"]
+        ep, ev = wp.export(k).download(), wv.export(k).download()
+        assert np.array_equal(ep.offset_row, ev.offset_row) and np.array_equal(ep.col_idx, ev.col_idx)
+        assert np.array_equal(ep.data, ev.data)
+
+
+def test_batch_engine_strip_order_equals_vertex_order(ctx):
+    M = mg()
+    from matrixgraph_amd import workloads as WL
+
+    specs = [WL.GraphSpec("rmat", 13, 61), WL.GraphSpec("er", 12, 62), WL.GraphSpec("rmat", 12, 63)]
+    graphs = [WL.build_graph(ctx, sp)[0] for sp in specs]
+    qs = WL.batch_queries(specs, 40, 21, max_sources=2000, max_hops=5)
+    bp = M.WalkBatch(ctx, graphs, qs)
+    bv = M.WalkBatch(ctx, graphs, qs, vertex_order=True)
+    for b in (bp, bv):
+        b.run()
+        b.run()
+    ctx.synchronize()
+    assert bp.totals() == bv.totals()
+    for i in range(len(qs)):
+        assert bp.query_digest(i) == bv.query_digest(i), i
+        if i % 4 == 0:
+            assert np.array_equal(bp.query_frontier(i), bv.query_frontier(i)), i
+            ep, ev = bp.query_export(i).download(), bv.query_export(i).download()
+            assert np.array_equal(ep.col_idx, ev.col_idx) and np.array_equal(ep.data, ev.data), i
