@@ -97,6 +97,8 @@ def cpu_baseline(M, O, ctx, w, at, n_edges, hops, n_src, stride, threads, reps=1
         "value": n_src / est_total,
         "unit": "source-walks/s",
         "cores": threads,
+        "host_cpus": os.cpu_count(),
+        "nproc": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": (f"oracle tile algorithm (OpenMP, {threads} threads) on every output strip of each of the {hops} "
                    f"hops (the whole walk), {reps} times, {sample_s:.1f}s measured; every output row checked bit-exact "
@@ -349,6 +351,8 @@ def main():
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
     ap.add_argument("--sweep-first-hop", action="store_true",
                     help="sweep every strip at the first hop too (no push from the source strips)")
+    ap.add_argument("--vertex-order", action="store_true",
+                    help="keep frontier strips in vertex order (no internal strip permutation)")
     ap.add_argument("--two-hop", type=int, default=0, metavar="SCALE",
                     help="only time the two-hop pair enumeration (mg_edges_two_hop) on an R-MAT graph of this "
                          "scale (edge factor 16, seed 2), beside the oracle's bitset product on the host")
@@ -364,16 +368,33 @@ def main():
     if args.two_hop:
         two_hop(M, ctx, args.two_hop)
         return
+    ctx.synchronize()
     t0 = time.perf_counter()
     el = M.EdgeList.rmat(ctx, args.scale, args.edge_factor, 0.57, 0.19, 0.19, args.seed)
     n_vertices, n_edges = el.info()
+    ctx.synchronize()
+    t1 = time.perf_counter()
     at = el.to_tiled_matrix(transpose=True)
+    ctx.synchronize()
+    t2 = time.perf_counter()
     el.close()
     ns, n_tiles, _ = at.metadata()
-    log(f"graph: 2^{args.scale} vertices, {n_edges} edges, {n_tiles} A^T tiles, setup {time.perf_counter() - t0:.1f}s")
+    log(f"graph: 2^{args.scale} vertices, {n_edges} edges, {n_tiles} A^T tiles, setup {t2 - t0:.1f}s")
     sources = M.sample_sources(n_vertices, args.sources, args.seed)
-    w = M.Walk(ctx, at, args.sources, args.hops,
-               (M.WALK_PRUNE if args.prune else 0) | (M.WALK_SWEEP_FIRST_HOP if args.sweep_first_hop else 0))
+    flags = (M.WALK_PRUNE if args.prune else 0) | (M.WALK_SWEEP_FIRST_HOP if args.sweep_first_hop else 0)
+    flags |= M.WALK_VERTEX_ORDER if args.vertex_order else 0
+    t3 = time.perf_counter()
+    w = M.Walk(ctx, at, args.sources, args.hops, flags)
+    ctx.synchronize()
+    t4 = time.perf_counter()
+    setup = {
+        "note": "per graph, before the timed walks (not in value)",
+        "edge_list_ms": round((t1 - t0) * 1e3, 1),
+        "tile_build_ms": round((t2 - t1) * 1e3, 1),
+        "walk_create_ms": round((t4 - t3) * 1e3, 1),
+        "walk_create_covers": "A^T index relabelled to the engine's strip order, row masks, compact payloads, "
+                              "dense-tile rows, scheduler chunks and costs, column index for the pushed first hop",
+    }
 
     for _ in range(args.warmup):
         w.run(sources, args.hops)
@@ -390,48 +411,65 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     per_hop = []
     total_tp = 0
-    sum_bytes = 0
-    sum_numeric_ms = 0.0
-    n_launch = 0
+    total_gathers = 0
+    launches = []  # the roofline kernel's launches (hops that sweep every strip)
+    kernel_names = {M.HOP_SWEEP: "sweep", M.HOP_PUSH: "push", M.HOP_JOBS: "jobs"}
     for k in range(1, args.hops + 1):
         st = w.hop_stats(k)
         sym_ms, num_ms, n_runs = w.kernel_times(k)
         avg_num = num_ms / max(n_runs, 1)
         avg_sym = sym_ms / max(n_runs, 1)
         total_tp += st["tile_products"]
-        sweep = st["kernel"] == M.HOP_SWEEP
-        if sweep:  # the roofline kernel: hops that sweep every strip (walk_numeric64_kernel)
-            sum_bytes += st["alg_bytes"]
-            sum_numeric_ms += avg_num
-            n_launch += 1
+        total_gathers += st["edges_visited"]
+        kname = kernel_names[st["kernel"]]
+        if st["kernel"] == M.HOP_SWEEP:
+            launches.append({"hop": k, "alg_bytes": st["alg_bytes_numeric"], "ms": round(avg_num, 4),
+                             "frac": round(st["alg_bytes_numeric"] / (avg_num * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
         per_hop.append({
-            "hop": k, "kernel": "sweep" if sweep else "push", "tile_products": st["tile_products"],
-            "alg_bytes": st["alg_bytes"],
-            "left_tiles": st["left_tiles"], "right_tiles_used": st["right_tiles_used"],
+            "hop": k, "kernel": kname, "tile_products": st["tile_products"],
+            "alg_bytes": st["alg_bytes"], "alg_bytes_numeric": st["alg_bytes_numeric"],
+            "left_tiles": st["left_tiles"], "left_tiles_numeric": st["left_tiles_numeric"],
+            "right_tiles_used": st["right_tiles_used"], "right_tiles_used_numeric": st["right_tiles_used_numeric"],
             "out_tiles": st["out_tiles"], "out_tiles_numeric": st["out_tiles_numeric"],
-            "popcount": st["popcount"], "edges_visited": st["edges_visited"],
+            "popcount": st["popcount"], "gathers": st["edges_visited"],
             "numeric_kernel_ms": round(avg_num, 4), "symbolic_ms": round(avg_sym, 4),
-            "numeric_GBps_alg": round(st["alg_bytes"] / (avg_num * 1e-3) / 1e9, 1) if avg_num > 0 else None,
+            "GBps_alg_numeric": round(st["alg_bytes_numeric"] / (avg_num * 1e-3) / 1e9, 1) if avg_num > 0 else None,
         })
-        log(f"hop {k} ({'sweep' if sweep else 'push'}): TP={st['tile_products']} B_alg={st['alg_bytes'] / 1e9:.2f}GB "
-            f"kernels={avg_num:.3f}ms popcount={st['popcount']}")
-    n_launch = max(n_launch, 1)
-    achieved = (sum_bytes / n_launch) / ((sum_numeric_ms / n_launch) * 1e-3) / 1e9 if sum_numeric_ms > 0 else 0.0
+        log(f"hop {k} ({kname}): TP={st['tile_products']} B_alg={st['alg_bytes'] / 1e9:.2f}GB "
+            f"B_alg(numeric)={st['alg_bytes_numeric'] / 1e9:.2f}GB kernels={avg_num:.3f}ms popcount={st['popcount']}")
+    # The roofline is stated over the steady hops (3 and later), where X_k is numerically full.
+    # At hop 2 the byte model charges every A^T tile in a used column at 520 B although the
+    # kernel reads a 16-B payload per tile, which puts that launch above peak; it is listed in
+    # per_launch but not averaged in.
+    steady = [x for x in launches if x["hop"] >= 3] or launches
+    for x in launches:
+        x["in_roofline"] = x in steady
+    n_launch = max(len(steady), 1)
+    sum_bytes = sum(x["alg_bytes"] for x in steady)
+    sum_ms = sum(x["ms"] for x in steady)
+    achieved = (sum_bytes / n_launch) / ((sum_ms / n_launch) * 1e-3) / 1e9 if sum_ms > 0 else 0.0
     traffic = None
     tfile = ROOT / "profiles" / "walk_numeric_rmat22_traffic.json"
-    if tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune:
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    if (tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune
+            and not args.vertex_order):
+        traffic = json.loads(tfile.read_text()).get("bytes_per_steady_launch")
     roofline = {
         "kernel": "walk_numeric64_kernel",
-        "launches_per_walk": sum(1 for h in per_hop if h["kernel"] == "sweep"),
+        "launches": f"hops {', '.join(str(x['hop']) for x in steady)} of each walk (steady: X_k numerically full)",
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        "traffic_source": ("stored: HBM-side bytes per steady launch (TCC_EA0_RDREQ_32B/64B/128B, WRREQ/WRREQ_64B) "
+                           "from rocprofv3 --pmc runs of this bench, profiles/walk_numeric_rmat22_traffic.md; "
+                           "not measured in this run") if traffic else None,
         "alg_bytes_per_launch": sum_bytes // n_launch,
-        "avg_launch_ms": round(sum_numeric_ms / n_launch, 4),
+        "alg_bytes_definition": "520*(T_L + T_R,used + T_C) + 4*(2*rows + 2) over numerically non-empty tiles "
+                                "(T_L: X_k tiles, T_R,used: A^T tiles in their rows, T_C: X_{k+1} tiles)",
+        "avg_launch_ms": round(sum_ms / n_launch, 4),
+        "per_launch": launches,
     }
 
     result = {
@@ -455,11 +493,17 @@ def main():
             "hops": args.hops,
             "tile_size": 64,
             "structural_mode": "prune" if args.prune else "canonical",
+            "strip_order": "vertex" if args.vertex_order else "engine (fixed permutation)",
             "parallelism": "single-gpu",
         },
         "walk_queries_per_s": round(1.0 / (ms_per_step * 1e-3), 3),
         "tile_products_per_s": round(total_tp / (ms_per_step * 1e-3), 1),
         "tile_products_per_walk": total_tp,
+        "tile_products_note": "structural (r,k,c) triples counted by the symbolic pass; the kernel executes one "
+                              "512-B frontier-row gather per A^T bit whose source is non-empty (gathers)",
+        "gathers_per_walk": total_gathers,
+        "gathers_per_s": round(total_gathers / (ms_per_step * 1e-3), 1),
+        "setup": setup,
         "per_hop": per_hop,
         "roofline": roofline,
         "cpu_baseline": None,

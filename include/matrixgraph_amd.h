@@ -350,6 +350,12 @@ typedef struct mg_hop_stats {
   double numeric_ms;          /* hop kernels (structural + numeric) of the last logged run, else 0 */
   uint32_t kernel;            /* MG_HOP_SWEEP, MG_HOP_PUSH or MG_HOP_JOBS */
   uint32_t segments;          /* MG_HOP_JOBS: row segments of the job-list kernel, else 0 */
+  /* Single walks only (batch query statistics leave them 0): the same counts over
+   * numerically non-empty tiles. */
+  uint64_t left_tiles_numeric;       /* numerically non-empty tiles of X_k */
+  uint64_t right_tiles_used_numeric; /* tiles of A in rows referenced by those tiles */
+  uint64_t alg_bytes_numeric;        /* 520*(left_numeric + right_used_numeric + out_tiles_numeric)
+                                        + 4*(2*rows+2) */
 } mg_hop_stats;
 
 /* Allocates X_0..X_max_hops (W = ceil(max_sources/64) rounded up to a power of two,

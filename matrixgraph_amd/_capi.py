@@ -39,6 +39,9 @@ class HopStats(C.Structure):
         ("numeric_ms", f64),
         ("kernel", u32),
         ("segments", u32),
+        ("left_tiles_numeric", u64),
+        ("right_tiles_used_numeric", u64),
+        ("alg_bytes_numeric", u64),
     ]
 
 

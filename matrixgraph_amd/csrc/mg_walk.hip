@@ -1484,6 +1484,26 @@ __global__ void __launch_bounds__(kBlock) frontier_digest_kernel(const uint64_t*
   if (lane_id() == 0 && acc) atomicAdd(out, static_cast<unsigned long long>(acc));
 }
 
+// Numeric occupancy of a hop's left operand X_k (per-strip numeric masks n_prev, words of
+// live queries `alive`): out[0] += A^T tiles whose column strip holds a numerically non-empty
+// X_k tile (T_R,used counted numerically), out[1] += numerically non-empty X_k tiles.
+__global__ void __launch_bounds__(kBlock) numeric_use_kernel(const uint32_t* __restrict__ col, uint32_t n_tiles,
+                                                             const uint64_t* __restrict__ n_prev, uint32_t n_strips,
+                                                             uint64_t alive, unsigned long long* __restrict__ out) {
+  uint64_t used = 0, left = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_tiles; i += stride)
+    used += (n_prev[col[i]] & alive) != 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_strips; t += stride)
+    left += __popcll(n_prev[t] & alive);
+  used = wave_sum(used);
+  left = wave_sum(left);
+  if (lane_id() == 0) {
+    if (used) atomicAdd(out, static_cast<unsigned long long>(used));
+    if (left) atomicAdd(out + 1, static_cast<unsigned long long>(left));
+  }
+}
+
 // Copy words [w0, w0+n_words) of every vertex, in vertex order, into a compact
 // (n_vertices x n_words) array.
 __global__ void __launch_bounds__(kBlock) frontier_slice_kernel(const uint64_t* __restrict__ x, uint64_t n_vertices,
@@ -2489,6 +2509,24 @@ extern "C" int mg_walk_hop_stats(mg_context* ctx, mg_walk* w, uint32_t hop, mg_h
     MG_HIP(hipMemcpyAsync(c, w->e.ctr(0, hop), sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     MG_HIP(hipStreamSynchronize(ctx->stream));
     hop_stats_from(c, (w->n_sources + 63) / 64, out);
+    {
+      // Numeric occupancy of X_{hop-1}, from its per-strip numeric masks.
+      const GraphPlan& gp = w->e.graphs[0];
+      const Panel& pn = w->e.panels[0];
+      unsigned long long* d = ctx->counters.get();
+      MG_HIP(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), ctx->stream));
+      const uint32_t nt = gp.at->n_nz_tile;
+      numeric_use_kernel<<<grid_for(std::max<uint32_t>(nt, pn.n_strips), kBlock, 4096), kBlock, 0, ctx->stream>>>(
+          gp.col, nt, pn.n[hop - 1].get(), pn.n_strips, words_mask(0, (w->n_sources + 63) / 64), d);
+      MG_LAUNCH_CHECK();
+      unsigned long long h[2] = {0, 0};
+      MG_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+      MG_HIP(hipStreamSynchronize(ctx->stream));
+      out->right_tiles_used_numeric = h[0];
+      out->left_tiles_numeric = h[1];
+      out->alg_bytes_numeric =
+          520ull * (h[0] + h[1] + out->out_tiles_numeric) + 4ull * (2ull * ((w->n_sources + 63) / 64) + 2);
+    }
     if (hop == 1 && w->last_push) {
       out->kernel = MG_HOP_PUSH;
     } else if (w->e.W == 64 && w->e.seg && hop - 1 >= w->e.seg_from) {
