@@ -262,6 +262,104 @@ def learned_section(M, ctx, train_specs, train_graphs, train_edges, chunk_tiles,
     }
 
 
+def prealloc_section(M, ctx, train_specs, train_graphs, n_train, n_eval):
+    """Learned pre-allocation of pruned products: an MLP (ROW_FEATURES -> 64 -> 64 -> 2, the
+    library's kernels) trained on per-row outcomes of pruned product walks over the seeds 10-17
+    graphs predicts each output row's non-empty tiles from its symbolic-pass features; the
+    predicted count sizes mg_multiply_pruned's single numeric pass (exact retry on overflow).
+    Evaluated on the first n_eval queries of the batch over fresh graphs (seeds 20-27, query
+    seed 2), each run as a product walk X_{k+1} = X_k * A with pruned frontiers, against the two
+    model-free ways to size a pruned product (exact: flag pass + write pass; structural: one
+    pass into scratch for every structural tile) and against the reference's structural
+    product (symbolic allocation of every structural tile, zero tiles kept)."""
+    from matrixgraph_amd import learned as LN
+    from matrixgraph_amd import workloads as WL
+
+    t0 = time.perf_counter()
+    train_q = WL.batch_queries(train_specs, n_train, 201)
+    groups = []
+    X, T, pairs = LN.collect_products(ctx, train_graphs, train_q, log=log, groups=groups)
+    mlp = M.MLP(ctx, M.ROW_FEATURES, LN.ROW_TARGETS, seed=7)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(len(X))
+    l0, _ = mlp.loss_grad(X[perm[:65536]], T[perm[:65536]])
+    l1 = mlp.train(X[perm], T[perm], epochs=40, batch=512, lr=2e-3)
+    margin = LN.calibrate_margin(mlp, X, T, groups)
+    t_train = time.perf_counter() - t0
+    log(f"prealloc: trained on {len(X)} rows ({pairs} tile pairs), loss {l0:.5f} -> {l1:.5f}, margin {margin:.3f}, "
+        f"{t_train:.1f}s")
+    specs = WL.batch_graph_specs(20)
+    built = [WL.build_graph(ctx, sp) for sp in specs]
+    graphs = [g for g, _ in built]
+    queries = WL.batch_queries(specs, 512, 2)[:n_eval]
+    modes = {}
+    digests = {}
+    for mode in ("learned", "exact", "structural"):
+        tot = None
+        ds = []
+        for q in queries:
+            d, st = LN.pruned_walk(ctx, graphs[q.graph], q.sources, q.hops, mode, mlp, margin)
+            ds.append(d)
+            if tot is None:
+                tot = dict(st)
+            else:
+                for k, v in st.items():
+                    tot[k] = max(tot[k], v) if k in ("scratch_bytes", "output_bytes") else tot[k] + v
+        digests[mode] = ds
+        modes[mode] = tot
+        log(f"prealloc {mode}: {tot['seconds'] * 1e3:.1f} ms, passes {tot['passes']}, fallbacks {tot['fallbacks']}, "
+            f"peak scratch {tot['scratch_bytes'] / 2**20:.1f} MiB")
+    exact = digests["learned"] == digests["exact"] == digests["structural"]
+    # The reference's sizing: every structural tile allocated (zero tiles kept in X_k).
+    ref_s = 0.0
+    ref_bytes = 0
+    for q in queries:
+        xs, stats = WL.product_walk(ctx, graphs[q.graph], graphs[q.graph].n_strips * 64, q.sources, q.hops, None)
+        ref_s += sum(st["symbolic_ms"] + st["numeric_ms"] for st in stats) * 1e-3
+        ref_bytes = max(ref_bytes, max(x.n_nz_tile for x in xs[1:]) * 512)
+        for x in xs:
+            x.close()
+    for g in graphs:
+        g.close()
+    mlp.close()
+    if not exact:
+        raise SystemExit("prealloc: pruned walks differ between sizing modes")
+    L, E, S = modes["learned"], modes["exact"], modes["structural"]
+    return {
+        "workload": f"first {n_eval} of the 512 queries over 8 fresh graphs (seeds 20-27, query seed 2), each as a "
+                    "product walk X_{k+1} = X_k * A with pruned frontiers",
+        "model": f"MLP {M.ROW_FEATURES}->64->64->{LN.ROW_TARGETS}, BF16 weights, FP32 accumulation, trained on the GPU",
+        "training": {"rows": int(len(X)), "tile_pairs": int(pairs), "queries": n_train,
+                     "source": "per-row outcomes of pruned product walks over the seeds 10-17 graphs, query seed 201",
+                     "loss_initial": round(float(l0), 6), "loss_final": round(float(l1), 6),
+                     "seconds": round(t_train, 1),
+                     "capacity_margin": round(margin, 4),
+                     "margin_rule": "95th percentile over the training products of actual/predicted - 1"},
+        "products": L["products"],
+        "learned_ms": round(L["seconds"] * 1e3, 2),
+        "exact_two_pass_ms": round(E["seconds"] * 1e3, 2),
+        "structural_scratch_ms": round(S["seconds"] * 1e3, 2),
+        "reference_structural_ms": round(ref_s * 1e3, 2),
+        "learned_vs_exact": round(E["seconds"] / L["seconds"], 3),
+        "learned_vs_structural_scratch": round(S["seconds"] / L["seconds"], 3),
+        "numeric_passes": {"learned": L["passes"], "exact": E["passes"], "structural": S["passes"]},
+        "fallbacks": L["fallbacks"],
+        "fallback_frequency": round(L["fallbacks"] / max(L["products"], 1), 4),
+        "rows_predicted": L["rows"],
+        "row_misprediction_rate": round(L["rows_mispredicted"] / max(L["rows"], 1), 4),
+        "misprediction_definition": "predicted/actual non-empty tiles of a row outside [1/2, 2] (+1 smoothing)",
+        "predicted_tiles": L["predicted_tiles"], "numeric_tiles": L["numeric_tiles"],
+        "capacity_tiles": L["capacity_tiles"],
+        "allocated_tile_slots": {"reference_structural": L["structural_tiles"], "learned": L["allocated_tiles"],
+                                 "exact": E["allocated_tiles"], "structural_scratch": S["allocated_tiles"]},
+        "peak_scratch_bytes": {"learned": L["scratch_bytes"], "exact": E["scratch_bytes"],
+                               "structural": S["scratch_bytes"]},
+        "peak_output_bytes_reference_structural": ref_bytes,
+        "peak_output_bytes_pruned": L["output_bytes"],
+        "bit_exact": exact,
+    }
+
+
 def dense_section(M, ctx, thresholds=(4097, 2048, 1024, 512, 0, None)):
     """Density-mixed 3-hop walk: R-MAT scale 20 (edge factor 16, seed 3) union 64 planted
     communities of 4096 vertices (community c = vertices [16384c, 16384c + 4096), p = 0.3),
@@ -346,6 +444,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
     ap.add_argument("--no-dense", action="store_true", help="skip the density-mixed (matrix-core) section")
+    ap.add_argument("--no-prealloc", action="store_true", help="skip the learned pre-allocation section")
+    ap.add_argument("--prealloc-train", type=int, default=384, help="training queries of the pre-allocation model")
+    ap.add_argument("--prealloc-eval", type=int, default=128, help="evaluated queries of the pre-allocation section")
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
@@ -524,6 +625,9 @@ def main():
             result["learned"] = learned_section(M, ctx, specs, graphs, edges, best["chunk_tiles"], args.batch_queries,
                                                 args.batch_reps)
             result["learned_batched_vs_plain"] = result["learned"]["learned_batched_vs_plain"]
+        if not args.no_prealloc:
+            result["learned_prealloc"] = prealloc_section(M, ctx, specs, graphs, args.prealloc_train,
+                                                          args.prealloc_eval)
     print(json.dumps(result), flush=True)
 
 

@@ -247,6 +247,40 @@ int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tile
  * No counterpart in the reference (its product is VALU only). */
 int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                               mg_tiled_matrix* c, uint32_t min_bits, uint64_t* dense_products);
+/* Pruned product: C = A*B holding only its numerically non-empty tiles (the structural tile
+ * set of mg_multiply_symbolic less the tiles whose product is zero), row-major; every stored
+ * tile is bit-identical to the same tile of mg_multiply_symbolic + mg_multiply_numeric. No
+ * counterpart in the reference, which allocates every structural tile
+ * (ppr_query.cu:288-314). capacity chooses how the output is sized:
+ *   MG_PRUNED_EXACT (0): a flag pass finds the non-empty tiles, a second pass writes each at
+ *     its final position (two numeric passes, memory = the output);
+ *   MG_PRUNED_STRUCTURAL: one pass into scratch for every structural tile, then sorted;
+ *   any other value (e.g. a cost model's prediction): one pass into that many slots, then
+ *     sorted; if more tiles are non-empty the pass is run again with the exact count (fallback).
+ * Synchronises. stats may be NULL. */
+#define MG_PRUNED_EXACT 0ull
+#define MG_PRUNED_STRUCTURAL (~0ull)
+typedef struct mg_pruned_stats {
+  uint64_t structural_tiles; /* structural tiles of C */
+  uint64_t numeric_tiles;    /* stored (non-empty) tiles */
+  uint64_t tile_products;    /* structural (r,k,c) triples */
+  uint64_t capacity;         /* output slots of the first pass */
+  uint64_t scratch_bytes;    /* largest pass scratch (0 for MG_PRUNED_EXACT) */
+  uint64_t output_bytes;     /* tile data of the result */
+  uint32_t passes;           /* numeric passes run */
+  uint32_t fallback;         /* 1: the capacity was too small and the pass ran again */
+} mg_pruned_stats;
+int mg_multiply_pruned(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, uint64_t capacity,
+                       mg_tiled_matrix** c, mg_pruned_stats* stats);
+/* Features of every row x of C = A*B known after the symbolic pass, MG_ROW_FEATURES floats per
+ * row into features[x * MG_ROW_FEATURES ...]: log1p of the structural tiles / 16, of the tile
+ * products / 20, of A's tiles in row x / 16, of their set bits / 20, of the set bits of the B
+ * rows they reach / 24; log1p of the mean set bits per A tile / 8 and per reached B tile / 8;
+ * structural tiles / n_strips. structural (may be NULL) receives the structural tiles per
+ * row, *tile_products (may be NULL) their total. No counterpart in the reference. */
+#define MG_ROW_FEATURES 8
+int mg_multiply_row_features(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                             float* features, uint32_t* structural, uint64_t* tile_products);
 /* Allocation-free numeric pass: same result as mg_multiply_numeric, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). No counterpart in the reference. Makes no hipMalloc/hipFree call: the
@@ -518,6 +552,17 @@ int mg_mlp_num_params(const mg_mlp* p, uint32_t* n_params);
 int mg_mlp_get_params(mg_context* ctx, const mg_mlp* p, float* master, uint16_t* bf16);
 int mg_mlp_set_params(mg_context* ctx, mg_mlp* p, const float* master);
 int mg_mlp_forward(mg_context* ctx, const mg_mlp* p, const float* x, uint32_t n, float* y);
+/* mg_multiply_pruned sized by a cost model: after the structural pass, the model (n_in =
+ * MG_ROW_FEATURES; output 0 = predicted fraction of a row's structural tiles that are
+ * non-empty) runs on the rows' features on the device, and the first numeric pass gets
+ * sum over rows of min(structural, ceil(structural * prediction * (1 + margin))) slots; a
+ * prediction below the non-empty count makes the pass run again at the exact count (the
+ * exact fallback), so the result is always that of mg_multiply_pruned. row_predicted (host,
+ * n_strips floats, may be NULL) receives structural * prediction per row. No counterpart in the
+ * reference. */
+int mg_multiply_pruned_model(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                             const mg_mlp* model, float margin, float* row_predicted, mg_tiled_matrix** c,
+                             mg_pruned_stats* stats);
 /* loss = mean over samples and outputs of (y - t)^2; grad (host, n_params) of that loss
  * with respect to the BF16 weights. Either output may be NULL. Synchronises. */
 int mg_mlp_loss_grad(mg_context* ctx, mg_mlp* p, const float* x, const float* t, uint32_t n, float* loss, float* grad);

@@ -226,6 +226,14 @@ class TiledMatrix:
                "mg_tiled_matrix_download")
         return HostTiledMatrix(ns, off, row, col, data)
 
+    def download_offsets(self) -> np.ndarray:
+        """tile_offset_row only (n_strips + 1 entries)."""
+        ns, _, _ = self.metadata()
+        off = np.empty(ns + 1, np.uint32)
+        _check(lib().mg_tiled_matrix_download(self.ctx.h, self.h, _ptr(off), None, None, None, None),
+               "mg_tiled_matrix_download")
+        return off
+
     def device_arrays(self):
         p = [C.c_void_p() for _ in range(4)]
         _check(lib().mg_tiled_matrix_device_arrays(self.h, *[C.byref(x) for x in p]), "mg_tiled_matrix_device_arrays")
@@ -285,6 +293,42 @@ def multiply_numeric_noalloc(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: T
 
 
 NOALLOC_MAX_TILES = (32 << 20) // 16 - 1
+
+PRUNED_EXACT = 0
+PRUNED_STRUCTURAL = (1 << 64) - 1
+ROW_FEATURES = 8
+
+
+def multiply_pruned(ctx: Context, a: TiledMatrix, bt: TiledMatrix, capacity: int = PRUNED_EXACT):
+    """C = A * B keeping only numerically non-empty tiles (mg_multiply_pruned); capacity:
+    PRUNED_EXACT (two passes), PRUNED_STRUCTURAL (one pass, structural scratch) or a predicted
+    tile count (one pass, exact retry on overflow). Returns (C, stats dict)."""
+    h = C.c_void_p()
+    st = _capi.PrunedStats()
+    _check(lib().mg_multiply_pruned(ctx.h, a.h, bt.h, int(capacity), C.byref(h), C.byref(st)), "mg_multiply_pruned")
+    return TiledMatrix(ctx, h), {f: getattr(st, f) for f, _ in _capi.PrunedStats._fields_}
+
+
+def multiply_pruned_model(ctx: Context, a: TiledMatrix, bt: TiledMatrix, model: "MLP", margin: float = 0.25):
+    """mg_multiply_pruned sized by the cost model (device-side features and inference, exact
+    retry on overflow). Returns (C, stats dict, per-row predicted non-empty tiles)."""
+    h = C.c_void_p()
+    st = _capi.PrunedStats()
+    pred = np.zeros(a.n_strips, np.float32)
+    _check(lib().mg_multiply_pruned_model(ctx.h, a.h, bt.h, model.h, margin, _ptr(pred), C.byref(h), C.byref(st)),
+           "mg_multiply_pruned_model")
+    return TiledMatrix(ctx, h), {f: getattr(st, f) for f, _ in _capi.PrunedStats._fields_}, pred
+
+
+def multiply_row_features(ctx: Context, a: TiledMatrix, bt: TiledMatrix):
+    """(features float32 (n_strips, ROW_FEATURES), structural tiles per C row, tile products)."""
+    n = a.n_strips
+    feat = np.zeros((n, ROW_FEATURES), np.float32)
+    cnt = np.zeros(n, np.uint32)
+    tp = C.c_uint64()
+    _check(lib().mg_multiply_row_features(ctx.h, a.h, bt.h, _ptr(feat), _ptr(cnt),
+                                          C.byref(tp)), "mg_multiply_row_features")
+    return feat, cnt, tp.value
 
 
 def multiply(ctx: Context, a: TiledMatrix, bt: TiledMatrix):

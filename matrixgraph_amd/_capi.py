@@ -25,6 +25,11 @@ class ProductStats(C.Structure):
     _fields_ = [("tile_products", u64), ("out_tiles", u64), ("right_tiles_used", u64), ("alg_bytes", u64)]
 
 
+class PrunedStats(C.Structure):
+    _fields_ = [("structural_tiles", u64), ("numeric_tiles", u64), ("tile_products", u64), ("capacity", u64),
+                ("scratch_bytes", u64), ("output_bytes", u64), ("passes", u32), ("fallback", u32)]
+
+
 class HopStats(C.Structure):
     _fields_ = [
         ("tile_products", u64),
@@ -101,6 +106,9 @@ SIGNATURES = {
     "mg_multiply_numeric": (i32, [vp, vp, vp, vp]),
     "mg_multiply_numeric_dense": (i32, [vp, vp, vp, vp, u32, P_u64]),
     "mg_multiply_numeric_noalloc": (i32, [vp, vp, vp, vp]),
+    "mg_multiply_pruned": (i32, [vp, vp, vp, u64, C.POINTER(vp), C.POINTER(PrunedStats)]),
+    "mg_multiply_row_features": (i32, [vp, vp, vp, vp, vp, P_u64]),
+    "mg_multiply_pruned_model": (i32, [vp, vp, vp, vp, C.c_float, vp, C.POINTER(vp), C.POINTER(PrunedStats)]),
     # edges
     "mg_edges_rmat": (i32, [vp, u32, u32, f64, f64, f64, u64, C.POINTER(vp)]),
     "mg_edges_er": (i32, [vp, u32, u64, u32, u64, C.POINTER(vp)]),

@@ -189,6 +189,9 @@ void noalloc_scratch_resolve(int device);
 // The address looked up by noalloc_scratch_resolve for `device`.
 void* noalloc_scratch(int device);
 
+// Input and output widths of a cost-model MLP (mg_mlp.hip).
+void mlp_dims(const mg_mlp* p, uint32_t* n_in, uint32_t* n_out);
+
 // Helpers implemented in mg_matrix.hip / mg_graph.hip, shared across units.
 void alloc_tiled_matrix(mg_tiled_matrix* m, uint32_t n_strips, uint32_t n_nz_tile);
 uint64_t device_popcount(mg_context* ctx, const uint64_t* data, uint64_t n_words);

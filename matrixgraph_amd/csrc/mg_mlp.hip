@@ -286,6 +286,13 @@ void compute_grad(mg_context* ctx, mg_mlp* p, const float* x, const float* t, ui
 
 }  // namespace
 
+namespace mgamd {
+void mlp_dims(const mg_mlp* p, uint32_t* n_in, uint32_t* n_out) {
+  *n_in = p->d.n_in;
+  *n_out = p->d.n_out;
+}
+}  // namespace mgamd
+
 extern "C" int mg_mlp_create(mg_context* ctx, uint32_t n_in, uint32_t n_out, uint64_t seed, mg_mlp** out) {
   return guarded([&] {
     MG_REQUIRE(ctx && out, "mg_mlp_create: NULL argument");

@@ -12,6 +12,8 @@
 #include <mutex>
 #include <string>
 
+#include <hipcub/hipcub.hpp>
+
 #include "mg_device.hpp"
 #include "mg_internal.hpp"
 
@@ -24,6 +26,7 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 #define MG_PAIR_BATCH 4
 #endif
 constexpr int kPairBatch = MG_PAIR_BATCH;  // tile pairs whose loads fill_tiles_kernel issues together
+constexpr uint32_t kRowFeatures = MG_ROW_FEATURES;  // features per C row (mg_multiply_row_features)
 
 __device__ inline uint64_t tile_product(uint64_t l_row, uint64_t rt_row) {
   uint64_t nz = ballot(rt_row != 0);
@@ -210,14 +213,28 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
 // accumulator. Pairs with a denser tile take the full-tile wave product, kPairBatch at a time.
 // kMfma: a pair whose two tiles both have at least min_bits set bits (bits_a, bits_bt) is
 // taken first, by the matrix cores, one pair at a time; dense_count counts those pairs.
-template <bool kMfma>
+// kOut: where a finished C tile goes (PrunedOut): ORed into data_c at its structural index
+// (kOutStruct), or, for a pruned product, only if it is non-zero: appended at the next slot
+// of out.data with its structural index in out.key (kOutAppend, slots past out.capacity are
+// counted but not written), flagged in out.flag (kOutFlag, no data written), or written at
+// out.pos[i] (kOutPos).
+enum : int { kOutStruct = 0, kOutAppend = 1, kOutFlag = 2, kOutPos = 3 };
+struct PrunedOut {
+  uint64_t* data;
+  uint32_t* key;
+  uint32_t* flag;
+  const uint32_t* pos;
+  unsigned long long* counter;
+  uint64_t capacity;
+};
+template <bool kMfma, int kOut = kOutStruct>
 __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
     uint32_t n_c, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ off_bt,
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
     const WidePayload* __restrict__ pl_a, const WidePayload* __restrict__ pl_bt, uint64_t* __restrict__ data_c,
     const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt, uint32_t min_bits,
-    unsigned long long* __restrict__ dense_count) {
+    unsigned long long* __restrict__ dense_count, PrunedOut out = PrunedOut{}) {
   __shared__ unsigned long long s_acc[kWavesPerBlock][kTile];
   const uint32_t lane = lane_id();
   uint64_t n_mfma = 0;
@@ -359,8 +376,27 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
     acc |= acc_l[lane];
     acc_l[lane] = 0;
     wave_lds_sync();
-    uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
-    if (acc) *dst |= acc;
+    if constexpr (kOut == kOutStruct) {
+      uint64_t* dst = data_c + static_cast<uint64_t>(i) * kTile + lane;
+      if (acc) *dst |= acc;
+    } else {
+      const bool nz = ballot(acc != 0) != 0;  // wave-uniform
+      if constexpr (kOut == kOutFlag) {
+        if (lane == 0) out.flag[i] = nz ? 1u : 0u;
+      } else if constexpr (kOut == kOutPos) {
+        if (nz) out.data[static_cast<uint64_t>(out.pos[i]) * kTile + lane] = acc;
+      } else {
+        if (nz) {
+          unsigned long long slot = 0;
+          if (lane == 0) slot = atomicAdd(out.counter, 1ull);
+          slot = read_first_lane64(slot);
+          if (slot < out.capacity) {
+            out.data[slot * kTile + lane] = acc;
+            if (lane == 0) out.key[slot] = i;
+          }
+        }
+      }
+    }
   }
   if (kMfma && dense_count && lane == 0 && n_mfma)
     atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(n_mfma));
@@ -625,6 +661,193 @@ void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_
   MG_LAUNCH_CHECK();
 }
 
+
+// --- structural pass shared by the handle-level products ----------------------
+// Row form of B (B.row(t) = { y : Bt(y,t) present }), per-row structural counts and tile
+// products of C = A*B, C's row pointer, and the totals.
+struct Structural {
+  uint32_t n = 0;  // n_strips
+  DevBuf<uint32_t> off_b, col_b, cnt, off_c;
+  DevBuf<unsigned long long> tp;
+  uint64_t n_c = 0, tile_products = 0, right_used = 0;
+};
+
+void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, Structural& S,
+                     const char* who) {
+  MG_REQUIRE(a->n_strips == bt->n_strips, std::string(who) + ": operands differ in n_strips");
+  const uint32_t n = a->n_strips;
+  const uint32_t n_words = (n + 63) / 64;
+  if (n_words > kMaxLdsBitmapWords) fail(MG_ERR_UNSUPPORTED, std::string(who) + ": n_strips > 524288 not supported");
+  hipStream_t s = ctx->stream;
+  S.n = n;
+  DevBuf<uint32_t> hist(n), cursor(n);
+  S.off_b.alloc(n + 1ull);
+  S.col_b.alloc(bt->n_nz_tile ? bt->n_nz_tile : 1);
+  MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * n, s));
+  MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * n, s));
+  if (bt->n_nz_tile) {
+    col_hist_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile, hist.get());
+    MG_LAUNCH_CHECK();
+  }
+  exclusive_scan_u32(ctx, hist.get(), S.off_b.get(), n);
+  if (bt->n_nz_tile) {
+    col_scatter_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->row_idx.get(), bt->col_idx.get(),
+                                                                    bt->n_nz_tile, S.off_b.get(), cursor.get(),
+                                                                    S.col_b.get());
+    MG_LAUNCH_CHECK();
+  }
+  S.cnt.alloc(n);
+  S.off_c.alloc(n + 1ull);
+  S.tp.alloc(n);
+  DevBuf<unsigned long long> sums(2);
+  MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 2, s));
+  const size_t lds = sizeof(uint64_t) * n_words;
+  const unsigned grid = grid_for(n, 1, 16384);
+  symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
+                                                       S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
+                                                       nullptr);
+  MG_LAUNCH_CHECK();
+  // C's tile index is 32-bit (the reference's n_nz_tile is u32): bound the 64-bit total
+  // before the 32-bit scan.
+  const uint64_t n_c64 = device_sum_u32(ctx, S.cnt.get(), n);
+  MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, std::string(who) + ": " + std::to_string(n_c64) +
+                                         " output tiles, more than a u32 tile index holds");
+  exclusive_scan_u32(ctx, S.cnt.get(), S.off_c.get(), n);
+  sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(S.tp.get(), n, sums.get());
+  MG_LAUNCH_CHECK();
+  DevBuf<uint8_t> ref(n);
+  MG_HIP(hipMemsetAsync(ref.get(), 0, n, s));
+  if (a->n_nz_tile) {
+    mark_referenced_kernel<<<grid_for(a->n_nz_tile, 256), 256, 0, s>>>(a->col_idx.get(), a->n_nz_tile, ref.get());
+    MG_LAUNCH_CHECK();
+  }
+  used_right_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(ref.get(), S.off_b.get(), n, sums.get() + 1);
+  MG_LAUNCH_CHECK();
+  unsigned long long h_sums[2] = {0, 0};
+  MG_HIP(hipMemcpyAsync(h_sums, sums.get(), sizeof(h_sums), hipMemcpyDeviceToHost, s));
+  MG_HIP(hipStreamSynchronize(s));  // the temporaries here are freed on return
+  S.n_c = n_c64;
+  S.tile_products = h_sums[0];
+  S.right_used = h_sums[1];
+}
+
+// The structural (row, col) index of C, row-major, into row_c / col_c (S.n_c entries).
+void structural_emit(mg_context* ctx, const mg_tiled_matrix* a, const Structural& S, uint32_t* row_c,
+                     uint32_t* col_c) {
+  if (!S.n_c) return;
+  const size_t lds = sizeof(uint64_t) * ((S.n + 63) / 64);
+  symbolic_row_kernel<true><<<grid_for(S.n, 1, 16384), kBlock, lds, ctx->stream>>>(
+      S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
+      row_c, col_c);
+  MG_LAUNCH_CHECK();
+}
+
+// --- pruned product: only the numerically non-empty tiles of C ------------------
+// Final index of the appended tiles: sorted keys (structural indices) give each output
+// tile its row and column; data moves from its slot. One wave per tile.
+__global__ void __launch_bounds__(kBlock) gather_pruned_kernel(uint32_t n, const uint32_t* __restrict__ keys,
+                                                               const uint32_t* __restrict__ slots,
+                                                               const uint32_t* __restrict__ row_s,
+                                                               const uint32_t* __restrict__ col_s,
+                                                               const uint64_t* __restrict__ scratch,
+                                                               uint32_t* __restrict__ row_c,
+                                                               uint32_t* __restrict__ col_c,
+                                                               uint64_t* __restrict__ data_c) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t j = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); j < n; j += n_waves) {
+    const uint32_t k = keys[j];
+    data_c[static_cast<uint64_t>(j) * kTile + lane] = scratch[static_cast<uint64_t>(slots[j]) * kTile + lane];
+    if (lane == 0) {
+      row_c[j] = row_s[k];
+      col_c[j] = col_s[k];
+    }
+  }
+}
+
+// Index of the flagged tiles at their exact positions (kOutPos pass): pos = exclusive scan
+// of the flags.
+__global__ void __launch_bounds__(kBlock) place_pruned_index_kernel(uint32_t n_s, const uint32_t* __restrict__ flag,
+                                                                    const uint32_t* __restrict__ pos,
+                                                                    const uint32_t* __restrict__ row_s,
+                                                                    const uint32_t* __restrict__ col_s,
+                                                                    uint32_t* __restrict__ row_c,
+                                                                    uint32_t* __restrict__ col_c) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_s; i += gridDim.x * blockDim.x)
+    if (flag[i]) {
+      row_c[pos[i]] = row_s[i];
+      col_c[pos[i]] = col_s[i];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) iota_u32_kernel(uint32_t* __restrict__ out, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) row_hist_kernel(const uint32_t* __restrict__ row, uint32_t n,
+                                                          uint32_t* __restrict__ hist) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(hist + row[i], 1u);
+}
+
+template <int kOut>
+void launch_pruned(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, uint32_t n_s,
+                   const uint32_t* row_s, const uint32_t* col_s, WidePayload* pl, const PrunedOut& out) {
+  if (!n_s) return;
+  fill_tiles_payload_kernel<false, kOut><<<grid_for(n_s, kWavesPerBlock, 65536), kBlock, 0, ctx->stream>>>(
+      n_s, a->offset_row.get(), bt->offset_row.get(), row_s, col_s, a->col_idx.get(), bt->col_idx.get(),
+      a->data.get(), bt->data.get(), pl, pl + a->n_nz_tile, nullptr, nullptr, nullptr, 0, nullptr, out);
+  MG_LAUNCH_CHECK();
+}
+
+// Per C row: features of the cost model (see mg_multiply_row_features).
+__global__ void __launch_bounds__(kBlock) brow_bits_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                           const uint32_t* __restrict__ bits,
+                                                           unsigned long long* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(out + col[i], static_cast<unsigned long long>(bits[i]));
+}
+
+__global__ void __launch_bounds__(kBlock) row_features_kernel(uint32_t n, const uint32_t* __restrict__ off_a,
+                                                              const uint32_t* __restrict__ col_a,
+                                                              const uint32_t* __restrict__ bits_a,
+                                                              const unsigned long long* __restrict__ brow_bits,
+                                                              const uint32_t* __restrict__ cnt,
+                                                              const unsigned long long* __restrict__ tp,
+                                                              float* __restrict__ feat) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t x = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); x < n; x += n_waves) {
+    const uint32_t a0 = off_a[x], a1 = off_a[x + 1];
+    uint64_t abits = 0, bbits = 0, amax = 0;
+    for (uint32_t k = a0 + lane; k < a1; k += kWave) {
+      const uint64_t ab = bits_a[k];
+      abits += ab;
+      amax = ab > amax ? ab : amax;
+      bbits += brow_bits[col_a[k]];
+    }
+    abits = wave_sum(abits);
+    bbits = wave_sum(bbits);
+    for (int o = 32; o >= 1; o >>= 1) {
+      const uint64_t v = static_cast<uint64_t>(__shfl_xor(static_cast<long long>(amax), o, kWave));
+      amax = v > amax ? v : amax;
+    }
+    if (lane == 0) {
+      const float nt = static_cast<float>(a1 - a0);
+      const float c = static_cast<float>(cnt[x]);
+      const float t = static_cast<float>(tp[x]);
+      float* f = feat + static_cast<uint64_t>(x) * kRowFeatures;
+      f[0] = log1pf(c) / 16.0f;
+      f[1] = log1pf(t) / 20.0f;
+      f[2] = log1pf(nt) / 16.0f;
+      f[3] = log1pf(static_cast<float>(abits)) / 20.0f;
+      f[4] = log1pf(static_cast<float>(bbits)) / 24.0f;
+      f[5] = log1pf(nt > 0 ? static_cast<float>(abits) / nt : 0.0f) / 8.0f;
+      f[6] = log1pf(t > 0 ? static_cast<float>(bbits) / t : 0.0f) / 8.0f;
+      f[7] = c / static_cast<float>(n);
+    }
+  }
+}
 }  // namespace
 
 // Payload scratch of mg_multiply_numeric_noalloc: one copy per device, held from the moment
@@ -727,77 +950,25 @@ extern "C" int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, c
   return guarded([&] {
     MG_REQUIRE(ctx && a && bt && c_out, "mg_multiply_symbolic: NULL argument");
     *c_out = nullptr;
-    MG_REQUIRE(a->n_strips == bt->n_strips, "mg_multiply_symbolic: operands differ in n_strips");
-    const uint32_t n = a->n_strips;
-    const uint32_t n_words = (n + 63) / 64;
-    if (n_words > kMaxLdsBitmapWords)
-      fail(MG_ERR_UNSUPPORTED, "mg_multiply_symbolic: n_strips > 524288 not supported");
     DeviceGuard g(ctx->device);
     hipStream_t s = ctx->stream;
-    // Row form of B: B.row(t) = { y : Bt(y,t) present }.
-    DevBuf<uint32_t> hist(n), off_b(n + 1ull), cursor(n), col_b(bt->n_nz_tile ? bt->n_nz_tile : 1);
-    MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * n, s));
-    MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * n, s));
-    if (bt->n_nz_tile) {
-      col_hist_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile, hist.get());
-      MG_LAUNCH_CHECK();
-    }
-    exclusive_scan_u32(ctx, hist.get(), off_b.get(), n);
-    if (bt->n_nz_tile) {
-      col_scatter_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->row_idx.get(), bt->col_idx.get(),
-                                                                      bt->n_nz_tile, off_b.get(), cursor.get(),
-                                                                      col_b.get());
-      MG_LAUNCH_CHECK();
-    }
-    // Count pass.
-    DevBuf<uint32_t> cnt(n), off_c(n + 1ull);
-    DevBuf<unsigned long long> tp(n), sums(2);
-    MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 2, s));
-    const size_t lds = sizeof(uint64_t) * n_words;
-    const unsigned grid = grid_for(n, 1, 16384);
-    symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), off_b.get(),
-                                                         col_b.get(), cnt.get(), tp.get(), nullptr, nullptr,
-                                                         nullptr);
-    MG_LAUNCH_CHECK();
-    // C's tile index is 32-bit (the reference's n_nz_tile is u32): bound the 64-bit total
-    // before the 32-bit scan.
-    const uint64_t n_c64 = device_sum_u32(ctx, cnt.get(), n);
-    MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, "mg_multiply_symbolic: " + std::to_string(n_c64) +
-                                           " output tiles, more than a u32 tile index holds");
-    exclusive_scan_u32(ctx, cnt.get(), off_c.get(), n);
-    sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(tp.get(), n, sums.get());
-    MG_LAUNCH_CHECK();
-    DevBuf<uint8_t> ref(n);
-    MG_HIP(hipMemsetAsync(ref.get(), 0, n, s));
-    if (a->n_nz_tile) {
-      mark_referenced_kernel<<<grid_for(a->n_nz_tile, 256), 256, 0, s>>>(a->col_idx.get(), a->n_nz_tile, ref.get());
-      MG_LAUNCH_CHECK();
-    }
-    used_right_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(ref.get(), off_b.get(), n, sums.get() + 1);
-    MG_LAUNCH_CHECK();
-    uint32_t n_c = 0;
-    unsigned long long h_sums[2] = {0, 0};
-    MG_HIP(hipMemcpyAsync(&n_c, off_c.get() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    MG_HIP(hipMemcpyAsync(h_sums, sums.get(), sizeof(h_sums), hipMemcpyDeviceToHost, s));
-    MG_HIP(hipStreamSynchronize(s));
+    Structural S;
+    structural_pass(ctx, a, bt, S, "mg_multiply_symbolic");
     auto c = std::make_unique<mg_tiled_matrix>();
     c->device = ctx->device;
-    alloc_tiled_matrix(c.get(), n, n_c);
-    MG_HIP(hipMemcpyAsync(c->offset_row.get(), off_c.get(), sizeof(uint32_t) * (n + 1ull), hipMemcpyDeviceToDevice,
-                          s));
-    if (n_c) {
+    alloc_tiled_matrix(c.get(), S.n, static_cast<uint32_t>(S.n_c));
+    MG_HIP(hipMemcpyAsync(c->offset_row.get(), S.off_c.get(), sizeof(uint32_t) * (S.n + 1ull),
+                          hipMemcpyDeviceToDevice, s));
+    if (S.n_c) {
       MG_HIP(hipMemsetAsync(c->data.get(), 0, c->data.bytes(), s));
-      symbolic_row_kernel<true><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), off_b.get(),
-                                                          col_b.get(), nullptr, nullptr, off_c.get(),
-                                                          c->row_idx.get(), c->col_idx.get());
-      MG_LAUNCH_CHECK();
+      structural_emit(ctx, a, S, c->row_idx.get(), c->col_idx.get());
     }
     MG_HIP(hipStreamSynchronize(s));
     if (stats) {
-      stats->tile_products = h_sums[0];
-      stats->right_tiles_used = h_sums[1];
-      stats->out_tiles = n_c;
-      stats->alg_bytes = 520ull * (a->n_nz_tile + h_sums[1] + n_c) + 4ull * (2ull * n + 2);
+      stats->tile_products = S.tile_products;
+      stats->right_tiles_used = S.right_used;
+      stats->out_tiles = S.n_c;
+      stats->alg_bytes = 520ull * (a->n_nz_tile + S.right_used + S.n_c) + 4ull * (2ull * S.n + 2);
     }
     *c_out = c.release();
   });
@@ -813,6 +984,274 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     if (c->n_nz_tile == 0) return;
     WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
     launch_payload_product(ctx, a, bt, c, pl, nullptr, nullptr, 0, nullptr);
+  });
+}
+
+namespace mgamd {
+namespace {
+
+// Structural index, operand payloads and features of a pruned product.
+struct PrunedPlan {
+  Structural S;
+  DevBuf<uint32_t> row_s, col_s;
+  WidePayload* pl = nullptr;
+};
+
+void pruned_prepare(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, PrunedPlan& P,
+                    const char* who) {
+  hipStream_t s = ctx->stream;
+  structural_pass(ctx, a, bt, P.S, who);
+  const uint32_t n_s = static_cast<uint32_t>(P.S.n_c);
+  P.row_s.alloc(std::max<uint32_t>(1, n_s));
+  P.col_s.alloc(std::max<uint32_t>(1, n_s));
+  structural_emit(ctx, a, P.S, P.row_s.get(), P.col_s.get());
+  P.pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
+  if (a->n_nz_tile) {
+    payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(), a->n_nz_tile,
+                                                                                   P.pl);
+    MG_LAUNCH_CHECK();
+  }
+  if (bt->n_nz_tile) {
+    payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(bt->data.get(), bt->n_nz_tile,
+                                                                                     P.pl + a->n_nz_tile);
+    MG_LAUNCH_CHECK();
+  }
+}
+
+// Features of every C row into feat (device, n_strips x kRowFeatures).
+void row_features_into(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, const Structural& S,
+                       float* feat) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = S.n;
+  DevBuf<uint32_t> bits_a(std::max<uint32_t>(1, a->n_nz_tile)), bits_b(std::max<uint32_t>(1, bt->n_nz_tile));
+  DevBuf<unsigned long long> brow(std::max<uint32_t>(1, n));
+  MG_HIP(hipMemsetAsync(brow.get(), 0, sizeof(unsigned long long) * std::max<uint32_t>(1, n), s));
+  if (a->n_nz_tile) {
+    tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(), a->n_nz_tile,
+                                                                                       bits_a.get());
+    MG_LAUNCH_CHECK();
+  }
+  if (bt->n_nz_tile) {
+    tile_bits_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(bt->data.get(), bt->n_nz_tile,
+                                                                                        bits_b.get());
+    MG_LAUNCH_CHECK();
+    // Set bits of each row t of B = of the Bt tiles in column t.
+    brow_bits_kernel<<<grid_for(bt->n_nz_tile, kBlock), kBlock, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile,
+                                                                       bits_b.get(), brow.get());
+    MG_LAUNCH_CHECK();
+  }
+  if (n) {
+    row_features_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+        n, a->offset_row.get(), a->col_idx.get(), bits_a.get(), brow.get(), S.cnt.get(), S.tp.get(), feat);
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+}
+
+// Output slots from a prediction: per row min(structural, ceil(structural * clamp(y) *
+// (1 + margin))); the total into *out, the unrounded prediction per row into pred.
+__global__ void __launch_bounds__(kBlock) capacity_kernel(uint32_t n, const uint32_t* __restrict__ cnt,
+                                                          const float* __restrict__ y, uint32_t y_stride,
+                                                          float margin, float* __restrict__ pred,
+                                                          unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
+    const float c = static_cast<float>(cnt[x]);
+    const float f = fminf(fmaxf(y[static_cast<uint64_t>(x) * y_stride], 0.0f), 1.0f);
+    const float pr = c * f;
+    pred[x] = pr;
+    const float cap = fminf(ceilf(pr * (1.0f + margin)), c);
+    local += static_cast<uint64_t>(cap);
+  }
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+}
+
+// Numeric passes of a pruned product into *c (see mg_multiply_pruned for `capacity`).
+void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, PrunedPlan& P,
+                    uint64_t capacity, mg_tiled_matrix* c, mg_pruned_stats& st) {
+  hipStream_t s = ctx->stream;
+  const Structural& S = P.S;
+  const uint32_t n_s = static_cast<uint32_t>(S.n_c);
+  const uint32_t* row_s = P.row_s.get();
+  const uint32_t* col_s = P.col_s.get();
+  st.structural_tiles = n_s;
+  st.tile_products = S.tile_products;
+  uint64_t n_num = 0;
+  uint64_t cap = capacity == MG_PRUNED_STRUCTURAL ? n_s : std::min<uint64_t>(capacity, n_s);
+  if (capacity == MG_PRUNED_EXACT || n_s == 0) {
+    // Exact: flag pass, scan, then each non-empty tile written at its final position.
+    DevBuf<uint32_t> flag(std::max<uint32_t>(1, n_s)), pos(n_s + 1ull);
+    PrunedOut fo{};
+    fo.flag = flag.get();
+    launch_pruned<kOutFlag>(ctx, a, bt, n_s, row_s, col_s, P.pl, fo);
+    exclusive_scan_u32(ctx, flag.get(), pos.get(), n_s);
+    uint32_t n32 = 0;
+    MG_HIP(hipMemcpyAsync(&n32, pos.get() + n_s, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+    n_num = n32;
+    alloc_tiled_matrix(c, S.n, n32);
+    if (n32) {
+      PrunedOut po{};
+      po.data = c->data.get();
+      po.pos = pos.get();
+      launch_pruned<kOutPos>(ctx, a, bt, n_s, row_s, col_s, P.pl, po);
+      place_pruned_index_kernel<<<grid_for(n_s, kBlock), kBlock, 0, s>>>(n_s, flag.get(), pos.get(), row_s, col_s,
+                                                                        c->row_idx.get(), c->col_idx.get());
+      MG_LAUNCH_CHECK();
+    }
+    st.passes = 2;
+    st.capacity = n32;
+    st.scratch_bytes = 0;
+    MG_HIP(hipStreamSynchronize(s));  // flag and pos are freed on return
+  } else {
+    // One pass into `cap` slots; a capacity below the non-empty count is found by the
+    // counter, which keeps counting past it, and the pass is run again with the exact count.
+    DevBuf<unsigned long long> counter(1);
+    st.capacity = cap;
+    for (int attempt = 0;; ++attempt) {
+      const uint64_t slots = std::max<uint64_t>(1, cap);
+      DevBuf<uint64_t> scratch(slots * kTile);
+      DevBuf<uint32_t> keys(slots);
+      MG_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned long long), s));
+      PrunedOut ao{};
+      ao.data = scratch.get();
+      ao.key = keys.get();
+      ao.counter = counter.get();
+      ao.capacity = cap;
+      launch_pruned<kOutAppend>(ctx, a, bt, n_s, row_s, col_s, P.pl, ao);
+      unsigned long long got = 0;
+      MG_HIP(hipMemcpyAsync(&got, counter.get(), sizeof(got), hipMemcpyDeviceToHost, s));
+      MG_HIP(hipStreamSynchronize(s));
+      st.passes += 1;
+      st.scratch_bytes = std::max<uint64_t>(st.scratch_bytes, slots * kTile * sizeof(uint64_t));
+      if (got > cap) {
+        MG_REQUIRE(attempt == 0, "mg_multiply_pruned: internal capacity overflow after the exact retry");
+        st.fallback = 1;
+        cap = got;
+        continue;
+      }
+      n_num = got;
+      const uint32_t n32 = static_cast<uint32_t>(got);
+      alloc_tiled_matrix(c, S.n, n32);
+      if (n32) {
+        // Row-major order: sort the slots by structural index, then move the tiles.
+        DevBuf<uint32_t> slot_in(n32), keys_out(n32), slot_out(n32);
+        iota_u32_kernel<<<grid_for(n32, kBlock), kBlock, 0, s>>>(slot_in.get(), n32);
+        MG_LAUNCH_CHECK();
+        int bits = 1;
+        while ((1ull << bits) < n_s) ++bits;
+        size_t temp_bytes = 0;
+        MG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.get(), keys_out.get(), slot_in.get(),
+                                                  slot_out.get(), static_cast<int>(n32), 0, bits, s));
+        DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
+        MG_HIP(hipcub::DeviceRadixSort::SortPairs(temp.get(), temp_bytes, keys.get(), keys_out.get(), slot_in.get(),
+                                                  slot_out.get(), static_cast<int>(n32), 0, bits, s));
+        gather_pruned_kernel<<<grid_for(n32, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+            n32, keys_out.get(), slot_out.get(), row_s, col_s, scratch.get(), c->row_idx.get(), c->col_idx.get(),
+            c->data.get());
+        MG_LAUNCH_CHECK();
+      }
+      MG_HIP(hipStreamSynchronize(s));  // scratch, keys and the sort buffers are freed next
+      break;
+    }
+  }
+  // Row pointer of the pruned C from its row index.
+  DevBuf<uint32_t> hist(std::max<uint32_t>(1, S.n));
+  MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * S.n, s));
+  if (n_num) {
+    row_hist_kernel<<<grid_for(n_num, kBlock), kBlock, 0, s>>>(c->row_idx.get(), static_cast<uint32_t>(n_num),
+                                                               hist.get());
+    MG_LAUNCH_CHECK();
+  }
+  exclusive_scan_u32(ctx, hist.get(), c->offset_row.get(), S.n);
+  MG_HIP(hipStreamSynchronize(s));
+  st.numeric_tiles = n_num;
+  st.output_bytes = n_num * kTile * sizeof(uint64_t);
+}
+
+}  // namespace
+}  // namespace mgamd
+
+extern "C" int mg_multiply_pruned(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                  uint64_t capacity, mg_tiled_matrix** c_out, mg_pruned_stats* stats) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && c_out, "mg_multiply_pruned: NULL argument");
+    *c_out = nullptr;
+    DeviceGuard g(ctx->device);
+    PrunedPlan P;
+    pruned_prepare(ctx, a, bt, P, "mg_multiply_pruned");
+    auto c = std::make_unique<mg_tiled_matrix>();
+    c->device = ctx->device;
+    mg_pruned_stats st{};
+    pruned_execute(ctx, a, bt, P, capacity, c.get(), st);
+    if (stats) *stats = st;
+    *c_out = c.release();
+  });
+}
+
+extern "C" int mg_multiply_pruned_model(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                        const mg_mlp* model, float margin, float* row_predicted,
+                                        mg_tiled_matrix** c_out, mg_pruned_stats* stats) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && model && c_out, "mg_multiply_pruned_model: NULL argument");
+    *c_out = nullptr;
+    MG_REQUIRE(margin >= 0.0f && margin <= 16.0f, "mg_multiply_pruned_model: margin must be in [0, 16]");
+    uint32_t n_in = 0, n_out = 0;
+    mlp_dims(model, &n_in, &n_out);
+    MG_REQUIRE(n_in == kRowFeatures && n_out >= 1,
+               "mg_multiply_pruned_model: the model must take MG_ROW_FEATURES inputs");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    PrunedPlan P;
+    pruned_prepare(ctx, a, bt, P, "mg_multiply_pruned_model");
+    const uint32_t n = P.S.n;
+    uint64_t capacity = MG_PRUNED_EXACT;
+    if (P.S.n_c) {
+      DevBuf<float> feat(static_cast<uint64_t>(n) * kRowFeatures), y(static_cast<uint64_t>(n) * n_out), pred(n);
+      DevBuf<unsigned long long> total(1);
+      row_features_into(ctx, a, bt, P.S, feat.get());
+      if (mg_mlp_forward(ctx, model, feat.get(), n, y.get()) != MG_OK)
+        fail(MG_ERR_INTERNAL, std::string("mg_multiply_pruned_model: ") + mg_last_error());
+      MG_HIP(hipMemsetAsync(total.get(), 0, sizeof(unsigned long long), s));
+      capacity_kernel<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(n, P.S.cnt.get(), y.get(), n_out, margin,
+                                                                  pred.get(), total.get());
+      MG_LAUNCH_CHECK();
+      unsigned long long h = 0;
+      MG_HIP(hipMemcpyAsync(&h, total.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+      if (row_predicted) MG_HIP(hipMemcpyAsync(row_predicted, pred.get(), sizeof(float) * n, hipMemcpyDeviceToHost, s));
+      MG_HIP(hipStreamSynchronize(s));
+      capacity = std::max<unsigned long long>(1, h);
+    } else if (row_predicted && n) {
+      std::fill(row_predicted, row_predicted + n, 0.0f);
+    }
+    auto c = std::make_unique<mg_tiled_matrix>();
+    c->device = ctx->device;
+    mg_pruned_stats st{};
+    pruned_execute(ctx, a, bt, P, capacity, c.get(), st);
+    if (stats) *stats = st;
+    *c_out = c.release();
+  });
+}
+
+extern "C" int mg_multiply_row_features(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
+                                        float* features, uint32_t* structural, uint64_t* tile_products) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && a && bt && features, "mg_multiply_row_features: NULL argument");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = ctx->stream;
+    Structural S;
+    structural_pass(ctx, a, bt, S, "mg_multiply_row_features");
+    const uint32_t n = S.n;
+    DevBuf<float> feat(std::max<uint64_t>(1, static_cast<uint64_t>(n) * kRowFeatures));
+    row_features_into(ctx, a, bt, S, feat.get());
+    if (n) {
+      MG_HIP(hipMemcpyAsync(features, feat.get(), sizeof(float) * n * kRowFeatures, hipMemcpyDeviceToHost, s));
+      if (structural)
+        MG_HIP(hipMemcpyAsync(structural, S.cnt.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    }
+    MG_HIP(hipStreamSynchronize(s));
+    if (tile_products) *tile_products = S.tile_products;
   });
 }
 

@@ -213,3 +213,154 @@ def time_batch(ctx, batch, reps=3):
         batch.run()
     ctx.synchronize()
     return (time.perf_counter() - t0) / reps
+
+
+# ----------------------------------------------------------------------------
+# Pre-allocation of pruned products from the predicted output-tile occupancy
+# ----------------------------------------------------------------------------
+# A second use of the same network shape (n_in = ROW_FEATURES -> 64 -> 64 -> 2): per row x
+# of a product C = A * B, from the features the symbolic pass gives (mg_multiply_row_features),
+# predict the fraction of the row's structural tiles that are numerically non-empty
+# (target 0) and log1p(non-empty tiles) / 16 (target 1). The predicted non-empty count sizes
+# the output of mg_multiply_pruned, which then needs one numeric pass instead of the exact
+# two (flag, then write); a prediction below the true count is caught by the pass's counter
+# and the pass runs again at the exact count (the fallback), so the result never changes.
+
+ROW_TARGETS = 2
+CAPACITY_MARGIN = 0.25  # predicted count x (1 + margin), rounded up per row
+
+
+def row_outcomes(c_pruned, structural):
+    """Per-row non-empty tiles of a pruned product and the training targets."""
+    numeric = np.diff(c_pruned.download_offsets()).astype(np.float64)
+    frac = numeric / np.maximum(structural.astype(np.float64), 1.0)
+    T = np.zeros((len(numeric), ROW_TARGETS), np.float32)
+    T[:, 0] = np.clip(frac, 0.0, 1.0)
+    T[:, 1] = np.log1p(numeric) / 16
+    return numeric, T
+
+
+def product_samples(ctx, a, bt):
+    """(X, T, tile_products, C) for one product: rows with structural tiles only; C is the
+    pruned product (exact mode), returned for chaining walks."""
+    import matrixgraph_amd as M
+
+    feat, cnt, tp = M.multiply_row_features(ctx, a, bt)
+    c, _ = M.multiply_pruned(ctx, a, bt, M.PRUNED_EXACT)
+    _, T = row_outcomes(c, cnt)
+    rows = cnt > 0
+    return feat[rows], T[rows], tp, c
+
+
+def predict_capacity(mlp, feat, structural, margin=CAPACITY_MARGIN):
+    """Output slots for a pruned product: per row round_up(structural * predicted fraction *
+    (1 + margin)), at most the structural count; returns (capacity, per-row predicted counts)."""
+    rows = structural > 0
+    pred = np.zeros(len(structural), np.float64)
+    if rows.any():
+        frac = np.clip(mlp.forward(feat[rows])[:, 0].astype(np.float64), 0.0, 1.0)
+        pred[rows] = frac * structural[rows]
+    per_row = np.minimum(np.ceil(pred * (1.0 + margin)), structural.astype(np.float64))
+    return int(per_row.sum()), pred
+
+
+def collect_products(ctx, graphs, queries, square_graphs=(), log=None, groups=None):
+    """Training samples from pruned product walks X_{k+1} = X_k * A of the queries (each hop's
+    pruned X feeds the next) and from A * A of `square_graphs` ((A, A^T) pairs). With a list
+    `groups`, the product each row came from is appended per row (for margin calibration)."""
+    from . import workloads as WL
+
+    xs, ts = [], []
+    pairs = 0
+    n_products = 0
+
+    def add(X, T, tp):
+        nonlocal pairs, n_products
+        xs.append(X)
+        ts.append(T)
+        pairs += tp
+        if groups is not None:
+            groups.extend([n_products] * len(X))
+        n_products += 1
+
+    for q in queries:
+        at = graphs[q.graph]
+        x = WL.sources_matrix(ctx, at.n_strips * 64, q.sources)
+        for _ in range(q.hops):
+            X, T, tp, c = product_samples(ctx, x, at)
+            add(X, T, tp)
+            x.close()
+            x = c
+        x.close()
+    for a, at in square_graphs:
+        X, T, tp, c = product_samples(ctx, a, at)
+        add(X, T, tp)
+        c.close()
+    if log:
+        log(f"collect products: {sum(len(x) for x in xs)} row samples from {n_products} products covering "
+            f"{pairs} tile pairs")
+    return np.concatenate(xs), np.concatenate(ts), pairs
+
+
+def calibrate_margin(mlp, X, T, groups, quantile=0.95, floor=0.05):
+    """The capacity margin under which `quantile` of the training products would have fitted:
+    per product, predicted non-empty tiles (rows' structural x predicted fraction) against the
+    actual ones, both from the logged rows (structural tiles = expm1(16 f0))."""
+    groups = np.asarray(groups)
+    structural = np.expm1(X[:, 0].astype(np.float64) * 16)
+    actual = np.expm1(T[:, 1].astype(np.float64) * 16)
+    pred = structural * np.clip(mlp.forward(X)[:, 0].astype(np.float64), 0.0, 1.0)
+    n = groups.max() + 1 if len(groups) else 0
+    pred_p = np.bincount(groups, weights=pred, minlength=n)
+    act_p = np.bincount(groups, weights=actual, minlength=n)
+    need = (act_p + 1.0) / (pred_p + 1.0) - 1.0
+    return float(max(floor, np.quantile(need, quantile))) if n else floor
+
+
+def pruned_walk(ctx, at, sources, hops, mode, mlp=None, margin=CAPACITY_MARGIN):
+    """Product walk with pruned frontiers; mode 'learned' (capacity from the model),
+    'exact' (two numeric passes) or 'structural' (one pass, structural-size scratch).
+    Returns (X_hops digest, stats dict)."""
+    import time
+
+    import matrixgraph_amd as M
+    from . import workloads as WL
+
+    x = WL.sources_matrix(ctx, at.n_strips * 64, sources)
+    st = {"products": 0, "fallbacks": 0, "passes": 0, "scratch_bytes": 0, "output_bytes": 0, "seconds": 0.0,
+          "rows": 0, "rows_mispredicted": 0, "predicted_tiles": 0, "numeric_tiles": 0, "capacity_tiles": 0,
+          "structural_tiles": 0, "allocated_tiles": 0}
+    for _ in range(hops):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        if mode == "learned":
+            c, ps, pred = M.multiply_pruned_model(ctx, x, at, mlp, margin)
+        else:
+            pred = None
+            c, ps = M.multiply_pruned(ctx, x, at, M.PRUNED_EXACT if mode == "exact" else M.PRUNED_STRUCTURAL)
+        ctx.synchronize()
+        st["seconds"] += time.perf_counter() - t0
+        st["products"] += 1
+        st["fallbacks"] += ps["fallback"]
+        st["passes"] += ps["passes"]
+        st["scratch_bytes"] = max(st["scratch_bytes"], ps["scratch_bytes"])
+        st["output_bytes"] = max(st["output_bytes"], ps["output_bytes"])
+        st["numeric_tiles"] += ps["numeric_tiles"]
+        st["capacity_tiles"] += ps["capacity"]
+        st["structural_tiles"] += ps["structural_tiles"]
+        # Tile slots allocated for the result: the first pass's slots (plus the retry's), or
+        # the exact count.
+        st["allocated_tiles"] += (ps["numeric_tiles"] if mode == "exact"
+                                  else ps["capacity"] + (ps["numeric_tiles"] if ps["fallback"] else 0))
+        if pred is not None:
+            actual = np.diff(c.download_offsets()).astype(np.float64)
+            rows = (pred > 0) | (actual > 0)
+            ratio = (pred[rows] + 1.0) / (actual[rows] + 1.0)
+            st["rows"] += int(rows.sum())
+            st["rows_mispredicted"] += int(np.sum((ratio > 2.0) | (ratio < 0.5)))
+            st["predicted_tiles"] += int(pred.sum())
+        x.close()
+        x = c
+    digest = x.digest()
+    x.close()
+    return digest, st
