@@ -262,6 +262,78 @@ def learned_section(M, ctx, train_specs, train_graphs, train_edges, chunk_tiles,
     }
 
 
+def square_section(M, ctx, scale=14, seed=1, reps=50):
+    """C = A * A (boolean) of R-MAT scale 14 (edge factor 16, seed 1): the symbolic pass
+    (structural tile set, allocation of C) and the numeric pass, inputs resident on the device.
+    Wall time per product (host clock around the two calls and a synchronisation), device time
+    on the context's stream (HIP events around the same calls), tile products per second, and
+    the oracle's tile algorithm (OpenMP) on the host for the same product, checked equal."""
+    import oracle as O
+
+    n = 1 << scale
+    el = M.EdgeList.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed)
+    a = el.to_tiled_matrix(transpose=False)
+    at = el.to_tiled_matrix(transpose=True)
+    _, m = el.info()
+    el.close()
+
+    def once():
+        c, st = M.multiply_symbolic(ctx, a, at)
+        M.multiply_numeric(ctx, a, at, c)
+        ctx.synchronize()
+        return c, st
+
+    for _ in range(5):
+        c, st = once()
+        c.close()
+    wall, dev = [], []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.timer_start()
+        c, st = once()
+        dev.append(ctx.timer_stop())
+        wall.append((time.perf_counter() - t0) * 1e3)
+        if len(wall) < reps:
+            c.close()
+    got = c.download()
+    c.close()
+    s, d = O.rmat_edges(scale, 16, 0.57, 0.19, 0.19, seed)
+    ah = O.edges_to_tiles(n, s, d)
+    ath = O.edges_to_tiles(n, s, d, transpose=True)
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    cpu_reps = 3
+    for _ in range(cpu_reps):
+        c_off, c_row, c_col, tp = O.tile_symbolic(ah[0], ah[1], ah[3], ath[1], ath[3])
+        c_data = O.tile_numeric(ah[0], ah[1:], ath[1:], c_row, c_col, threads=threads)
+    cpu_ms = (time.perf_counter() - t0) / cpu_reps * 1e3
+    equal = (np.array_equal(got.row_idx, c_row) and np.array_equal(got.col_idx, c_col)
+             and np.array_equal(got.data, c_data) and st["tile_products"] == tp)
+    if not equal:
+        raise SystemExit("square: C = A*A differs from the oracle")
+    wall_ms = float(np.median(wall))
+    dev_ms = float(np.median(dev))
+    return {
+        "workload": f"C = A*A, A = R-MAT scale {scale} (edge factor 16, seed {seed}, {m} edges, {a.n_nz_tile} tiles)",
+        "tile_products": st["tile_products"],
+        "out_tiles": st["out_tiles"],
+        "wall_ms": round(wall_ms, 4),
+        "device_ms": round(dev_ms, 4),
+        "wall_over_device": round(wall_ms / dev_ms, 3),
+        "tile_products_per_s": round(st["tile_products"] / (wall_ms * 1e-3), 1),
+        "alg_bytes": st["alg_bytes"],
+        "dense_equivalent_lane_ops_per_s": round(st["tile_products"] * 8192 / (dev_ms * 1e-3), 1),
+        "dense_equivalent_note": "8192 32-bit lane-ops per tile product (64x64 AND-and-test of 64-bit rows); the "
+                                 "kernels multiply sparse tiles from their entry lists and do far less",
+        "reps": reps,
+        "bit_exact_vs_oracle": equal,
+        "cpu_baseline": {"value": round(st["tile_products"] / (cpu_ms * 1e-3), 1), "unit": "tile-products/s",
+                         "ms": round(cpu_ms, 3), "cores": threads, "kind": "port",
+                         "sample": f"the whole product: oracle tile_symbolic + tile_numeric (OpenMP), {cpu_reps} runs"},
+    }
+
+
 def prealloc_section(M, ctx, train_specs, train_graphs, n_train, n_eval):
     """Learned pre-allocation of pruned products: an MLP (ROW_FEATURES -> 64 -> 64 -> 2, the
     library's kernels) trained on per-row outcomes of pruned product walks over the seeds 10-17
@@ -452,6 +524,9 @@ def main():
     ap.add_argument("--prune", action="store_true", help="structural tile sets from numerically non-empty tiles")
     ap.add_argument("--sweep-first-hop", action="store_true",
                     help="sweep every strip at the first hop too (no push from the source strips)")
+    ap.add_argument("--square-only", action="store_true",
+                    help="only the C = A*A section (R-MAT 14, seed 1) as its own JSON line")
+    ap.add_argument("--no-square", action="store_true", help="skip the C = A*A section")
     ap.add_argument("--vertex-order", action="store_true",
                     help="keep frontier strips in vertex order (no internal strip permutation)")
     ap.add_argument("--two-hop", type=int, default=0, metavar="SCALE",
@@ -468,6 +543,9 @@ def main():
     ctx = M.Context(0)
     if args.two_hop:
         two_hop(M, ctx, args.two_hop)
+        return
+    if args.square_only:
+        print(json.dumps({"square": square_section(M, ctx)}), flush=True)
         return
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -616,6 +694,8 @@ def main():
         threads = O.max_threads()
         result["cpu_baseline"] = cpu_baseline(M, O, ctx, w, at, n_edges, args.hops, args.sources, args.cpu_stride,
                                               threads, reps=max(1, args.cpu_reps))
+    if not args.no_square:
+        result["square"] = square_section(M, ctx)
     if not args.no_dense:
         result["dense"] = dense_section(M, ctx)
     if not args.no_batch:

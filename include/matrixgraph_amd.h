@@ -67,6 +67,10 @@ int mg_context_destroy(mg_context* ctx);
  * first waits for the work queued on the previous one. */
 int mg_context_set_stream(mg_context* ctx, void* stream);
 int mg_context_synchronize(mg_context* ctx);
+/* Device time on the context's stream between the two calls (HIP events; _stop waits for the
+ * stream). Measurement helpers for the C-ABI's callers; no counterpart in the reference. */
+int mg_context_timer_start(mg_context* ctx);
+int mg_context_timer_stop(mg_context* ctx, double* ms);
 int mg_context_device(const mg_context* ctx, int* device, int* n_compute_units);
 /* The context's hipStream_t (for layer-1 calls ordered with the context's work). */
 int mg_context_stream(const mg_context* ctx, void** stream);
@@ -230,13 +234,16 @@ typedef struct mg_product_stats {
 /* Symbolic pass: allocates C with the structural tile set of A*B (tile (x,y) present iff
  * some t has A(x,t) and Bt(y,t) present), tiles zeroed. Both operands share n_strips.
  * Replaces PPRQuery::InitResultMatrix (ppr_query.cu:73-457): MatrixBitAnd + MatrixBitCount +
- * InitBitTiledMatrixMetadataByLayoutMatrix + host prefix sum. stats may be NULL. */
+ * InitBitTiledMatrixMetadataByLayoutMatrix + host prefix sum. Synchronises once (to read the
+ * tile count); C's index and zero fill complete asynchronously on the context's stream, and
+ * the temporaries are stream-ordered pool allocations. stats may be NULL. */
 int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                          mg_tiled_matrix** c, mg_product_stats* stats);
 /* Numeric pass into a C produced by mg_multiply_symbolic (OR-accumulates).
  * Replaces PPRQuery::FillTilesUnifiedMemory (ppr_query.cu:459-780). Asynchronous on the
- * context's stream; keeps 16 bytes per operand tile in the context's scratch, whose growth
- * (first call, or larger operands) waits for the work already queued on that stream. */
+ * context's stream; keeps 16 bytes per operand tile in the context's scratch, which grows in
+ * stream order (hipMallocAsync/hipFreeAsync on that stream: queued work that reads the old
+ * buffer finishes before its memory is reused). */
 int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                         mg_tiled_matrix* c);
 /* The numeric pass with a per-tile-pair choice of engine: a pair whose two tiles both have

@@ -131,6 +131,15 @@ class Context:
     def synchronize(self):
         _check(lib().mg_context_synchronize(self.h), "mg_context_synchronize")
 
+    def timer_start(self):
+        _check(lib().mg_context_timer_start(self.h), "mg_context_timer_start")
+
+    def timer_stop(self) -> float:
+        """Milliseconds of device time on the context's stream since timer_start (waits)."""
+        ms = C.c_double()
+        _check(lib().mg_context_timer_stop(self.h, C.byref(ms)), "mg_context_timer_stop")
+        return ms.value
+
     def set_stream(self, stream_ptr: int | None):
         _check(lib().mg_context_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None),
                "mg_context_set_stream")

@@ -106,6 +106,8 @@ SIGNATURES = {
     "mg_multiply_numeric": (i32, [vp, vp, vp, vp]),
     "mg_multiply_numeric_dense": (i32, [vp, vp, vp, vp, u32, P_u64]),
     "mg_multiply_numeric_noalloc": (i32, [vp, vp, vp, vp]),
+    "mg_context_timer_start": (i32, [vp]),
+    "mg_context_timer_stop": (i32, [vp, C.POINTER(f64)]),
     "mg_multiply_pruned": (i32, [vp, vp, vp, u64, C.POINTER(vp), C.POINTER(PrunedStats)]),
     "mg_multiply_row_features": (i32, [vp, vp, vp, vp, vp, P_u64]),
     "mg_multiply_pruned_model": (i32, [vp, vp, vp, vp, C.c_float, vp, C.POINTER(vp), C.POINTER(PrunedStats)]),

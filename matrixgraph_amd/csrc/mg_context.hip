@@ -36,6 +36,13 @@ extern "C" int mg_context_create(int device, mg_context** out) {
               std::to_string(n_dev) + " devices)");
     DeviceGuard g(device);
     noalloc_scratch_resolve(device);
+    {
+      // Temporaries come from the device's default pool (StreamBuf); keep freed memory there.
+      hipMemPool_t pool = nullptr;
+      MG_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+      uint64_t keep = ~0ull;
+      MG_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
     hipDeviceProp_t prop;
     MG_HIP(hipGetDeviceProperties(&prop, device));
     auto* ctx = new mg_context();
@@ -62,8 +69,11 @@ extern "C" int mg_context_destroy(mg_context* ctx) {
   return guarded([&] {
     if (!ctx) return;
     DeviceGuard g(ctx->device);
+    ctx->scratch.release();  // stream-ordered: before the stream goes away
     (void)hipStreamSynchronize(ctx->stream);
     ctx->counters.release();
+    for (hipEvent_t& e : ctx->timer)
+      if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
   });
@@ -77,6 +87,7 @@ extern "C" int mg_context_set_stream(mg_context* ctx, void* stream) {
       // Work queued on the previous stream may still read the context's scratch, which the
       // next call (ordered on the new stream only) may overwrite or regrow.
       DeviceGuard g(ctx->device);
+      ctx->scratch.release();  // freed on the stream that used it; the next call allocates on the new one
       MG_HIP(hipStreamSynchronize(ctx->stream));
     }
     ctx->stream = next;
@@ -179,5 +190,28 @@ extern "C" int mg_buffer_copy(mg_context* ctx, void* dst, const void* src, size_
     MG_REQUIRE(ctx && dst && src, "mg_buffer_copy: NULL argument");
     DeviceGuard g(ctx->device);
     if (bytes) MG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+extern "C" int mg_context_timer_start(mg_context* ctx) {
+  return guarded([&] {
+    MG_REQUIRE(ctx != nullptr, "mg_context_timer_start: ctx is NULL");
+    DeviceGuard g(ctx->device);
+    for (hipEvent_t& e : ctx->timer)
+      if (!e) MG_HIP(hipEventCreate(&e));
+    MG_HIP(hipEventRecord(ctx->timer[0], ctx->stream));
+  });
+}
+
+extern "C" int mg_context_timer_stop(mg_context* ctx, double* ms) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && ms, "mg_context_timer_stop: NULL argument");
+    MG_REQUIRE(ctx->timer[0] && ctx->timer[1], "mg_context_timer_stop: timer not started");
+    DeviceGuard g(ctx->device);
+    MG_HIP(hipEventRecord(ctx->timer[1], ctx->stream));
+    MG_HIP(hipEventSynchronize(ctx->timer[1]));
+    float t = 0;
+    MG_HIP(hipEventElapsedTime(&t, ctx->timer[0], ctx->timer[1]));
+    *ms = t;
   });
 }

@@ -102,6 +102,45 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Stream-ordered temporary from the device's default memory pool: hipMallocAsync and
+// hipFreeAsync on one stream (the caller's context stream). Kernels queued on that stream
+// before the buffer goes out of scope finish before its memory is reused, so a call can
+// return without synchronising just to free its temporaries. mg_context_create keeps freed
+// memory in the pool (release threshold: never), so repeated calls reuse it instead of
+// going back to the driver. Counterpart of DeviceOwnedBuffer::Init (device_buffer.cuh:92-106)
+// for temporaries; long-lived results keep DevBuf.
+template <class T>
+class StreamBuf {
+ public:
+  StreamBuf() = default;
+  StreamBuf(size_t n, hipStream_t s) { alloc(n, s); }
+  ~StreamBuf() { release(); }
+  // At least one element, so get() is never null.
+  void alloc(size_t n, hipStream_t s) {
+    release();
+    s_ = s;
+    if (n == 0) n = 1;
+    MG_HIP(hipMallocAsync(reinterpret_cast<void**>(&p_), n * sizeof(T), s_));
+    n_ = n;
+  }
+  void release() {
+    if (p_) (void)hipFreeAsync(p_, s_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  StreamBuf(const StreamBuf&) = delete;
+  StreamBuf& operator=(const StreamBuf&) = delete;
+  StreamBuf(StreamBuf&& o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  size_t bytes() const { return n_ * sizeof(T); }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+  hipStream_t s_ = nullptr;
+};
+
 // Pinned host staging buffer (hipHostMalloc; never hipHostRegister).
 template <class T>
 class PinnedBuf {
@@ -144,9 +183,10 @@ struct mg_context {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   mgamd::DevBuf<unsigned long long> counters;  // small scratch for reductions (host-synchronised calls)
+  hipEvent_t timer[2] = {nullptr, nullptr};    // mg_context_timer_start / _stop
   // Per-call device temporaries of asynchronous calls on `stream` (calls on one context run
-  // in stream order, so the next call may reuse them); grown by ctx_scratch.
-  mgamd::DevBuf<uint64_t> scratch;
+  // in stream order, so the next call may reuse them); grown by ctx_scratch, stream-ordered.
+  mgamd::StreamBuf<uint64_t> scratch;
 };
 
 struct mg_tiled_matrix {
@@ -172,10 +212,8 @@ namespace mgamd {
 // At least n words of ctx->scratch; growing it first waits for the work queued on the
 // context's stream, which may still read the old buffer.
 inline uint64_t* ctx_scratch(mg_context* ctx, size_t n) {
-  if (ctx->scratch.size() < n) {
-    MG_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->scratch.alloc(n);
-  }
+  // The old buffer is freed in stream order: work already queued that reads it finishes first.
+  if (ctx->scratch.size() < n) ctx->scratch.alloc(n, ctx->stream);
   return ctx->scratch.get();
 }
 

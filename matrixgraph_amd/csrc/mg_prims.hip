@@ -66,7 +66,7 @@ void exclusive_scan_u32_on(hipStream_t st, const uint32_t* in, uint32_t* out, ui
   // is avoided by scanning n items then adding the last count on device.
   size_t temp_bytes = 0;
   MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, in, out + 1, n, st));
-  DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
+  StreamBuf<uint8_t> temp(temp_bytes, st);
   MG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), st));
   if (n) MG_HIP(hipcub::DeviceScan::InclusiveSum(temp.get(), temp_bytes, in, out + 1, n, st));
 }
@@ -83,19 +83,19 @@ uint64_t sort_unique_u64(mg_context* ctx, DevBuf<uint64_t>& keys, uint64_t n, in
   hipcub::DoubleBuffer<uint64_t> db(keys.get(), alt.get());
   size_t temp_bytes = 0;
   MG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, db, ni, 0, end_bit, ctx->stream));
-  DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
+  StreamBuf<uint8_t> temp(temp_bytes, ctx->stream);
   MG_HIP(hipcub::DeviceRadixSort::SortKeys(temp.get(), temp_bytes, db, ni, 0, end_bit, ctx->stream));
   uint64_t* sorted = db.Current();
   uint64_t* other = (sorted == keys.get()) ? alt.get() : keys.get();
   // Unique: flag first occurrences, then select.
-  DevBuf<uint32_t> keep(n);
+  StreamBuf<uint32_t> keep(n, ctx->stream);
   mark_dups_kernel<<<grid_for(n, 256), 256, 0, ctx->stream>>>(sorted, n, keep.get());
   MG_LAUNCH_CHECK();
-  DevBuf<int> n_sel(1);
+  StreamBuf<int> n_sel(1, ctx->stream);
   size_t temp2 = 0;
   MG_HIP(hipcub::DeviceSelect::Flagged(nullptr, temp2, sorted, keep.get(), other, n_sel.get(), ni,
                                        ctx->stream));
-  DevBuf<uint8_t> temp_sel(temp2 ? temp2 : 1);
+  StreamBuf<uint8_t> temp_sel(temp2, ctx->stream);
   MG_HIP(hipcub::DeviceSelect::Flagged(temp_sel.get(), temp2, sorted, keep.get(), other, n_sel.get(),
                                        ni, ctx->stream));
   int h = 0;

@@ -601,6 +601,14 @@ __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
   }
 }
 
+__global__ void __launch_bounds__(kBlock) sum_u32_to_u64_kernel(const uint32_t* __restrict__ v, uint32_t n,
+                                                                unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) local += v[i];
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+}
+
 __global__ void __launch_bounds__(kBlock) sum_u64_kernel(const unsigned long long* __restrict__ v, uint32_t n,
                                                          unsigned long long* __restrict__ out) {
   uint64_t local = 0;
@@ -667,8 +675,8 @@ void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_
 // products of C = A*B, C's row pointer, and the totals.
 struct Structural {
   uint32_t n = 0;  // n_strips
-  DevBuf<uint32_t> off_b, col_b, cnt, off_c;
-  DevBuf<unsigned long long> tp;
+  StreamBuf<uint32_t> off_b, col_b, cnt, off_c;
+  StreamBuf<unsigned long long> tp;
   uint64_t n_c = 0, tile_products = 0, right_used = 0;
 };
 
@@ -680,9 +688,9 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
   if (n_words > kMaxLdsBitmapWords) fail(MG_ERR_UNSUPPORTED, std::string(who) + ": n_strips > 524288 not supported");
   hipStream_t s = ctx->stream;
   S.n = n;
-  DevBuf<uint32_t> hist(n), cursor(n);
-  S.off_b.alloc(n + 1ull);
-  S.col_b.alloc(bt->n_nz_tile ? bt->n_nz_tile : 1);
+  StreamBuf<uint32_t> hist(n, s), cursor(n, s);
+  S.off_b.alloc(n + 1ull, s);
+  S.col_b.alloc(bt->n_nz_tile, s);
   MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * n, s));
   MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * n, s));
   if (bt->n_nz_tile) {
@@ -696,26 +704,25 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
                                                                     S.col_b.get());
     MG_LAUNCH_CHECK();
   }
-  S.cnt.alloc(n);
-  S.off_c.alloc(n + 1ull);
-  S.tp.alloc(n);
-  DevBuf<unsigned long long> sums(2);
-  MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 2, s));
+  S.cnt.alloc(n, s);
+  S.off_c.alloc(n + 1ull, s);
+  S.tp.alloc(n, s);
+  StreamBuf<unsigned long long> sums(3, s);  // tile products, right tiles used, C tiles
+  MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 3, s));
   const size_t lds = sizeof(uint64_t) * n_words;
   const unsigned grid = grid_for(n, 1, 16384);
   symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
                                                        S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
                                                        nullptr);
   MG_LAUNCH_CHECK();
-  // C's tile index is 32-bit (the reference's n_nz_tile is u32): bound the 64-bit total
-  // before the 32-bit scan.
-  const uint64_t n_c64 = device_sum_u32(ctx, S.cnt.get(), n);
-  MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, std::string(who) + ": " + std::to_string(n_c64) +
-                                         " output tiles, more than a u32 tile index holds");
+  // C's tile index is 32-bit (the reference's n_nz_tile is u32): the 64-bit total is read with
+  // the other sums and checked before anything uses the 32-bit scan.
+  sum_u32_to_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(S.cnt.get(), n, sums.get() + 2);
+  MG_LAUNCH_CHECK();
   exclusive_scan_u32(ctx, S.cnt.get(), S.off_c.get(), n);
   sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(S.tp.get(), n, sums.get());
   MG_LAUNCH_CHECK();
-  DevBuf<uint8_t> ref(n);
+  StreamBuf<uint8_t> ref(n, s);
   MG_HIP(hipMemsetAsync(ref.get(), 0, n, s));
   if (a->n_nz_tile) {
     mark_referenced_kernel<<<grid_for(a->n_nz_tile, 256), 256, 0, s>>>(a->col_idx.get(), a->n_nz_tile, ref.get());
@@ -723,9 +730,12 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
   }
   used_right_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(ref.get(), S.off_b.get(), n, sums.get() + 1);
   MG_LAUNCH_CHECK();
-  unsigned long long h_sums[2] = {0, 0};
+  unsigned long long h_sums[3] = {0, 0, 0};
   MG_HIP(hipMemcpyAsync(h_sums, sums.get(), sizeof(h_sums), hipMemcpyDeviceToHost, s));
-  MG_HIP(hipStreamSynchronize(s));  // the temporaries here are freed on return
+  MG_HIP(hipStreamSynchronize(s));  // the one synchronisation of the pass: the totals
+  const uint64_t n_c64 = h_sums[2];
+  MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, std::string(who) + ": " + std::to_string(n_c64) +
+                                         " output tiles, more than a u32 tile index holds");
   S.n_c = n_c64;
   S.tile_products = h_sums[0];
   S.right_used = h_sums[1];
@@ -906,7 +916,7 @@ extern "C" int mg_init_bit_tiled_matrix_metadata_by_layout_matrix(void* stream, 
     hipStream_t s = static_cast<hipStream_t>(stream);
     // Temporary row counts and their exclusive scan (the reference computes ranks
     // with an O(words) prefix popcount per bit instead, matrix_operations.cu:149-174).
-    DevBuf<uint32_t> cnt(n_strips), start(n_strips + 1ull);
+    StreamBuf<uint32_t> cnt(n_strips, s), start(n_strips + 1ull, s);
     layout_row_count_kernel<<<grid_for(n_strips, kWavesPerBlock, 65536), kBlock, 0, s>>>(layout_matrix, n_strips,
                                                                                           cnt.get());
     MG_LAUNCH_CHECK();
@@ -918,7 +928,6 @@ extern "C" int mg_init_bit_tiled_matrix_metadata_by_layout_matrix(void* stream, 
     MG_LAUNCH_CHECK();
     add_counts_kernel<<<grid_for(n_strips, 256), 256, 0, s>>>(tile_count_row, cnt.get(), n_strips);
     MG_LAUNCH_CHECK();
-    MG_HIP(hipStreamSynchronize(s));  // temporaries are freed on return
   });
 }
 
@@ -963,7 +972,6 @@ extern "C" int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, c
       MG_HIP(hipMemsetAsync(c->data.get(), 0, c->data.bytes(), s));
       structural_emit(ctx, a, S, c->row_idx.get(), c->col_idx.get());
     }
-    MG_HIP(hipStreamSynchronize(s));
     if (stats) {
       stats->tile_products = S.tile_products;
       stats->right_tiles_used = S.right_used;
@@ -993,7 +1001,7 @@ namespace {
 // Structural index, operand payloads and features of a pruned product.
 struct PrunedPlan {
   Structural S;
-  DevBuf<uint32_t> row_s, col_s;
+  StreamBuf<uint32_t> row_s, col_s;
   WidePayload* pl = nullptr;
 };
 
@@ -1002,8 +1010,8 @@ void pruned_prepare(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
   hipStream_t s = ctx->stream;
   structural_pass(ctx, a, bt, P.S, who);
   const uint32_t n_s = static_cast<uint32_t>(P.S.n_c);
-  P.row_s.alloc(std::max<uint32_t>(1, n_s));
-  P.col_s.alloc(std::max<uint32_t>(1, n_s));
+  P.row_s.alloc(n_s, s);
+  P.col_s.alloc(n_s, s);
   structural_emit(ctx, a, P.S, P.row_s.get(), P.col_s.get());
   P.pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
   if (a->n_nz_tile) {
@@ -1023,8 +1031,8 @@ void row_features_into(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled
                        float* feat) {
   hipStream_t s = ctx->stream;
   const uint32_t n = S.n;
-  DevBuf<uint32_t> bits_a(std::max<uint32_t>(1, a->n_nz_tile)), bits_b(std::max<uint32_t>(1, bt->n_nz_tile));
-  DevBuf<unsigned long long> brow(std::max<uint32_t>(1, n));
+  StreamBuf<uint32_t> bits_a(a->n_nz_tile, s), bits_b(bt->n_nz_tile, s);
+  StreamBuf<unsigned long long> brow(n, s);
   MG_HIP(hipMemsetAsync(brow.get(), 0, sizeof(unsigned long long) * std::max<uint32_t>(1, n), s));
   if (a->n_nz_tile) {
     tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(), a->n_nz_tile,
@@ -1045,7 +1053,6 @@ void row_features_into(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled
         n, a->offset_row.get(), a->col_idx.get(), bits_a.get(), brow.get(), S.cnt.get(), S.tp.get(), feat);
     MG_LAUNCH_CHECK();
   }
-  MG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
 }
 
 // Output slots from a prediction: per row min(structural, ceil(structural * clamp(y) *
@@ -1081,7 +1088,7 @@ void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
   uint64_t cap = capacity == MG_PRUNED_STRUCTURAL ? n_s : std::min<uint64_t>(capacity, n_s);
   if (capacity == MG_PRUNED_EXACT || n_s == 0) {
     // Exact: flag pass, scan, then each non-empty tile written at its final position.
-    DevBuf<uint32_t> flag(std::max<uint32_t>(1, n_s)), pos(n_s + 1ull);
+    StreamBuf<uint32_t> flag(n_s, s), pos(n_s + 1ull, s);
     PrunedOut fo{};
     fo.flag = flag.get();
     launch_pruned<kOutFlag>(ctx, a, bt, n_s, row_s, col_s, P.pl, fo);
@@ -1103,16 +1110,15 @@ void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
     st.passes = 2;
     st.capacity = n32;
     st.scratch_bytes = 0;
-    MG_HIP(hipStreamSynchronize(s));  // flag and pos are freed on return
   } else {
     // One pass into `cap` slots; a capacity below the non-empty count is found by the
     // counter, which keeps counting past it, and the pass is run again with the exact count.
-    DevBuf<unsigned long long> counter(1);
+    StreamBuf<unsigned long long> counter(1, s);
     st.capacity = cap;
     for (int attempt = 0;; ++attempt) {
       const uint64_t slots = std::max<uint64_t>(1, cap);
-      DevBuf<uint64_t> scratch(slots * kTile);
-      DevBuf<uint32_t> keys(slots);
+      StreamBuf<uint64_t> scratch(slots * kTile, s);
+      StreamBuf<uint32_t> keys(slots, s);
       MG_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned long long), s));
       PrunedOut ao{};
       ao.data = scratch.get();
@@ -1136,7 +1142,7 @@ void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
       alloc_tiled_matrix(c, S.n, n32);
       if (n32) {
         // Row-major order: sort the slots by structural index, then move the tiles.
-        DevBuf<uint32_t> slot_in(n32), keys_out(n32), slot_out(n32);
+        StreamBuf<uint32_t> slot_in(n32, s), keys_out(n32, s), slot_out(n32, s);
         iota_u32_kernel<<<grid_for(n32, kBlock), kBlock, 0, s>>>(slot_in.get(), n32);
         MG_LAUNCH_CHECK();
         int bits = 1;
@@ -1144,7 +1150,7 @@ void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
         size_t temp_bytes = 0;
         MG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.get(), keys_out.get(), slot_in.get(),
                                                   slot_out.get(), static_cast<int>(n32), 0, bits, s));
-        DevBuf<uint8_t> temp(temp_bytes ? temp_bytes : 1);
+        StreamBuf<uint8_t> temp(temp_bytes, s);
         MG_HIP(hipcub::DeviceRadixSort::SortPairs(temp.get(), temp_bytes, keys.get(), keys_out.get(), slot_in.get(),
                                                   slot_out.get(), static_cast<int>(n32), 0, bits, s));
         gather_pruned_kernel<<<grid_for(n32, kWavesPerBlock, 65536), kBlock, 0, s>>>(
@@ -1152,12 +1158,11 @@ void pruned_execute(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
             c->data.get());
         MG_LAUNCH_CHECK();
       }
-      MG_HIP(hipStreamSynchronize(s));  // scratch, keys and the sort buffers are freed next
       break;
     }
   }
   // Row pointer of the pruned C from its row index.
-  DevBuf<uint32_t> hist(std::max<uint32_t>(1, S.n));
+  StreamBuf<uint32_t> hist(S.n, s);
   MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * S.n, s));
   if (n_num) {
     row_hist_kernel<<<grid_for(n_num, kBlock), kBlock, 0, s>>>(c->row_idx.get(), static_cast<uint32_t>(n_num),
@@ -1208,8 +1213,9 @@ extern "C" int mg_multiply_pruned_model(mg_context* ctx, const mg_tiled_matrix* 
     const uint32_t n = P.S.n;
     uint64_t capacity = MG_PRUNED_EXACT;
     if (P.S.n_c) {
-      DevBuf<float> feat(static_cast<uint64_t>(n) * kRowFeatures), y(static_cast<uint64_t>(n) * n_out), pred(n);
-      DevBuf<unsigned long long> total(1);
+      StreamBuf<float> feat(static_cast<uint64_t>(n) * kRowFeatures, s), y(static_cast<uint64_t>(n) * n_out, s),
+          pred(n, s);
+      StreamBuf<unsigned long long> total(1, s);
       row_features_into(ctx, a, bt, P.S, feat.get());
       if (mg_mlp_forward(ctx, model, feat.get(), n, y.get()) != MG_OK)
         fail(MG_ERR_INTERNAL, std::string("mg_multiply_pruned_model: ") + mg_last_error());
@@ -1243,7 +1249,7 @@ extern "C" int mg_multiply_row_features(mg_context* ctx, const mg_tiled_matrix* 
     Structural S;
     structural_pass(ctx, a, bt, S, "mg_multiply_row_features");
     const uint32_t n = S.n;
-    DevBuf<float> feat(std::max<uint64_t>(1, static_cast<uint64_t>(n) * kRowFeatures));
+    StreamBuf<float> feat(static_cast<uint64_t>(n) * kRowFeatures, s);
     row_features_into(ctx, a, bt, S, feat.get());
     if (n) {
       MG_HIP(hipMemcpyAsync(features, feat.get(), sizeof(float) * n * kRowFeatures, hipMemcpyDeviceToHost, s));
@@ -1284,7 +1290,7 @@ extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix*
                "mg_multiply_numeric_dense: operands differ in n_strips");
     DeviceGuard g(ctx->device);
     hipStream_t st = ctx->stream;
-    DevBuf<uint32_t> ba(std::max<uint32_t>(1, a->n_nz_tile)), bb(std::max<uint32_t>(1, bt->n_nz_tile));
+    StreamBuf<uint32_t> ba(a->n_nz_tile, st), bb(bt->n_nz_tile, st);
     if (a->n_nz_tile) {
       tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
                                                                                          ba.get());
