@@ -432,6 +432,64 @@ def prealloc_section(M, ctx, train_specs, train_graphs, n_train, n_eval):
     }
 
 
+def mfma_section(M, ctx, densities=(0.01, 0.03, 0.1, 0.3), reps=3):
+    """Dense tile pairs on the matrix cores against the vector ALUs: C = A * A of 64 planted
+    communities of 4096 vertices (community c = vertices [16384c, 16384c + 4096) of 2^20) at
+    edge probability p, so every C tile of a community sums 64 pairs of tiles holding about
+    4096 p bits each (16.8 M pairs). Per p: the numeric pass with every pair on the matrix cores
+    (min_bits 0) and with none (min_bits 4097), bit-exact against each other; the crossover
+    density sets where the dispatch threshold pays."""
+    from matrixgraph_amd import workloads as WL
+
+    n = 1 << 20
+    out = []
+    for p in densities:
+        num = int(round(p * 2**32))
+        el = M.EdgeList.communities(ctx, n, 64, 4096, 16384, num, 32, 3)
+        a = el.to_tiled_matrix(transpose=False)
+        at = el.to_tiled_matrix(transpose=True)
+        _, m = el.info()
+        el.close()
+        row = {"p": p, "edges": m, "tiles": a.n_nz_tile}
+        digests = {}
+        for mb in (4097, 0):
+            times = []
+            for r in range(reps + 1):
+                c, st = M.multiply_symbolic(ctx, a, at)
+                ctx.synchronize()
+                ctx.timer_start()
+                dense = M.multiply_numeric_dense(ctx, a, at, c, mb)
+                ms = ctx.timer_stop()
+                if r:
+                    times.append(ms)
+                if r == reps:
+                    digests[mb] = c.digest()
+                c.close()
+            key = "vector" if mb > 4096 else "matrix_cores"
+            row[key + "_ms"] = round(float(np.median(times)), 3)
+            row[key + "_products"] = dense
+            row["tile_products"] = st["tile_products"]
+        row["matrix_cores_vs_vector"] = round(row["vector_ms"] / row["matrix_cores_ms"], 3)
+        row["int8_ops_per_s_matrix_cores"] = round(row["matrix_cores_products"] * 2 * 64**3
+                                                   / (row["matrix_cores_ms"] * 1e-3), 1)
+        row["bit_exact"] = digests[4097] == digests[0]
+        if not row["bit_exact"]:
+            raise SystemExit("mfma: matrix-core product differs from the vector product")
+        log(f"mfma p={p}: vector {row['vector_ms']} ms, matrix cores {row['matrix_cores_ms']} ms "
+            f"({row['matrix_cores_vs_vector']}x), {row['tile_products']} pairs")
+        out.append(row)
+        a.close()
+        at.close()
+    wins = [r["p"] for r in out if r["matrix_cores_vs_vector"] > 1.0]
+    return {
+        "workload": "C = A*A, A = 64 communities of 4096 vertices (stride 16384, 2^20 vertices) at edge "
+                    "probability p, seed 3; numeric pass only (HIP events), symbolic pass excluded",
+        "per_density": out,
+        "matrix_cores_win_from_p": min(wins) if wins else None,
+        "int8_peak_ops_per_s": 5.0e15,
+    }
+
+
 def dense_section(M, ctx, thresholds=(4097, 2048, 1024, 512, 0, None)):
     """Density-mixed 3-hop walk: R-MAT scale 20 (edge factor 16, seed 3) union 64 planted
     communities of 4096 vertices (community c = vertices [16384c, 16384c + 4096), p = 0.3),
@@ -516,6 +574,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
     ap.add_argument("--no-dense", action="store_true", help="skip the density-mixed (matrix-core) section")
+    ap.add_argument("--mfma-only", action="store_true",
+                    help="only the matrix-core community A*A section as its own JSON line")
     ap.add_argument("--no-prealloc", action="store_true", help="skip the learned pre-allocation section")
     ap.add_argument("--prealloc-train", type=int, default=384, help="training queries of the pre-allocation model")
     ap.add_argument("--prealloc-eval", type=int, default=128, help="evaluated queries of the pre-allocation section")
@@ -546,6 +606,9 @@ def main():
         return
     if args.square_only:
         print(json.dumps({"square": square_section(M, ctx)}), flush=True)
+        return
+    if args.mfma_only:
+        print(json.dumps({"mfma": mfma_section(M, ctx)}), flush=True)
         return
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -698,6 +761,7 @@ def main():
         result["square"] = square_section(M, ctx)
     if not args.no_dense:
         result["dense"] = dense_section(M, ctx)
+        result["mfma"] = mfma_section(M, ctx)
     if not args.no_batch:
         w.close()
         result["batch"], specs, graphs, edges, best = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)

@@ -247,10 +247,13 @@ int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_til
 int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                         mg_tiled_matrix* c);
 /* The numeric pass with a per-tile-pair choice of engine: a pair whose two tiles both have
- * at least min_bits set bits is multiplied on the matrix cores (int8 0/1 products, int32
- * counts, then > 0); every other pair on the vector ALUs. min_bits = 0 sends every pair to
- * the matrix cores, min_bits > 4096 none. *dense_products (may be NULL) receives the number
- * of matrix-core tile products. Same result as mg_multiply_numeric. Synchronises.
+ * at least min_bits set bits is multiplied on the matrix cores; every other pair on the
+ * vector ALUs. The dense tiles of each operand are expanded once to one byte per bit (4 KiB
+ * per tile, at most 32 GiB per operand), the vector pass records the dense pairs of every C
+ * tile, and a second kernel sums each C tile's dense pairs in int32 (int8 0/1 products) and
+ * turns counts > 0 into bits once. min_bits = 0 sends every pair to the matrix cores,
+ * min_bits > 4096 none. *dense_products (may be NULL) receives the number of matrix-core tile
+ * products. Same result as mg_multiply_numeric. Synchronises.
  * No counterpart in the reference (its product is VALU only). */
 int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                               mg_tiled_matrix* c, uint32_t min_bits, uint64_t* dense_products);

@@ -41,52 +41,113 @@ __device__ inline uint64_t tile_product(uint64_t l_row, uint64_t rt_row) {
   return out;
 }
 
-// Dense tile pairs on the matrix cores. Out(r, c) = OR_k L(r, k) AND R(k, c) with
-// R(k, c) = bit k of Rt[c], computed as int8 products of 0/1 bytes accumulated in int32
-// (v_mfma_i32_32x32x16_i8: a 32 x 32 output block per instruction, K = 16), then count > 0:
-// four 32 x 32 blocks, four K-steps each. Operand lane layout (32x32x16 i8): lane l holds
-// A[l % 32][8 * (l / 32) + j] and B[8 * (l / 32) + j][l % 32], j = 0..7; result register v of
-// lane l is C[8 * (v / 4) + 4 * (l / 32) + v % 4][l % 32].
+// Dense tile pairs on the matrix cores, as one contraction per C tile: for every pair
+// (A(x,k), Bt(y,k)) whose two tiles both hold at least min_bits set bits, the vector kernel
+// records the pair (its two tile indices) instead of multiplying it; then
+// dense_contract_kernel sums, per C tile, the int8 0/1 products of all its recorded pairs in
+// int32 (v_mfma_i32_32x32x32_i8, four 32 x 32 output blocks, two K-steps of 32 per pair) and
+// turns counts > 0 into bit rows once, ORed into C. Operands are expanded from the bit tiles
+// in registers: a lane loads one 64-bit row of each tile (A row r, Bt row c = B column c) and
+// turns 4 bits at a time into 4 bytes of 0/1 with a 24-bit multiply.
+// Lane layout (32x32x32 i8): lane l holds 16 bytes of A row l % 32 and of B column l % 32,
+// at k = 16 * (l / 32) .. +15 of the K-step (any k order works if both operands use it);
+// result register v of lane l is C[8 * (v / 4) + 4 * (l / 32) + v % 4][l % 32].
 typedef int mg_v16i __attribute__((ext_vector_type(16)));
+typedef int mg_v4i __attribute__((ext_vector_type(4)));
 
-// Bits k0..k0+7 of x as eight bytes of value 0 or 1 (byte j = bit k0 + j).
-__device__ inline long bits_to_bytes(uint64_t x, uint32_t k0) {
-  uint64_t b = (x >> k0) & 0xFFull;
-  b *= 0x0101010101010101ull;   // the byte in every byte position
-  b &= 0x8040201008040201ull;   // byte j keeps bit j
-  b = (b + 0x7F7F7F7F7F7F7F7Full) & 0x8080808080808080ull;  // non-zero byte -> 0x80
-  return static_cast<long>(b >> 7);
+// Bits 0..15 of x as 16 bytes of value 0 or 1 (byte j = bit j): nibble * 0x204081 puts bit
+// i of the nibble at bit 8i (the four shifted copies do not overlap), the mask keeps those.
+__device__ inline mg_v4i bits16_to_bytes(uint32_t x) {
+  mg_v4i out;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    out[q] = static_cast<int>(__umul24((x >> (4 * q)) & 0xFu, 0x204081u) & 0x01010101u);
+  return out;
 }
 
-// Same contract as tile_product: lane r holds L[r] and Rt[r]; returns output row `lane`.
-// Kept out of line: inlined into the pair loops its unrolled blocks would hold registers
-// across the whole kernel.
-__device__ __attribute__((noinline)) uint64_t tile_product_mfma(uint64_t l_row, uint64_t rt_row, uint32_t lane) {
+// Recorded dense pairs of one C tile (from one 64-tile step of its index intersection).
+struct DenseTask {
+  uint32_t tile, base, n;
+};
+struct DenseRecord {
+  uint2* pairs;  // (A tile, Bt tile)
+  DenseTask* tasks;
+  unsigned long long* counter;  // pairs recorded (low 32 bits) and tasks (high 32 bits)
+  uint32_t cap;            // capacity of pairs and of tasks (the exact dense pair count)
+  const uint32_t* bits_a;  // set bits per tile
+  const uint32_t* bits_bt;
+  uint32_t min_bits;
+};
+
+// One wave per recorded task: int32 counts of the 64 x 64 output over all the task's pairs
+// on the matrix cores, then one threshold; the rows are ORed into C atomically (several
+// tasks may share a C tile).
+__global__ void __launch_bounds__(kBlock) dense_contract_kernel(const DenseTask* __restrict__ tasks,
+                                                                const unsigned long long* __restrict__ counter,
+                                                                uint32_t cap, const uint2* __restrict__ pairs,
+                                                                const uint64_t* __restrict__ data_a,
+                                                                const uint64_t* __restrict__ data_bt,
+                                                                uint64_t* __restrict__ data_c) {
+  const uint32_t lane = lane_id();
   const uint32_t lo = lane & 31, hi = lane >> 5;
-  uint64_t out = 0;
+  const uint32_t n_tasks = min(static_cast<uint32_t>(*counter >> 32), cap);
+  const uint32_t n_waves = gridDim.x * kWavesPerBlock;
+  for (uint32_t t = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); t < n_tasks; t += n_waves) {
+    const DenseTask T = tasks[t];
+    mg_v16i acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};
+    const uint32_t p_end = min(T.base + T.n, cap);
+    for (uint32_t p = T.base; p < p_end; ++p) {
+      const uint2 pr = pairs[p];
+      const uint64_t* A = data_a + static_cast<uint64_t>(pr.x) * kTile;
+      const uint64_t* B = data_bt + static_cast<uint64_t>(pr.y) * kTile;
+      const uint64_t ra0 = A[lo], ra1 = A[32 + lo], rb0 = B[lo], rb1 = B[32 + lo];
 #pragma unroll
-  for (uint32_t mb = 0; mb < 2; ++mb) {
-    const uint64_t a_src = shfl64(l_row, mb * 32 + lo);
-#pragma unroll
-    for (uint32_t nb = 0; nb < 2; ++nb) {
-      const uint64_t b_src = shfl64(rt_row, nb * 32 + lo);
-      mg_v16i acc = {};
-#pragma unroll
-      for (uint32_t st = 0; st < 4; ++st) {
-        const uint32_t k0 = 16 * st + 8 * hi;
-        acc = __builtin_amdgcn_mfma_i32_32x32x16_i8(bits_to_bytes(a_src, k0), bits_to_bytes(b_src, k0), acc, 0, 0,
-                                                     0);
-      }
-#pragma unroll
-      for (uint32_t v = 0; v < 16; ++v) {
-        const uint64_t bal = ballot(acc[v] > 0);
-        const uint32_t i0 = mb * 32 + 8 * (v / 4) + (v % 4);
-        if (lane == i0) out |= (bal & 0xFFFFFFFFull) << (nb * 32);
-        if (lane == i0 + 4) out |= (bal >> 32) << (nb * 32);
+      for (uint32_t st = 0; st < 2; ++st) {
+        const uint32_t k = 32 * st + 16 * hi;
+        const mg_v4i a0 = bits16_to_bytes(static_cast<uint32_t>(ra0 >> k));
+        const mg_v4i a1 = bits16_to_bytes(static_cast<uint32_t>(ra1 >> k));
+        const mg_v4i b0 = bits16_to_bytes(static_cast<uint32_t>(rb0 >> k));
+        const mg_v4i b1 = bits16_to_bytes(static_cast<uint32_t>(rb1 >> k));
+        acc00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, acc11, 0, 0, 0);
       }
     }
+    uint64_t out = 0;  // output row `lane`
+#pragma unroll
+    for (uint32_t v = 0; v < 16; ++v) {
+      const uint32_t i0 = 8 * (v / 4) + (v % 4);
+      const uint64_t b00 = ballot(acc00[v] > 0), b01 = ballot(acc01[v] > 0);
+      const uint64_t b10 = ballot(acc10[v] > 0), b11 = ballot(acc11[v] > 0);
+      if (lane == i0) out |= (b00 & 0xFFFFFFFFull) | ((b01 & 0xFFFFFFFFull) << 32);
+      if (lane == i0 + 4) out |= (b00 >> 32) | ((b01 >> 32) << 32);
+      if (lane == 32 + i0) out |= (b10 & 0xFFFFFFFFull) | ((b11 & 0xFFFFFFFFull) << 32);
+      if (lane == 32 + i0 + 4) out |= (b10 >> 32) | ((b11 >> 32) << 32);
+    }
+    if (out) atomicOr(reinterpret_cast<unsigned long long*>(data_c) + static_cast<uint64_t>(T.tile) * kTile + lane,
+                      static_cast<unsigned long long>(out));
   }
-  return out;
+}
+
+// Per Bt column k: the dense Bt tiles in it; then the dense pairs of the product: sum over
+// dense A tiles (x, k) of that count (the structural pair count restricted to dense tiles).
+__global__ void __launch_bounds__(kBlock) dense_col_count_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                                 const uint32_t* __restrict__ bits,
+                                                                 uint32_t min_bits, uint32_t* __restrict__ count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (bits[i] >= min_bits) atomicAdd(count + col[i], 1u);
+}
+__global__ void __launch_bounds__(kBlock) dense_pair_count_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                                  const uint32_t* __restrict__ bits,
+                                                                  uint32_t min_bits,
+                                                                  const uint32_t* __restrict__ count,
+                                                                  unsigned long long* __restrict__ out) {
+  uint64_t local = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (bits[i] >= min_bits) local += count[col[i]];
+  local = wave_sum(local);
+  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
 }
 
 // Set bits of every tile (for the dense-pair dispatch).
@@ -211,8 +272,8 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
 // tiles is multiplied by that lane alone: every entry (r, j) of the A tile meets every entry
 // (c, j) of the Bt tile with the same j and sets bit c of row r in the wave's LDS
 // accumulator. Pairs with a denser tile take the full-tile wave product, kPairBatch at a time.
-// kMfma: a pair whose two tiles both have at least min_bits set bits (bits_a, bits_bt) is
-// taken first, by the matrix cores, one pair at a time; dense_count counts those pairs.
+// kMfma: a pair whose two tiles both hold at least rec.min_bits set bits is recorded for
+// dense_contract_kernel instead.
 // kOut: where a finished C tile goes (PrunedOut): ORed into data_c at its structural index
 // (kOutStruct), or, for a pruned product, only if it is non-zero: appended at the next slot
 // of out.data with its structural index in out.key (kOutAppend, slots past out.capacity are
@@ -233,11 +294,9 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
     const uint32_t* __restrict__ row_c, const uint32_t* __restrict__ col_c, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ col_bt, const uint64_t* __restrict__ data_a, const uint64_t* __restrict__ data_bt,
     const WidePayload* __restrict__ pl_a, const WidePayload* __restrict__ pl_bt, uint64_t* __restrict__ data_c,
-    const uint32_t* __restrict__ bits_a, const uint32_t* __restrict__ bits_bt, uint32_t min_bits,
-    unsigned long long* __restrict__ dense_count, PrunedOut out = PrunedOut{}) {
+    DenseRecord rec, PrunedOut out = PrunedOut{}) {
   __shared__ unsigned long long s_acc[kWavesPerBlock][kTile];
   const uint32_t lane = lane_id();
-  uint64_t n_mfma = 0;
   unsigned long long* const acc_l = s_acc[threadIdx.x / kWave];
   acc_l[lane] = 0;
   wave_lds_sync();
@@ -260,14 +319,15 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
       const bool has = pos >= 0;
       const uint32_t ia = has ? (a_short ? j : static_cast<uint32_t>(pos)) : 0u;
       const uint32_t ib = has ? (a_short ? static_cast<uint32_t>(pos) : j) : 0u;
+      bool mfma = false;
+      if constexpr (kMfma) mfma = has && rec.bits_a[ia] >= rec.min_bits && rec.bits_bt[ib] >= rec.min_bits;
+      // Payloads only for the pairs the vector ALUs multiply.
       WidePayload pa = {0, 0}, pb = {0, 0};
-      if (has) {
+      if (has && !mfma) {
         pa = pl_a[ia];
         pb = pl_bt[ib];
       }
       const uint32_t na = wide_count(pa), nb = wide_count(pb);
-      bool mfma = false;
-      if constexpr (kMfma) mfma = has && bits_a[ia] >= min_bits && bits_bt[ib] >= min_bits;
       const bool dense = has && !mfma && (na == kWideDense || nb == kWideDense);
       if (has && !mfma && !dense) {
         for (uint32_t u = 0; u < na; ++u) {
@@ -279,12 +339,21 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
         }
       }
       if constexpr (kMfma) {
-        for (uint64_t mm = ballot(mfma); mm; mm &= mm - 1) {
-          const int k = __ffsll(static_cast<unsigned long long>(mm)) - 1;
-          const uint64_t ka = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ia), k));
-          const uint64_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ib), k));
-          acc |= tile_product_mfma(data_a[ka * kTile + lane], data_bt[kb * kTile + lane], lane);
-          ++n_mfma;
+        // Record this step's dense pairs as one task of C tile i (dense_contract_kernel).
+        const uint64_t mm = ballot(mfma);
+        if (mm) {
+          const uint32_t n = static_cast<uint32_t>(__popcll(mm));
+          // One atomic reserves both the pair positions and the task slot.
+          unsigned long long both = 0;
+          if (lane == 0) both = atomicAdd(rec.counter, (1ull << 32) | n);
+          both = read_first_lane64(both);
+          const uint32_t base = static_cast<uint32_t>(both);
+          const uint32_t task = static_cast<uint32_t>(both >> 32);
+          if (mfma) {
+            const uint32_t at = base + static_cast<uint32_t>(__popcll(mm & ((1ull << lane) - 1)));
+            if (at < rec.cap) rec.pairs[at] = make_uint2(ia, ib);
+          }
+          if (lane == 0 && task < rec.cap) rec.tasks[task] = DenseTask{i, base, n};
         }
       }
       // Pairs with one dense tile: the dense tile's rows (lane = row) against the other tile's
@@ -398,8 +467,6 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
       }
     }
   }
-  if (kMfma && dense_count && lane == 0 && n_mfma)
-    atomicAdd(dense_count + (blockIdx.x % 64), static_cast<unsigned long long>(n_mfma));
 }
 
 // len (<= 64) bits of a bit-packed array starting at bit `start`.
@@ -630,8 +697,8 @@ void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const
 }
 
 // Numeric pass of the handle-level product: payloads of both operands (stream-ordered in the
-// context's scratch), then fill_tiles_payload_kernel; with bits_a/bits_bt, pairs of tiles with
-// at least min_bits bits each go to the matrix cores and are counted into dense_count.
+// context's scratch), then fill_tiles_payload_kernel; with a dense record (rec.pairs set), the
+// pairs of dense tiles are recorded there and dense_contract_kernel multiplies them after.
 // `pl` holds payload_words(a, bt) words of device scratch (the context's, or the static array
 // of the allocation-free entry point).
 size_t payload_words(const mg_tiled_matrix* a, const mg_tiled_matrix* bt) {
@@ -639,8 +706,7 @@ size_t payload_words(const mg_tiled_matrix* a, const mg_tiled_matrix* bt) {
 }
 
 void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
-                            mg_tiled_matrix* c, WidePayload* pl, const uint32_t* bits_a, const uint32_t* bits_bt,
-                            uint32_t min_bits, unsigned long long* dense_count) {
+                            mg_tiled_matrix* c, WidePayload* pl, const DenseRecord& rec) {
   if (c->n_nz_tile == 0) return;
   hipStream_t st = ctx->stream;
   WidePayload* pl_a = pl;
@@ -656,16 +722,14 @@ void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_
     MG_LAUNCH_CHECK();
   }
   const uint32_t grid = grid_for(c->n_nz_tile, kWavesPerBlock, 65536);
-  if (bits_a && bits_bt)
+  if (rec.pairs)
     fill_tiles_payload_kernel<true><<<grid, kBlock, 0, st>>>(
         c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
-        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), bits_a,
-        bits_bt, min_bits, dense_count);
+        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), rec);
   else
     fill_tiles_payload_kernel<false><<<grid, kBlock, 0, st>>>(
         c->n_nz_tile, a->offset_row.get(), bt->offset_row.get(), c->row_idx.get(), c->col_idx.get(),
-        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), nullptr,
-        nullptr, 0, nullptr);
+        a->col_idx.get(), bt->col_idx.get(), a->data.get(), bt->data.get(), pl_a, pl_bt, c->data.get(), rec);
   MG_LAUNCH_CHECK();
 }
 
@@ -806,7 +870,7 @@ void launch_pruned(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_mat
   if (!n_s) return;
   fill_tiles_payload_kernel<false, kOut><<<grid_for(n_s, kWavesPerBlock, 65536), kBlock, 0, ctx->stream>>>(
       n_s, a->offset_row.get(), bt->offset_row.get(), row_s, col_s, a->col_idx.get(), bt->col_idx.get(),
-      a->data.get(), bt->data.get(), pl, pl + a->n_nz_tile, nullptr, nullptr, nullptr, 0, nullptr, out);
+      a->data.get(), bt->data.get(), pl, pl + a->n_nz_tile, nullptr, DenseRecord{}, out);
   MG_LAUNCH_CHECK();
 }
 
@@ -991,7 +1055,7 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     DeviceGuard g(ctx->device);
     if (c->n_nz_tile == 0) return;
     WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
-    launch_payload_product(ctx, a, bt, c, pl, nullptr, nullptr, 0, nullptr);
+    launch_payload_product(ctx, a, bt, c, pl, DenseRecord{});
   });
 }
 
@@ -1278,7 +1342,7 @@ extern "C" int mg_multiply_numeric_noalloc(mg_context* ctx, const mg_tiled_matri
     // The device address of this device's copy of `scratch`, looked up when the context was
     // created; the kernels write every payload word they read, so nothing is cleared.
     WidePayload* pl = reinterpret_cast<WidePayload*>(noalloc_scratch(ctx->device));
-    launch_payload_product(ctx, a, bt, c, pl, nullptr, nullptr, 0, nullptr);
+    launch_payload_product(ctx, a, bt, c, pl, DenseRecord{});
   });
 }
 
@@ -1301,17 +1365,43 @@ extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix*
                                                                                           bt->n_nz_tile, bb.get());
       MG_LAUNCH_CHECK();
     }
-    unsigned long long* cnt = ctx->counters.get();  // 64 slots
-    MG_HIP(hipMemsetAsync(cnt, 0, 64 * sizeof(unsigned long long), st));
-    WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
-    launch_payload_product(ctx, a, bt, c, pl, ba.get(), bb.get(), min_bits, cnt);
-    unsigned long long h[64];
-    MG_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
-    MG_HIP(hipStreamSynchronize(st));
-    if (dense_products) {
-      uint64_t sum = 0;
-      for (unsigned long long v : h) sum += v;
-      *dense_products = sum;
+    // Exact number of dense pairs: the capacity of the pair and task lists.
+    uint64_t n_pairs = 0;
+    if (a->n_nz_tile && bt->n_nz_tile) {
+      StreamBuf<uint32_t> colcount(a->n_strips, st);
+      StreamBuf<unsigned long long> total(1, st);
+      MG_HIP(hipMemsetAsync(colcount.get(), 0, sizeof(uint32_t) * a->n_strips, st));
+      MG_HIP(hipMemsetAsync(total.get(), 0, sizeof(unsigned long long), st));
+      dense_col_count_kernel<<<grid_for(bt->n_nz_tile, kBlock), kBlock, 0, st>>>(bt->col_idx.get(), bt->n_nz_tile,
+                                                                                bb.get(), min_bits, colcount.get());
+      MG_LAUNCH_CHECK();
+      dense_pair_count_kernel<<<grid_for(a->n_nz_tile, kBlock, 4096), kBlock, 0, st>>>(
+          a->col_idx.get(), a->n_nz_tile, ba.get(), min_bits, colcount.get(), total.get());
+      MG_LAUNCH_CHECK();
+      unsigned long long h = 0;
+      MG_HIP(hipMemcpyAsync(&h, total.get(), sizeof(h), hipMemcpyDeviceToHost, st));
+      MG_HIP(hipStreamSynchronize(st));
+      n_pairs = h;
     }
+    MG_REQUIRE(n_pairs < (1ull << 31), "mg_multiply_numeric_dense: more than 2^31 dense tile pairs");
+    const uint32_t cap = static_cast<uint32_t>(n_pairs);
+    StreamBuf<uint2> pairs(cap, st);
+    StreamBuf<DenseTask> tasks(cap, st);
+    StreamBuf<unsigned long long> counter(1, st);
+    MG_HIP(hipMemsetAsync(counter.get(), 0, sizeof(unsigned long long), st));
+    DenseRecord rec{pairs.get(), tasks.get(), counter.get(), cap, ba.get(), bb.get(), min_bits};
+    WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
+    launch_payload_product(ctx, a, bt, c, pl, rec);
+    if (cap && c->n_nz_tile) {
+      dense_contract_kernel<<<grid_for(cap, kWavesPerBlock, 16384), kBlock, 0, st>>>(
+          tasks.get(), counter.get(), cap, pairs.get(), a->data.get(), bt->data.get(), c->data.get());
+      MG_LAUNCH_CHECK();
+    }
+    unsigned long long h = 0;
+    MG_HIP(hipMemcpyAsync(&h, counter.get(), sizeof(h), hipMemcpyDeviceToHost, st));
+    MG_HIP(hipStreamSynchronize(st));
+    const uint64_t recorded = h & 0xFFFFFFFFull;
+    MG_REQUIRE(recorded <= n_pairs, "mg_multiply_numeric_dense: more dense pairs recorded than counted");
+    if (dense_products) *dense_products = recorded;
   });
 }
