@@ -165,6 +165,7 @@ def batch_section(M, ctx, seed0: int, query_seed: int, n_queries: int, reps: int
 
     best, table = autotune.tune(ctx, graphs, queries, reps=2, log=log)
     b = M.WalkBatch(ctx, graphs, queries, packing=best["packing"], chunk_tiles=best["chunk_tiles"], use_graph=True)
+    b.set_hop_kernel(best.get("segments", 0))
     b.run()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -574,6 +575,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-query batch section")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduling section")
     ap.add_argument("--no-dense", action="store_true", help="skip the density-mixed (matrix-core) section")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the uneven-query batch section as its own JSON line")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the matrix-core community A*A section as its own JSON line")
     ap.add_argument("--no-prealloc", action="store_true", help="skip the learned pre-allocation section")
@@ -609,6 +612,10 @@ def main():
         return
     if args.mfma_only:
         print(json.dumps({"mfma": mfma_section(M, ctx)}), flush=True)
+        return
+    if args.batch_only:
+        res = batch_section(M, ctx, 10, 1, args.batch_queries, args.batch_reps)[0]
+        print(json.dumps({"batch": res}), flush=True)
         return
     ctx.synchronize()
     t0 = time.perf_counter()

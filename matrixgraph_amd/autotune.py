@@ -1,8 +1,10 @@
 """Autotuner for batched walks.
 
-Searches the launch configuration of mg_walk_batch (query packing and scheduler chunk
-size, in A^T tiles) by timing the real workload on the device: each candidate is built,
-run once to capture its HIP graph, then timed over `reps` replays; the fastest wins.
+Searches the launch configuration of mg_walk_batch by timing the real workload on the
+device: the scheduler chunk size (A^T tiles per workgroup), the hop kernel of the levels
+after the first (the tile-decoding sweep, or the job-list kernel with 2 or 4 row segments,
+mg_walk_batch_set_hop_kernel) and the query packing. Each candidate is built, run once to
+capture its HIP graph, then timed over `reps` replays; the fastest wins.
 Results are cached per workload signature (graph sizes, query sizes and hop counts) in
 memory, and optionally in a JSON file.
 """
@@ -14,11 +16,8 @@ import time
 from pathlib import Path
 
 DEFAULT_CANDIDATES = [
-    {"packing": 0, "chunk_tiles": 256},
-    {"packing": 0, "chunk_tiles": 512},
-    {"packing": 0, "chunk_tiles": 1024},
-    {"packing": 0, "chunk_tiles": 2048},
-    {"packing": 0, "chunk_tiles": 4096},
+    {"packing": 0, "chunk_tiles": ct, "segments": seg}
+    for ct in (512, 1024, 2048, 4096) for seg in (0, 2, 4)
 ]
 # MG_PACK_ONE_PER_PANEL is not a default candidate: every query then holds 4096-source
 # frontiers for each of its levels, which does not fit for hundreds of queries on large graphs.
@@ -48,6 +47,7 @@ def tune(ctx, graphs, queries, candidates=None, reps=3, cache_file: Path | None 
     table = []
     for opt in candidates or DEFAULT_CANDIDATES:
         b = M.WalkBatch(ctx, graphs, queries, packing=opt["packing"], chunk_tiles=opt["chunk_tiles"], use_graph=True)
+        b.set_hop_kernel(opt.get("segments", 0))
         b.run()
         ctx.synchronize()
         t0 = time.perf_counter()
