@@ -206,3 +206,44 @@ def test_two_hop_from_csr_tiled_grid(ctx, tmp_path, n_chunks, tile_size):
     np.fill_diagonal(c, 0)
     ws, wd = np.nonzero(c)
     assert np.array_equal(gs, ws.astype(np.uint32)) and np.array_equal(gd, wd.astype(np.uint32))
+
+
+def test_tile_to_edge_expansion_rejects_more_than_2_31_edges(ctx):
+    """524,288 full tiles hold 2^31 set bits: the expansion's 32-bit scan would wrap, so the
+    64-bit total is checked first and the call fails with an error instead of writing past
+    its allocation."""
+    M = mg()
+    ns = 725  # 725^2 = 525,625 tile positions
+    n = 1 << 19
+    rows = (np.arange(n) // ns).astype(np.uint32)
+    cols = (np.arange(n) % ns).astype(np.uint32)
+    off = np.searchsorted(rows, np.arange(ns + 1)).astype(np.uint32)
+    data = np.full((n, 64), np.uint64(0xFFFFFFFFFFFFFFFF), np.uint64)
+    m = M.TiledMatrix.upload(ctx, M.HostTiledMatrix(ns, off, rows, cols, data))
+    del data
+    with pytest.raises(M.MGError) as e:
+        m.to_edges()
+    assert "2^31" in str(e.value)
+    m.close()
+
+
+def test_symbolic_pass_rejects_more_than_u32_output_tiles(ctx):
+    """A with tile (x, 0) in every row and Bt with tile (y, 0) in every row: C = A*B has every
+    one of 65,537^2 > 2^32 - 1 tile positions, more than a u32 tile index holds; the symbolic
+    pass (and the pruned product built on it) fail with an error."""
+    M = mg()
+    ns = 65537
+    idx = np.arange(ns, dtype=np.uint32)
+    off = np.arange(ns + 1, dtype=np.uint32)
+    data = np.zeros((ns, 64), np.uint64)
+    data[:, 0] = 1
+    hm = M.HostTiledMatrix(ns, off, idx, np.zeros(ns, np.uint32), data)
+    a = M.TiledMatrix.upload(ctx, hm)
+    bt = M.TiledMatrix.upload(ctx, hm)
+    with pytest.raises(M.MGError) as e:
+        M.multiply_symbolic(ctx, a, bt)
+    assert "u32" in str(e.value)
+    with pytest.raises(M.MGError):
+        M.multiply_pruned(ctx, a, bt, M.PRUNED_EXACT)
+    a.close()
+    bt.close()
