@@ -590,6 +590,8 @@ def main():
     ap.add_argument("--square-only", action="store_true",
                     help="only the C = A*A section (R-MAT 14, seed 1) as its own JSON line")
     ap.add_argument("--no-square", action="store_true", help="skip the C = A*A section")
+    ap.add_argument("--hop-kernel", type=int, default=None, metavar="SEGMENTS",
+                    help="hop kernel after the first hop: 0 sweep, 1/2/4/8 job list (default: the library's)")
     ap.add_argument("--vertex-order", action="store_true",
                     help="keep frontier strips in vertex order (no internal strip permutation)")
     ap.add_argument("--two-hop", type=int, default=0, metavar="SCALE",
@@ -634,6 +636,8 @@ def main():
     flags |= M.WALK_VERTEX_ORDER if args.vertex_order else 0
     t3 = time.perf_counter()
     w = M.Walk(ctx, at, args.sources, args.hops, flags)
+    if args.hop_kernel is not None:
+        w.set_hop_kernel(args.hop_kernel)
     ctx.synchronize()
     t4 = time.perf_counter()
     setup = {
@@ -642,7 +646,8 @@ def main():
         "tile_build_ms": round((t2 - t1) * 1e3, 1),
         "walk_create_ms": round((t4 - t3) * 1e3, 1),
         "walk_create_covers": "A^T index relabelled to the engine's strip order, row masks, compact payloads, "
-                              "dense-tile rows, scheduler chunks and costs, column index for the pushed first hop",
+                              "dense-tile rows, scheduler chunks and costs, column index for the pushed first hop, "
+                              "job list and column counts of the hop kernel",
     }
 
     for _ in range(args.warmup):
@@ -663,6 +668,7 @@ def main():
     total_gathers = 0
     launches = []  # the roofline kernel's launches (hops that sweep every strip)
     kernel_names = {M.HOP_SWEEP: "sweep", M.HOP_PUSH: "push", M.HOP_JOBS: "jobs"}
+    roof_kernel = None  # device kernel name of the hops after the first
     for k in range(1, args.hops + 1):
         st = w.hop_stats(k)
         sym_ms, num_ms, n_runs = w.kernel_times(k)
@@ -671,11 +677,13 @@ def main():
         total_tp += st["tile_products"]
         total_gathers += st["edges_visited"]
         kname = kernel_names[st["kernel"]]
-        if st["kernel"] == M.HOP_SWEEP:
+        if st["kernel"] in (M.HOP_SWEEP, M.HOP_JOBS):
+            roof_kernel = ("walk_numeric64_kernel" if st["kernel"] == M.HOP_SWEEP else
+                           "walk_lean_kernel" if st["segments"] == 1 else f"walk_seg_kernel<{st['segments']}>")
             launches.append({"hop": k, "alg_bytes": st["alg_bytes_numeric"], "ms": round(avg_num, 4),
                              "frac": round(st["alg_bytes_numeric"] / (avg_num * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
         per_hop.append({
-            "hop": k, "kernel": kname, "tile_products": st["tile_products"],
+            "hop": k, "kernel": kname, "segments": st["segments"], "tile_products": st["tile_products"],
             "alg_bytes": st["alg_bytes"], "alg_bytes_numeric": st["alg_bytes_numeric"],
             "left_tiles": st["left_tiles"], "left_tiles_numeric": st["left_tiles_numeric"],
             "right_tiles_used": st["right_tiles_used"], "right_tiles_used_numeric": st["right_tiles_used_numeric"],
@@ -698,12 +706,13 @@ def main():
     sum_ms = sum(x["ms"] for x in steady)
     achieved = (sum_bytes / n_launch) / ((sum_ms / n_launch) * 1e-3) / 1e9 if sum_ms > 0 else 0.0
     traffic = None
-    tfile = ROOT / "profiles" / "walk_numeric_rmat22_traffic.json"
-    if (tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune
+    tname = {"walk_numeric64_kernel": "walk_numeric", "walk_lean_kernel": "walk_lean"}.get(roof_kernel)
+    tfile = ROOT / "profiles" / f"{tname}_rmat22_traffic.json"
+    if (tname and tfile.exists() and args.scale == 22 and args.sources == 4096 and args.hops == 4 and not args.prune
             and not args.vertex_order):
         traffic = json.loads(tfile.read_text()).get("bytes_per_steady_launch")
     roofline = {
-        "kernel": "walk_numeric64_kernel",
+        "kernel": roof_kernel,
         "launches": f"hops {', '.join(str(x['hop']) for x in steady)} of each walk (steady: X_k numerically full)",
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -712,7 +721,7 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
         "traffic_source": ("stored: HBM-side bytes per steady launch (TCC_EA0_RDREQ_32B/64B/128B, WRREQ/WRREQ_64B) "
-                           "from rocprofv3 --pmc runs of this bench, profiles/walk_numeric_rmat22_traffic.md; "
+                           f"from rocprofv3 --pmc runs of this bench, profiles/{tname}_rmat22_traffic.md; "
                            "not measured in this run") if traffic else None,
         "alg_bytes_per_launch": sum_bytes // n_launch,
         "alg_bytes_definition": "520*(T_L + T_R,used + T_C) + 4*(2*rows + 2) over numerically non-empty tiles "
@@ -748,8 +757,10 @@ def main():
         "walk_queries_per_s": round(1.0 / (ms_per_step * 1e-3), 3),
         "tile_products_per_s": round(total_tp / (ms_per_step * 1e-3), 1),
         "tile_products_per_walk": total_tp,
-        "tile_products_note": "structural (r,k,c) triples counted by the symbolic pass; the kernel executes one "
-                              "512-B frontier-row gather per A^T bit whose source is non-empty (gathers)",
+        "tile_products_note": "structural (r,k,c) triples counted by the symbolic pass; gathers = A^T bits whose "
+                              "source row is non-empty (one 512-B frontier-row gather each in the sweep kernel; "
+                              "the one-segment job-list kernel gathers a source row once per target strip and "
+                              "ORs it into all of that strip's rows it reaches)",
         "gathers_per_walk": total_gathers,
         "gathers_per_s": round(total_gathers / (ms_per_step * 1e-3), 1),
         "setup": setup,

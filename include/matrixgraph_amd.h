@@ -413,11 +413,13 @@ int mg_walk_create(mg_context* ctx, const mg_tiled_matrix* at, uint32_t max_sour
 /* Hop kernel of the 4096-source (64-word) levels from hop 2 on. segments = 0: the
  * tile-decoding sweep (every hop decodes the A^T tiles' payloads into gathers of whole
  * 512-byte source rows); 1, 2, 4 or 8: the job-list kernel, which reads a list of at's set
- * bits built once (4 bytes per set bit plus 4 per tile, on first use) and splits each
- * source row into that many segments, one per workgroup. The default is 4 (environment
- * MG_WALK_SEGMENTS overrides it at creation). Walks of fewer than 2049 sources, and graphs
- * above 2^26 vertices or 2^32 - 1 set bits, keep the sweep. Results and statistics are the
- * same for every choice. Synchronises the context's stream. */
+ * bits built once (4 bytes per set bit plus 4 per tile and per strip, on first use) and
+ * splits each source row into that many segments, one per workgroup. With 1 segment the
+ * kernel gathers whole rows, once per (strip, source) pair, with eight waves per workgroup.
+ * The default is 1 (environment MG_WALK_SEGMENTS overrides it at creation). Walks of fewer
+ * than 2049 sources, and graphs above 2^26 vertices or 2^32 - 65536 set bits, keep the
+ * sweep. Results and statistics are the same for every choice. Synchronises the context's
+ * stream. */
 int mg_walk_set_hop_kernel(mg_context* ctx, mg_walk* w, uint32_t segments);
 /* The segments in effect (0: sweep). */
 int mg_walk_hop_kernel(const mg_walk* w, uint32_t* segments);

@@ -2,7 +2,7 @@
 
 Searches the launch configuration of mg_walk_batch by timing the real workload on the
 device: the scheduler chunk size (A^T tiles per workgroup), the hop kernel of the levels
-after the first (the tile-decoding sweep, or the job-list kernel with 2 or 4 row segments,
+after the first (the tile-decoding sweep, or the job-list kernel with 1, 2 or 4 row segments,
 mg_walk_batch_set_hop_kernel) and the query packing. Each candidate is built, run once to
 capture its HIP graph, then timed over `reps` replays; the fastest wins.
 Results are cached per workload signature (graph sizes, query sizes and hop counts) in
@@ -17,7 +17,7 @@ from pathlib import Path
 
 DEFAULT_CANDIDATES = [
     {"packing": 0, "chunk_tiles": ct, "segments": seg}
-    for ct in (512, 1024, 2048, 4096) for seg in (0, 2, 4)
+    for ct in (512, 1024, 2048, 4096) for seg in (0, 1, 2, 4)
 ]
 # MG_PACK_ONE_PER_PANEL is not a default candidate: every query then holds 4096-source
 # frontiers for each of its levels, which does not fit for hundreds of queries on large graphs.

@@ -99,6 +99,7 @@ struct PanelLevel {
   unsigned long long* counters;
   const uint32_t* jobs;     // per A^T set bit: (source vertex << 6) | target row, tile order
   const uint32_t* job_off;  // per A^T tile: first position in jobs (n_tiles + 1 entries)
+  const uint32_t* col_count;  // per tile column of A^T: its tiles (with the job list)
   uint64_t* part;           // segmented hop kernel: per (segment, strip) partial N, V, Z masks
   uint64_t alive_in;    // words of X_k that propagate into X_{k+1}
   uint64_t alive_next;  // words of X_{k+1} that propagate further (they decide V_{k+1})
@@ -108,8 +109,11 @@ struct PanelLevel {
 
 // Chunk of one panel's A^T tile stream: strips [strip_begin, strip_end) restricted to
 // tiles [tile_begin, tile_end). A strip whose tiles exceed one chunk is split in pieces.
+// [col_begin, col_end): the tile columns whose per-column tile counts the chunk adds up
+// (one-segment job-list kernel); the chunks of a graph cover every column once.
 struct Chunk {
   uint32_t panel, strip_begin, strip_end, tile_begin, tile_end;
+  uint32_t col_begin, col_end;
 };
 struct StripItem {
   uint32_t panel, strip;
@@ -873,8 +877,10 @@ __global__ void __launch_bounds__(kBlock) tile_bits_kernel(const uint64_t* __res
   }
 }
 
-// Writes the jobs of every A^T tile at job_off[i]: lane = tile for the compact payloads; a
-// dense tile is expanded by the whole wave (lane = row) from its compacted rows.
+// Writes the jobs of every A^T tile at job_off[i], in column-major order within the tile
+// (by source vertex, then target row), so the jobs that gather the same source row into one
+// strip are adjacent: lane = tile for the compact payloads (entries sorted by column); a dense
+// tile is transposed by the whole wave (64 ballots: lane = column) and expanded column by column.
 __global__ void __launch_bounds__(kBlock) job_fill_kernel(const uint64_t* __restrict__ payload,
                                                           const uint32_t* __restrict__ col,
                                                           const uint64_t* __restrict__ dense,
@@ -892,29 +898,45 @@ __global__ void __launch_bounds__(kBlock) job_fill_kernel(const uint64_t* __rest
     const bool is_dense = valid && (pl & kPayloadDense) != 0;
     if (valid && !is_dense) {
       const uint32_t n_ent = static_cast<uint32_t>(pl >> 48) & 7u;
-      for (uint32_t k = 0; k < n_ent; ++k) {
+      // Entries as (column << 6) | row, insertion-sorted (at most kPayloadMax).
+      uint32_t key[kPayloadMax];
+#pragma unroll
+      for (uint32_t k = 0; k < kPayloadMax; ++k) {
         const uint32_t e = static_cast<uint32_t>(pl >> (12 * k)) & 0xFFFu;  // (row << 6) | column
-        jobs[at + k] = ((t * 64u + (e & 63u)) << 6) | (e >> 6);
+        key[k] = k < n_ent ? (((e & 63u) << 6) | (e >> 6)) : 0xFFFFu;
       }
+#pragma unroll
+      for (uint32_t a = 1; a < kPayloadMax; ++a)
+#pragma unroll
+        for (uint32_t b = a; b > 0; --b)
+          if (key[b] < key[b - 1]) {
+            const uint32_t x = key[b];
+            key[b] = key[b - 1];
+            key[b - 1] = x;
+          }
+      for (uint32_t k = 0; k < n_ent; ++k) jobs[at + k] = ((t * 64u + (key[k] >> 6)) << 6) | (key[k] & 63u);
     }
     for (uint64_t dm = ballot(is_dense); dm; dm &= dm - 1) {
       const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
       const uint64_t slot = (read_lane(pl, k) >> 13) & 0xFFFFFFFFull;
       const uint32_t tk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(t), k));
       const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(at), k));
-      uint64_t word = dense[slot * kTile + lane];
-      // Exclusive prefix of the rows' bit counts across the wave.
-      uint32_t pos = __popcll(word);
+      const uint64_t word = dense[slot * kTile + lane];  // row `lane` of the tile
+      uint64_t colmask = 0;                               // rows of column `lane`
+      for (int c = 0; c < kWave; ++c) {
+        const uint64_t m = ballot((word >> c) & 1);
+        if (lane == static_cast<uint32_t>(c)) colmask = m;
+      }
+      // Exclusive prefix of the columns' bit counts across the wave.
+      uint32_t pos = __popcll(colmask);
 #pragma unroll
       for (int m = 1; m < kWave; m <<= 1) {
         const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(pos), m, kWave));
         if (lane >= static_cast<uint32_t>(m)) pos += o;
       }
-      pos -= __popcll(word);
-      for (; word; word &= word - 1) {
-        const uint32_t c = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(word)) - 1);
-        jobs[base + pos++] = ((tk * 64u + c) << 6) | lane;
-      }
+      pos -= __popcll(colmask);
+      for (uint64_t r = colmask; r; r &= r - 1)
+        jobs[base + pos++] = ((tk * 64u + lane) << 6) | static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(r)) - 1);
     }
   }
 }
@@ -1098,6 +1120,304 @@ __global__ void __launch_bounds__(kSegBlock) walk_seg_kernel(const Chunk* __rest
   }
   __syncthreads();
   if (threadIdx.x == 0 && n_cs > 0) flush_masks(n_cs - 1);
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
+}
+
+// --- numeric, 64 words per vertex, from the job list, whole rows, eight waves ----------
+// The job-list kernel with one segment. The gathers of whole 512-byte source rows are bound
+// by how many waves keep loads in flight on a CU, not by the loads each wave keeps in
+// flight, so this kernel is built for the most resident waves: eight waves share one
+// 64 x 64-word strip accumulator (32 KiB of LDS: four workgroups, 32 waves per CU) and the
+// kernel fits in 64 VGPRs (eight waves per SIMD). A wave gathers one job at a time with all
+// 64 lanes (lane = word of the row): the job is read out of a register holding 64 jobs with
+// v_readlane, so the row address and the target row are scalar, and kLeanUnroll gathers
+// per register set, two sets alternating, stay in flight.
+//
+// Per strip: the structural sweep of its tiles (S mask, tile products, used tiles), the jobs
+// [job_off[t0], job_off[t1]) dealt to the waves in runs of kLeanRun (run R goes to wave
+// R mod 8, so a light strip still spreads over the waves), source rows whose V bit is clear
+// skipped, one barrier, and the write-out shared with the sweep kernel.
+#ifndef MG_LEAN_UNROLL
+#define MG_LEAN_UNROLL 4
+#endif
+constexpr int kLeanWaves = 8;
+constexpr int kLeanBlock = kLeanWaves * kWave;
+constexpr int kLeanUnroll = MG_LEAN_UNROLL;  // row gathers per register set
+constexpr uint32_t kLeanRun = 8;             // consecutive jobs of a strip dealt to one wave
+constexpr uint32_t kLeanSpan = kLeanRun * kLeanWaves * (kWave / kLeanRun);  // jobs per batch round
+static_assert(kWave % kLeanRun == 0, "a batch holds whole runs");
+
+struct LeanSet {
+  uint64_t v[kLeanUnroll];
+  uint64_t rows[kLeanUnroll];  // target rows of the gathered source row (0: none)
+};
+
+// Issues the gathers of the next kLeanUnroll sources of `m` (wave-uniform: lanes of `job` still
+// to gather): a scalar row address plus the lane's word. Jobs of one source that follow each
+// other in a run (the job list is column-major within a tile) share one gather: their rows are
+// collected into one mask and their lanes cleared from `m`.
+__device__ inline void lean_issue(LeanSet& s, uint64_t& m, uint32_t job, gptr<const uint64_t> x_in, uint32_t lane) {
+#pragma unroll
+  for (int k = 0; k < kLeanUnroll; ++k) {
+    s.rows[k] = 0;
+    s.v[k] = 0;
+    if (m) {
+      const int src = __ffsll(static_cast<unsigned long long>(m)) - 1;
+      m &= m - 1;
+      const uint32_t jk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), src));
+      const uint32_t u = jk >> 6;
+      uint64_t rows = 1ull << (jk & 63u);
+      for (int l = src + 1; (l % static_cast<int>(kLeanRun)) != 0 && ((m >> l) & 1); ++l) {
+        const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), l));
+        if ((jl >> 6) != u) break;
+        rows |= 1ull << (jl & 63u);
+        m &= ~(1ull << l);
+      }
+      const gptr<const uint64_t> row = x_in + static_cast<uint64_t>(u) * 64u;
+      s.v[k] = row[lane];
+      s.rows[k] = rows;
+    }
+  }
+}
+
+// ORs each gathered row into its target rows; with count_rows, adds the jobs whose source row
+// is non-empty in a live word to `jobs_ok` (wave-uniform).
+__device__ inline void lean_commit(const LeanSet& s, char* __restrict__ acc_lane, uint64_t keep, bool count_rows,
+                                   uint32_t& jobs_ok) {
+#pragma unroll
+  for (int k = 0; k < kLeanUnroll; ++k)
+    if (s.rows[k]) {
+      const uint64_t val = s.v[k] & keep;
+      if (count_rows && ballot(val != 0)) jobs_ok += static_cast<uint32_t>(__popcll(s.rows[k]));
+      for (uint64_t r = s.rows[k]; r; r &= r - 1)
+        atomicOr(reinterpret_cast<unsigned long long*>(acc_lane + 512u * (__ffsll(static_cast<unsigned long long>(r)) - 1)),
+                 val);
+    }
+}
+
+__global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu(kLeanWaves, kLeanWaves)))
+    walk_lean_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv) {
+  constexpr int kWords = 64 * 64;
+  constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+  __shared__ unsigned long long acc[kWords];
+  // Per strip parity: N, V, Z and S masks being reduced (thread 0 stores and clears a slot
+  // after the strip's second barrier; the other slot is in use by then).
+  __shared__ unsigned long long s_mask[2][4];
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const Chunk ch = chunks[blockIdx.x];
+  const PanelLevel& L = lv[ch.panel];
+  const gptr<const uint32_t> __restrict__ off = as_global(L.off);
+  const gptr<const uint32_t> __restrict__ col = as_global(L.col);
+  const gptr<const uint32_t> __restrict__ jobs = as_global(L.jobs);
+  const gptr<const uint32_t> __restrict__ job_off = as_global(L.job_off);
+  const gptr<const uint64_t> __restrict__ v_in = as_global(L.v_in);
+  const gptr<const uint64_t> __restrict__ s_in = as_global(L.s_in);
+  const gptr<const uint64_t> x_in = as_global(L.x_in);
+  char* const acc_lane = reinterpret_cast<char*>(acc) + 8u * lane;
+  const uint64_t alive_in = L.alive_in;
+  const uint64_t alive_next = L.alive_next;
+  const uint64_t keep = ((alive_in >> lane) & 1) ? ~0ull : 0ull;
+  const bool lane_next = (alive_next >> lane) & 1;  // this lane's word propagates further
+  // Per-strip values of the chunk, one strip per lane: row pointers and the job offsets of
+  // the strip boundaries (clamped to the chunk's tile range).
+  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
+  const uint32_t ys = ch.strip_begin + lane;
+  const uint32_t off_l = lane <= n_cs ? off[ys] : 0u;
+  const uint32_t cl_l = off_l < ch.tile_begin ? ch.tile_begin : (off_l > ch.tile_end ? ch.tile_end : off_l);
+  const uint32_t jo_l = lane <= n_cs ? job_off[cl_l] : 0u;
+  // This wave's batches over the whole chunk, in order: batch (strip i, base) holds the jobs
+  // base + 64 * (l / kLeanRun) + l % kLeanRun below jb(i) (lane l: run l / kLeanRun of the
+  // batch); the next one is base + kLeanSpan in the same strip, else the first batch of the
+  // next strip with more than kLeanRun * wave jobs (strip n_cs: none left).
+  const uint32_t jo_next_l = static_cast<uint32_t>(__shfl_down(static_cast<int>(jo_l), 1, kWave));
+  const uint64_t wave_strips = ballot(lane < n_cs && jo_next_l - jo_l > kLeanRun * wave);
+  auto first_batch = [&](uint32_t from, uint32_t& strip, uint32_t& base) {
+    const uint64_t m = from < 64 ? (wave_strips & (~0ull << from)) : 0ull;
+    strip = m ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1) : n_cs;
+    base = m ? static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jo_l), static_cast<int>(strip))) +
+                   kLeanRun * wave
+             : 0u;
+  };
+  auto next_batch = [&](uint32_t strip, uint32_t base, uint32_t& n_strip, uint32_t& n_base) {
+    const uint32_t end =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jo_l), static_cast<int>(strip + 1)));
+    if (base + kLeanSpan < end) {
+      n_strip = strip;
+      n_base = base + kLeanSpan;
+    } else {
+      first_batch(strip + 1, n_strip, n_base);
+    }
+  };
+  auto load_batch = [&](uint32_t strip, uint32_t base) -> uint32_t {
+    if (strip >= n_cs) return kNoJob;
+    const uint32_t end =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(jo_l), static_cast<int>(strip + 1)));
+    const uint32_t j = base + kLeanRun * kLeanWaves * (lane / kLeanRun) + lane % kLeanRun;
+    return j < end ? __builtin_nontemporal_load(jobs + j) : kNoJob;
+  };
+  // Source rows are filtered by their V bits unless at least two thirds of X_k's structural
+  // tiles are numerically non-empty (the previous level's counters): then nearly every source
+  // row is non-empty, gathering the few empty ones costs less than loading V words, and the
+  // gathered rows that are non-empty in a live word (the V bit's definition) are counted.
+  bool check_v;
+  {
+    const gptr<const unsigned long long> prev = as_global(static_cast<const unsigned long long*>(L.counters)) -
+                                                kCounterStride + lane * kNumCounters;
+    const uint64_t numeric = wave_sum(prev[kOutNumeric]);
+    const uint64_t structural = wave_sum(prev[kOut]);
+    check_v = !(structural > 0 && 3 * numeric >= 2 * structural);
+  }
+  uint32_t p_strip, p_base;
+  first_batch(0, p_strip, p_base);
+  uint32_t job = load_batch(p_strip, p_base);
+  uint64_t vw = (check_v && job != kNoJob) ? v_in[job >> 12] : 0ull;
+  uint32_t edges = 0;  // wave-uniform
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 8) (&s_mask[0][0])[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < kWords; i += kLeanBlock) acc[i] = 0;
+  __syncthreads();
+  // Tile products and used tiles of the hop, per tile column: column t holds col_count[t]
+  // tiles, each with S_k[t] & alive as its source mask. The chunk adds up its columns.
+  {
+    const gptr<const uint32_t> cc = as_global(L.col_count);
+    uint64_t tp = 0, used = 0;
+    for (uint32_t t = ch.col_begin + threadIdx.x; t < ch.col_end; t += kLeanBlock) {
+      const uint64_t sm = s_in[t] & alive_in;
+      const uint64_t c = cc[t];
+      tp += static_cast<uint64_t>(__popcll(sm)) * c;
+      used += sm ? c : 0ull;
+    }
+    tp = wave_sum(tp);
+    used = wave_sum(used);
+    if (lane == 0) {
+      if (tp) atomicAdd(&s_cnt[bTP], static_cast<unsigned long long>(tp));
+      if (used) atomicAdd(&s_cnt[bUsed], static_cast<unsigned long long>(used));
+    }
+  }
+
+  for (uint32_t i = 0; i < n_cs; ++i) {
+    const uint32_t y = ch.strip_begin + i;
+    const uint32_t sl = i & 1;
+    const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i)));
+    const uint32_t r1 =
+        static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(off_l), static_cast<int>(i + 1)));
+    const uint32_t t0 = r0 > ch.tile_begin ? r0 : ch.tile_begin;
+    const uint32_t t1 = r1 < ch.tile_end ? r1 : ch.tile_end;
+    const bool split = (t0 != r0) || (t1 != r1);
+    // The strip's first kLeanBlock tile columns, for the structural mask after the gathers.
+    const uint32_t p0 = t0 + threadIdx.x;
+    const uint32_t c0 = p0 < t1 ? __builtin_nontemporal_load(col + p0) : kNoTile;
+    // Gathers of this wave's batches in strip i. The next batch's jobs are loaded before this
+    // batch's gathers start and its V words right after the first register set is issued,
+    // so every load has a whole batch to arrive.
+    while (p_strip == i) {
+      uint32_t n_strip, n_base;
+      next_batch(p_strip, p_base, n_strip, n_base);
+      const uint32_t job_n = load_batch(n_strip, n_base);
+      const bool ok = check_v ? ((vw >> ((job >> 6) & 63u)) & 1) : job != kNoJob;
+      uint64_t m = ballot(ok);
+      if (check_v) edges += static_cast<uint32_t>(__popcll(m));
+      const bool count_rows = !check_v;
+      LeanSet a, b;
+      lean_issue(a, m, job, x_in, lane);
+      const uint64_t vw_n = (check_v && job_n != kNoJob) ? v_in[job_n >> 12] : 0ull;
+      for (;;) {
+        if (!m) {
+          lean_commit(a, acc_lane, keep, count_rows, edges);
+          break;
+        }
+        lean_issue(b, m, job, x_in, lane);
+        lean_commit(a, acc_lane, keep, count_rows, edges);
+        if (!m) {
+          lean_commit(b, acc_lane, keep, count_rows, edges);
+          break;
+        }
+        lean_issue(a, m, job, x_in, lane);
+        lean_commit(b, acc_lane, keep, count_rows, edges);
+      }
+      job = job_n;
+      vw = vw_n;
+      p_strip = n_strip;
+      p_base = n_base;
+    }
+    // Structural mask of the strip: OR of its tiles' S_k[t] & alive. It stops as soon as it
+    // holds every live word (exact: no tile can add to it), which at dense hops is after the
+    // first tiles; a wave that finds the mask full when its gathers end loads nothing more.
+    {
+      volatile unsigned long long* const sym = &s_mask[sl][3];
+      for (uint32_t p = p0; p < t1; p += kLeanBlock) {
+        if (read_first_lane64(*sym) == alive_in) break;
+        const uint32_t t = p == p0 ? c0 : __builtin_nontemporal_load(col + p);
+        const uint64_t sm = s_in[t] & alive_in;
+        if (sm) atomicOr(const_cast<unsigned long long*>(sym), static_cast<unsigned long long>(sm));
+      }
+    }
+    __syncthreads();
+    // Write-out: wave w stores rows w, w + 8, ... (lane = word) and clears them. An unsplit
+    // strip reduces its masks by ballots (N: non-zero words, V: rows non-zero in a word that
+    // propagates, Z: non-zero rows) and skips zero words outside the previous Z x N; a piece
+    // of a split strip ORs into memory zeroed before the launch.
+    const gptr<uint64_t> dst = as_global(L.x_out) + static_cast<uint64_t>(y) * kWords + lane;
+    if (!split) {
+      const uint64_t z_prev = as_global(L.z_out)[y];
+      const bool n_prev_lane = (as_global(L.n_out)[y] >> lane) & 1;
+      uint64_t zmask = 0, vmask = 0;
+      bool nz_lane = false;
+      uint32_t pc = 0;
+#pragma unroll 2
+      for (uint32_t r = wave; r < 64; r += kLeanWaves) {
+        const uint64_t val = acc[r * 64 + lane];
+        acc[r * 64 + lane] = 0;
+        const bool nz = val != 0;
+        const uint64_t rb = ballot(nz);
+        const uint64_t rv = ballot(nz && lane_next);
+        zmask |= rb ? (1ull << r) : 0ull;
+        vmask |= rv ? (1ull << r) : 0ull;
+        nz_lane |= nz;
+        pc += __popcll(val);
+        if (nz || (((z_prev >> r) & 1) && n_prev_lane)) __builtin_nontemporal_store(val, dst + r * 64u);
+      }
+      const uint64_t nmask = ballot(nz_lane);
+      cnt[bPop] += pc;
+      if (lane == 0) {
+        if (nmask) atomicOr(&s_mask[sl][0], static_cast<unsigned long long>(nmask));
+        if (vmask) atomicOr(&s_mask[sl][1], static_cast<unsigned long long>(vmask));
+        if (zmask) atomicOr(&s_mask[sl][2], static_cast<unsigned long long>(zmask));
+      }
+    } else {
+#pragma unroll 2
+      for (uint32_t r = wave; r < 64; r += kLeanWaves) {
+        const uint64_t val = acc[r * 64 + lane];
+        acc[r * 64 + lane] = 0;
+        if (val) atomicOr((unsigned long long*)(dst + r * 64u), static_cast<unsigned long long>(val));
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t sym = s_mask[sl][3];
+      if (!split) {
+        as_global(L.n_out)[y] = s_mask[sl][0];
+        as_global(L.v_out)[y] = s_mask[sl][1];
+        as_global(L.z_out)[y] = s_mask[sl][2];
+        as_global(L.s_out)[y] = sym;
+        cnt[bNumeric] += __popcll(s_mask[sl][0]);
+        cnt[bOut] += __popcll(sym);
+        cnt[bLeft] += __popcll(s_in[y] & alive_in);
+      } else if (sym) {
+        atomicOr((unsigned long long*)(as_global(L.s_out) + y), static_cast<unsigned long long>(sym));
+      }
+      s_mask[sl][0] = 0;
+      s_mask[sl][1] = 0;
+      s_mask[sl][2] = 0;
+      s_mask[sl][3] = 0;
+    }
+  }
+  if (lane == 0) cnt[bEdges] += edges;
   flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
 }
 
@@ -1585,6 +1905,12 @@ std::vector<Chunk> build_chunks(const std::vector<uint32_t>& off, uint32_t n_str
     ++cur_strips;
   }
   close(n_strips);
+  // Tile columns, split evenly over the chunks.
+  const uint64_t nc = chunks.size();
+  for (uint64_t i = 0; i < nc; ++i) {
+    chunks[i].col_begin = static_cast<uint32_t>(i * n_strips / nc);
+    chunks[i].col_end = static_cast<uint32_t>((i + 1) * n_strips / nc);
+  }
   return chunks;
 }
 
@@ -1630,6 +1956,7 @@ struct GraphPlan {
   // Job list of A^T for the segmented hop kernel (build_job_list): empty unless built.
   DevBuf<uint32_t> jobs;
   DevBuf<uint32_t> job_off;
+  DevBuf<uint32_t> col_count;  // tiles per tile column of A^T (built with the job list)
   bool has_jobs = false;
 };
 
@@ -1667,6 +1994,13 @@ bool job_list_fits(const GraphPlan& g) {
   return static_cast<uint64_t>(g.n_strips) * 64 <= kJobMaxVertices && g.total_bits <= 0xFFFF0000ull;
 }
 
+// Tiles per tile column of A^T (count zeroed by the caller).
+__global__ void __launch_bounds__(kBlock) column_count_kernel(const uint32_t* __restrict__ col, uint32_t n,
+                                                              uint32_t* __restrict__ count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(count + col[i], 1u);
+}
+
 // Builds the job list of A^T (once per graph): per-tile bit counts from the payloads, their
 // exclusive scan as job_off, then job_fill_kernel. 4 bytes per set bit plus 4 per tile.
 void build_job_list(GraphPlan& g, hipStream_t st) {
@@ -1675,8 +2009,9 @@ void build_job_list(GraphPlan& g, hipStream_t st) {
   const uint32_t n = g.at->n_nz_tile;
   g.job_off.alloc(n + 1ull);
   g.jobs.alloc(std::max<uint64_t>(1, g.total_bits));
+  DevBuf<uint32_t> bits;
   if (n) {
-    DevBuf<uint32_t> bits(n);
+    bits.alloc(n);
     tile_bits_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.payload.get(), n, bits.get());
     MG_LAUNCH_CHECK();
     exclusive_scan_u32_on(st, bits.get(), g.job_off.get(), n);
@@ -1684,11 +2019,17 @@ void build_job_list(GraphPlan& g, hipStream_t st) {
                                                                         g.dense.get(), g.job_off.get(), n,
                                                                         g.jobs.get());
     MG_LAUNCH_CHECK();
-    MG_HIP(hipStreamSynchronize(st));  // bits is freed on return
   } else {
     MG_HIP(hipMemsetAsync(g.job_off.get(), 0, sizeof(uint32_t), st));
-    MG_HIP(hipStreamSynchronize(st));
   }
+  const uint32_t ns = std::max<uint32_t>(1, g.n_strips);
+  g.col_count.alloc(ns);
+  MG_HIP(hipMemsetAsync(g.col_count.get(), 0, sizeof(uint32_t) * ns, st));
+  if (n) {
+    column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.col, n, g.col_count.get());
+    MG_LAUNCH_CHECK();
+  }
+  MG_HIP(hipStreamSynchronize(st));  // bits is freed on return
   g.has_jobs = true;
 }
 
@@ -1816,12 +2157,6 @@ GraphPlan plan_graph(const mg_tiled_matrix* at, uint32_t chunk_tiles, hipStream_
   }
   g.chunk_cost = chunk_costs(g, st, &g.total_bits);
   return g;
-}
-
-__global__ void __launch_bounds__(kBlock) column_count_kernel(const uint32_t* __restrict__ col, uint32_t n,
-                                                              uint32_t* __restrict__ count) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    atomicAdd(count + col[i], 1u);
 }
 
 // Order within a column is whatever the atomics give: every consumer ORs or sums.
@@ -2090,6 +2425,7 @@ void apply_seg(Engine* e, uint32_t G, hipStream_t st) {
   if (G) {
     for (auto& g : e->graphs) build_job_list(g, st);
     for (auto& p : e->panels) {
+      if (G == 1) break;  // the one-segment kernel writes the masks itself
       const uint64_t need = static_cast<uint64_t>(G) * 3 * std::max<uint32_t>(1, p.n_strips);
       if (p.part.size() < need) {
         p.part.release();
@@ -2105,6 +2441,7 @@ void apply_seg(Engine* e, uint32_t G, hipStream_t st) {
       const GraphPlan& g = e->graphs[e->panels[pi].graph];
       lv[pi].jobs = g.has_jobs ? g.jobs.get() : nullptr;
       lv[pi].job_off = g.has_jobs ? g.job_off.get() : nullptr;
+      lv[pi].col_count = g.has_jobs ? g.col_count.get() : nullptr;
       lv[pi].part = e->panels[pi].part.get();
     }
     MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * lv.size(), hipMemcpyHostToDevice, st));
@@ -2123,7 +2460,7 @@ bool use_relabel(bool identity_flag) {
 // Default hop kernel of 64-word levels (kDefaultSegments); MG_WALK_SEGMENTS (0, 1, 2, 4 or 8)
 // overrides it.
 #ifndef MG_DEFAULT_SEGMENTS
-#define MG_DEFAULT_SEGMENTS 0
+#define MG_DEFAULT_SEGMENTS 1
 #endif
 constexpr uint32_t kDefaultSegments = MG_DEFAULT_SEGMENTS;
 uint32_t default_segments() {
@@ -2148,7 +2485,7 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
       const uint32_t nb = e->n_chunks[k];
       switch (G) {
         case 0: walk_numeric64_kernel<<<nb, kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log); break;
-        case 1: walk_seg_kernel<1><<<nb, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        case 1: walk_lean_kernel<<<nb, kLeanBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
         case 2: walk_seg_kernel<2><<<nb * 2, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
         case 4: walk_seg_kernel<4><<<nb * 4, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
         case 8: walk_seg_kernel<8><<<nb * 8, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
@@ -2160,7 +2497,7 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
-  if (G) {
+  if (G > 1) {
     const uint32_t n_panels = static_cast<uint32_t>(e->panels.size());
     uint32_t max_strips = 1;
     for (const auto& p : e->panels) max_strips = std::max(max_strips, p.n_strips);

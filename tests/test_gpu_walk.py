@@ -296,9 +296,11 @@ def test_job_list_kernel_equals_sweep_and_oracle(ctx, segments):
         assert np.array_equal(wj.frontier(k), xs[k]), f"oracle hop {k}"
 
 
-def test_job_list_kernel_ragged_graph(ctx):
-    """The job-list kernel on a 1000-vertex graph (partial last strip, dense tiles, walks that
-    die out, self-loops) with a 4096-source walk: every hop equals the oracle."""
+@pytest.mark.parametrize("segments", [1, 4])
+def test_job_list_kernel_ragged_graph(ctx, segments):
+    """The job-list kernel (one segment: whole rows, eight waves; four segments) on a
+    1000-vertex graph (partial last strip, dense tiles, walks that die out, self-loops) with a
+    4096-source walk: every hop equals the oracle."""
     M = mg()
     n = 1000
     s, d = ragged_graph()
@@ -306,7 +308,7 @@ def test_job_list_kernel_ragged_graph(ctx):
     sources = np.array([0, 999, 5, 650, 320, 640, 998] + list(range(50, 90)), np.uint32)
     hops = 6
     w = M.Walk(ctx, at, 4096, hops)
-    assert w.set_hop_kernel(4) == 4
+    assert w.set_hop_kernel(segments) == segments
     w.run(sources, hops)
     xs = O.walk_frontiers(n, s, d, sources, hops, W=64)
     for k in range(1, hops + 1):
