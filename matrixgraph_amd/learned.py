@@ -141,12 +141,35 @@ def predict(mlp, X):
     return jobs, occ
 
 
-def order_by_cost(cost, buckets_per_octave: int = 2):
+def order_by_cost(cost, groups=None, buckets_per_octave: int = 2):
     """Longest-first launch order over cost buckets (buckets_per_octave per factor of 2); within a
-    bucket the current order is kept, so chunks of the same graph and strip range that the
-    static order places together (and that share cached A^T tiles) stay together."""
-    key = np.floor(np.log2(np.asarray(cost, np.float64) + 1.0) * buckets_per_octave)
-    return np.argsort(-key, kind="stable").astype(np.uint32)
+    bucket the current order is kept. With `groups` (one id per chunk: the chunks of one graph's
+    tile range in different panels, which read the same A^T data), whole groups are ordered by
+    their summed cost and a group's chunks stay adjacent in their current order, so the panels'
+    reads of the same job list and tiles follow each other and share L2."""
+    cost = np.asarray(cost, np.float64)
+    if groups is None:
+        key = np.floor(np.log2(cost + 1.0) * buckets_per_octave)
+        return np.argsort(-key, kind="stable").astype(np.uint32)
+    n = len(cost)
+    _, inv = np.unique(np.asarray(groups), return_inverse=True)
+    gcost = np.bincount(inv, weights=cost)
+    first = np.full(len(gcost), n, np.int64)
+    np.minimum.at(first, inv, np.arange(n))
+    gkey = np.floor(np.log2(gcost + 1.0) * buckets_per_octave)
+    rank = np.empty(len(gcost), np.int64)
+    rank[np.lexsort((first, -gkey))] = np.arange(len(gcost))
+    return np.lexsort((np.arange(n), rank[inv])).astype(np.uint32)
+
+
+def chunk_groups(batch, rec, panel_cache):
+    """Group id per chunk record: (graph of its panel, first A^T tile of the chunk)."""
+    g = np.zeros(len(rec), np.int64)
+    for p in np.unique(rec["panel"]):
+        if p not in panel_cache:
+            panel_cache[p] = batch.panel_info(int(p))
+        g[rec["panel"] == p] = panel_cache[p]["graph"]
+    return (g << 33) | rec["tile_begin"].astype(np.int64)
 
 
 def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_ratio=2.0):
@@ -158,7 +181,7 @@ def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_rat
     for k in range(levels):
         rec, X = level_features(batch, k, gstats, cache)
         jobs, occ = predict(mlp, X)
-        o = order_by_cost(chunk_cost(rec, jobs))
+        o = order_by_cost(chunk_cost(rec, jobs), chunk_groups(batch, rec, cache))
         batch.set_level_order(k, o)
         preds.append((jobs[o], occ[o]))  # aligned with the new launch order
     batch.set_logging(True)
@@ -178,7 +201,7 @@ def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_rat
         n_occ_bad += int(np.sum(np.abs(occ_pred - occ_act) > 0.25))
         tot_ratio = (jobs_pred.sum() + 1.0) / (actual.sum() + 1.0)
         if tot_ratio > fallback_ratio or tot_ratio < 1.0 / fallback_ratio:
-            batch.set_level_order(k, order_by_cost(chunk_cost(rec, actual)))
+            batch.set_level_order(k, order_by_cost(chunk_cost(rec, actual), chunk_groups(batch, rec, cache)))
             fallbacks += 1
     batch.set_logging(False)
     return {
@@ -196,12 +219,13 @@ def schedule_learned(ctx, batch, mlp, gstats, fallback_ratio=2.0, mispredict_rat
 def schedule_exact(ctx, batch):
     """Orders every level by the exact costs of a logged run (the fallback used for every level)."""
     levels = batch.info()[1]
+    cache = {}
     batch.set_logging(True)
     batch.run()
     ctx.synchronize()
     for k in range(levels):
         rec = batch.level_chunks(k)
-        batch.set_level_order(k, order_by_cost(chunk_cost(rec, rec["jobs"])))
+        batch.set_level_order(k, order_by_cost(chunk_cost(rec, rec["jobs"]), chunk_groups(batch, rec, cache)))
     batch.set_logging(False)
 
 
