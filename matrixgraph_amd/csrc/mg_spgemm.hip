@@ -330,11 +330,22 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
       const uint32_t na = wide_count(pa), nb = wide_count(pb);
       const bool dense = has && !mfma && (na == kWideDense || nb == kWideDense);
       if (has && !mfma && !dense) {
-        for (uint32_t u = 0; u < na; ++u) {
-          const uint32_t ea = wide_entry(pa, u);
+        // Only the inner indices j present in both tiles give bits: per common j, the Bt
+        // entries' columns are collected into one mask and ORed into the row of every A
+        // entry with that j (na + nb steps per common j instead of na * nb comparisons).
+        uint64_t jm_a = 0, jm_b = 0;
+        for (uint32_t u = 0; u < na; ++u) jm_a |= 1ull << (wide_entry(pa, u) & 63u);
+        for (uint32_t v = 0; v < nb; ++v) jm_b |= 1ull << (wide_entry(pb, v) & 63u);
+        for (uint64_t common = jm_a & jm_b; common; common &= common - 1) {
+          const uint32_t jj = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(common)) - 1);
+          uint64_t cols = 0;
           for (uint32_t v = 0; v < nb; ++v) {
             const uint32_t eb = wide_entry(pb, v);
-            if (((ea ^ eb) & 63u) == 0) atomicOr(acc_l + (ea >> 6), 1ull << (eb >> 6));
+            if ((eb & 63u) == jj) cols |= 1ull << (eb >> 6);
+          }
+          for (uint32_t u = 0; u < na; ++u) {
+            const uint32_t ea = wide_entry(pa, u);
+            if ((ea & 63u) == jj) atomicOr(acc_l + (ea >> 6), cols);
           }
         }
       }
