@@ -1502,12 +1502,11 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
 // X_0 is one-hot: only the A^T tile columns of the strips that hold sources carry numeric
 // work into X_1. The first hop of a walk can therefore read just those columns (through a
 // column index of A^T) and OR each source row into its target rows with global atomics,
-// instead of sweeping all strips with the LDS accumulator. Four launches produce the same
+// instead of sweeping all strips with the LDS accumulator. Three launches produce the same
 // X_1, masks and counters as the sweep: push_reset (clear the previous run's words of X_1,
-// reset its masks), push_structural (S_1 and the tile-product counts from the A^T index, one
-// workgroup per scheduler chunk, no global atomics on hot strips), push_first_hop (the
-// jobs) and push_finalize (N, V masks, popcount and strip counts, read back from the rows
-// the pushes marked in Z).
+// reset its masks), push_first_hop (the jobs, and S_1: only the source strips' columns add to
+// it) and push_finalize (N, V masks, popcount and strip counts, read back from the rows the
+// pushes marked in Z, and the tile counts per column).
 
 // Positions [begin, end) of the A^T column index that belong to source strip `strip`.
 struct PushItem {
@@ -1545,69 +1544,6 @@ __global__ void __launch_bounds__(kBlock) push_reset_kernel(const PanelLevel* __
   }
 }
 
-// S_1 and the tile-product counts of the first hop: per A^T tile (y, t), S_1[y] |= S_0[t] &
-// alive. One workgroup per scheduler chunk; the chunk's strips are reduced in LDS and stored
-// (pieces of split strips OR into S_1, zeroed by push_reset).
-__global__ void __launch_bounds__(kBlock) push_structural_kernel(const Chunk* __restrict__ chunks,
-                                                                 const PanelLevel* __restrict__ lv) {
-  __shared__ unsigned long long s_sym[kChunkStrips];
-  __shared__ unsigned long long s_cnt[kBlockCounters];
-  const Chunk ch = chunks[blockIdx.x];
-  const PanelLevel& L = lv[ch.panel];
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x / kWave;
-  const gptr<const uint32_t> col = as_global(L.col);
-  const gptr<const uint32_t> row = as_global(L.row);
-  const gptr<const uint64_t> s_in = as_global(L.s_in);
-  const uint64_t alive = L.alive_in;
-  uint32_t cnt[kBlockCounters];
-#pragma unroll
-  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
-  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
-  if (threadIdx.x < kChunkStrips) s_sym[threadIdx.x] = 0;
-  __syncthreads();
-  // kGroups groups of 64 tiles per wave step, their index loads issued together.
-  constexpr int kGroups = 4;
-  for (uint32_t g0 = ch.tile_begin + 64 * wave; g0 < ch.tile_end; g0 += 64 * kWaves * kGroups) {
-    uint32_t t[kGroups], y[kGroups];
-    uint64_t sv[kGroups];
-#pragma unroll
-    for (int q = 0; q < kGroups; ++q) {
-      const uint32_t p = g0 + 64 * kWaves * q + lane;
-      const bool v = p < ch.tile_end;
-      t[q] = v ? __builtin_nontemporal_load(col + p) : 0xFFFFFFFFu;
-      y[q] = v ? __builtin_nontemporal_load(row + p) : 0u;
-    }
-#pragma unroll
-    for (int q = 0; q < kGroups; ++q) sv[q] = t[q] != 0xFFFFFFFFu ? (s_in[t[q]] & alive) : 0ull;
-    // A group mostly lies in one strip: lanes in lane 0's strip are reduced across the wave
-    // first, so the LDS ORs do not all queue on one address.
-#pragma unroll
-    for (int q = 0; q < kGroups; ++q) {
-      cnt[bTP] += __popcll(sv[q]);
-      cnt[bUsed] += sv[q] != 0;
-      const uint32_t y0 = read_first_lane(y[q]);
-      const bool same = y[q] == y0;
-      const uint64_t red = wave_or(same ? sv[q] : 0ull);
-      if (lane == 0 && red) atomicOr(&s_sym[y0 - ch.strip_begin], static_cast<unsigned long long>(red));
-      if (!same && sv[q]) atomicOr(&s_sym[y[q] - ch.strip_begin], static_cast<unsigned long long>(sv[q]));
-    }
-  }
-  __syncthreads();
-  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
-  if (threadIdx.x < n_cs) {
-    const uint32_t y = ch.strip_begin + threadIdx.x;
-    const gptr<const uint32_t> off = as_global(L.off);
-    const bool split = ch.tile_begin != off[y] || ch.tile_end != off[y + 1];
-    if (split) {
-      if (s_sym[threadIdx.x]) atomicOr((unsigned long long*)(L.s_out + y), s_sym[threadIdx.x]);
-    } else {
-      as_global(L.s_out)[y] = s_sym[threadIdx.x];
-    }
-  }
-  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
-}
-
 // Pushes the first n jobs of a wave list; a job is (target vertex << 32) | source vertex.
 // Each source row is loaded whole (lane = word) and its non-zero words are OR-ed into the
 // target row; the target row is marked in Z.
@@ -1638,7 +1574,10 @@ __device__ inline void push_drain(unsigned long long* __restrict__ jobs, uint32_
 }
 
 // One workgroup per item; its waves take the item's column positions in groups of 64
-// (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job.
+// (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job. Every
+// tile (y, t) of the column also ORs S_0[t] & alive into S_1[y] (structural X_1; tiles of other
+// columns add nothing): a hub strip y is reached from most source columns, so S_1[y] is read
+// first and only a missing bit costs an atomic.
 template <int W>
 __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* __restrict__ items,
                                                                 const PanelLevel* __restrict__ lv) {
@@ -1658,6 +1597,7 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
   const gptr<uint64_t> z_out = as_global(L.z_out);
   const uint32_t t = it.strip;
   const uint64_t vm = as_global(L.v_in)[t];
+  const uint64_t sm = as_global(L.s_in)[t] & L.alive_in;
   const unsigned long long pad_job = static_cast<unsigned long long>(L.n_strips) * 64;  // zero source row
   unsigned long long* const jobs = jobs_all[wave];
   uint32_t cnt[kBlockCounters];
@@ -1673,6 +1613,11 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
     const uint32_t y = valid ? __builtin_nontemporal_load(csc_row + p) : 0u;
     const uint64_t pl = valid ? __builtin_nontemporal_load(csc_payload + p) : 0ull;
     const uint64_t rm_l = (pl & kPayloadDense) ? rmask[i] : 0ull;
+    if (valid && sm) {
+      unsigned long long* const s1 = reinterpret_cast<unsigned long long*>(L.s_out + y);
+      const uint64_t cur = __hip_atomic_load(s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((cur | sm) != cur) atomicOr(s1, static_cast<unsigned long long>(sm));
+    }
     const bool act = valid && vm != 0 && pl != 0;
     const bool dense = act && (pl & kPayloadDense) != 0;
     const uint64_t ent = (act && !dense) ? pl : 0ull;
@@ -1731,7 +1676,8 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
 }
 
 // One wave per strip: N and V masks and popcount of X_1 from the rows marked in Z, and the
-// per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1).
+// per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1); per tile column
+// t = the strip, the tile products and used tiles: col_count[t] tiles with S_0[t] & alive.
 template <int W>
 __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel* __restrict__ lv,
                                                                const uint32_t* __restrict__ panels) {
@@ -1745,6 +1691,7 @@ __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel*
   if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t n_waves = gridDim.x * kWaves;
+  uint64_t tp = 0, used = 0;  // lane 0
   for (uint32_t y = read_first_lane(blockIdx.x * kWaves + threadIdx.x / kWave); y < L.n_strips; y += n_waves) {
     const uint64_t z = as_global(L.z_out)[y];
     uint64_t nb = 0, vb = 0;
@@ -1774,8 +1721,16 @@ __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel*
       }
       cnt[bNumeric] += __popcll(nb);
       cnt[bOut] += __popcll(as_global(L.s_out)[y]);
-      cnt[bLeft] += __popcll(as_global(L.s_in)[y] & L.alive_in);
+      const uint64_t left = as_global(L.s_in)[y] & L.alive_in;
+      cnt[bLeft] += __popcll(left);
+      const uint64_t cc = as_global(L.col_count)[y];
+      tp += static_cast<uint64_t>(__popcll(left)) * cc;
+      used += left ? cc : 0ull;
     }
+  }
+  if (lane == 0) {
+    if (tp) atomicAdd(&s_cnt[bTP], static_cast<unsigned long long>(tp));
+    if (used) atomicAdd(&s_cnt[bUsed], static_cast<unsigned long long>(used));
   }
   flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
 }
@@ -1956,7 +1911,7 @@ struct GraphPlan {
   // Job list of A^T for the segmented hop kernel (build_job_list): empty unless built.
   DevBuf<uint32_t> jobs;
   DevBuf<uint32_t> job_off;
-  DevBuf<uint32_t> col_count;  // tiles per tile column of A^T (built with the job list)
+  DevBuf<uint32_t> col_count;  // tiles per tile column of A^T (with the column index or the job list)
   bool has_jobs = false;
 };
 
@@ -2022,12 +1977,14 @@ void build_job_list(GraphPlan& g, hipStream_t st) {
   } else {
     MG_HIP(hipMemsetAsync(g.job_off.get(), 0, sizeof(uint32_t), st));
   }
-  const uint32_t ns = std::max<uint32_t>(1, g.n_strips);
-  g.col_count.alloc(ns);
-  MG_HIP(hipMemsetAsync(g.col_count.get(), 0, sizeof(uint32_t) * ns, st));
-  if (n) {
-    column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.col, n, g.col_count.get());
-    MG_LAUNCH_CHECK();
+  if (!g.col_count.get()) {  // else the column index built it
+    const uint32_t ns = std::max<uint32_t>(1, g.n_strips);
+    g.col_count.alloc(ns);
+    MG_HIP(hipMemsetAsync(g.col_count.get(), 0, sizeof(uint32_t) * ns, st));
+    if (n) {
+      column_count_kernel<<<grid_for(n, kBlock), kBlock, 0, st>>>(g.col, n, g.col_count.get());
+      MG_LAUNCH_CHECK();
+    }
   }
   MG_HIP(hipStreamSynchronize(st));  // bits is freed on return
   g.has_jobs = true;
@@ -2195,6 +2152,10 @@ void build_column_index(GraphPlan& g, hipStream_t st) {
   MG_HIP(hipStreamSynchronize(st));
   g.host_csc_off.assign(ns + 1ull, 0);
   for (uint32_t t = 0; t < ns; ++t) g.host_csc_off[t + 1] = g.host_csc_off[t] + count[t];
+  if (!g.col_count.get()) {
+    g.col_count.alloc(std::max<uint32_t>(1, ns));
+    if (ns) MG_HIP(hipMemcpyAsync(g.col_count.get(), count.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, st));
+  }
   if (ns)
     MG_HIP(hipMemcpyAsync(cursor.get(), g.host_csc_off.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, st));
   g.csc_tile.alloc(std::max<uint32_t>(1, n));
@@ -2253,11 +2214,9 @@ struct Engine {
   DevBuf<SourceItem> sources;
   uint32_t n_sources = 0;
   int prune = 0;
-  // First hop by push: panels whose level 0 is pushed (their level-0 chunks go to push_chunks,
-  // which only the structural pass sweeps), their column items and the panel list.
+  // First hop by push: panels whose level 0 is pushed (their level-0 chunks are left out of
+  // the level-0 launch), their column items and the panel list.
   std::vector<uint8_t> push0;
-  DevBuf<Chunk> push_chunks;
-  uint32_t n_push_chunks = 0;
   DevBuf<PushItem> push_items;
   uint32_t n_push_items = 0;
   DevBuf<uint32_t> push_panels;
@@ -2310,7 +2269,7 @@ void build_engine(Engine* e, hipStream_t st) {
   e->n_split.assign(e->max_hops, 0);
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
-    std::vector<Chunk> ch, pch;
+    std::vector<Chunk> ch;
     std::vector<uint64_t> cost;  // per entry of ch
     std::vector<StripItem> sp;
     for (uint32_t pi = 0; pi < P; ++pi) {
@@ -2327,6 +2286,7 @@ void build_engine(Engine* e, hipStream_t st) {
       L.payload = g.payload.get();
       L.dense = g.dense.get();
       L.csc_tile = g.csc_tile.get();
+      L.col_count = g.col_count.get();
       L.csc_row = g.csc_row.get();
       L.csc_payload = g.csc_payload.get();
       L.x_in = p.x[k].get();
@@ -2345,9 +2305,7 @@ void build_engine(Engine* e, hipStream_t st) {
       for (size_t i = 0; i < g.chunks.size(); ++i) {
         Chunk c = g.chunks[i];
         c.panel = pi;
-        if (pushed) {
-          pch.push_back(c);
-        } else {
+        if (!pushed) {
           ch.push_back(c);
           cost.push_back(g.chunk_cost[i]);
         }
@@ -2382,16 +2340,10 @@ void build_engine(Engine* e, hipStream_t st) {
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
     if (!ch.empty()) MG_HIP(hipMemcpyAsync(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, st));
-    if (k == 0) {
-      e->n_push_chunks = static_cast<uint32_t>(pch.size());
-      e->push_chunks.alloc(std::max<size_t>(1, pch.size()));
-      if (!pch.empty())
-        MG_HIP(hipMemcpyAsync(e->push_chunks.get(), pch.data(), sizeof(Chunk) * pch.size(), hipMemcpyHostToDevice, st));
-    }
     e->n_split[k] = static_cast<uint32_t>(sp.size());
     e->split[k].alloc(std::max<size_t>(1, sp.size()));
     if (!sp.empty()) MG_HIP(hipMemcpyAsync(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice, st));
-    MG_HIP(hipStreamSynchronize(st));  // lv, ch, pch and sp are about to go out of scope
+    MG_HIP(hipStreamSynchronize(st));  // lv, ch and sp are about to go out of scope
   }
   // X_0 fills: frontier and masks of every panel, plus all counters.
   std::vector<PanelInit> inits(P);
@@ -2441,7 +2393,7 @@ void apply_seg(Engine* e, uint32_t G, hipStream_t st) {
       const GraphPlan& g = e->graphs[e->panels[pi].graph];
       lv[pi].jobs = g.has_jobs ? g.jobs.get() : nullptr;
       lv[pi].job_off = g.has_jobs ? g.job_off.get() : nullptr;
-      lv[pi].col_count = g.has_jobs ? g.col_count.get() : nullptr;
+      lv[pi].col_count = g.col_count.get();
       lv[pi].part = e->panels[pi].part.get();
     }
     MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * lv.size(), hipMemcpyHostToDevice, st));
@@ -2523,14 +2475,11 @@ void launch_level(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   }
 }
 
-// First hop by push for the panels listed in `panels` (level 0): `chunks` are their level-0
-// scheduler chunks (swept by the structural pass only), `items` their source columns; all
-// already on the device. max_strips bounds the strips of any listed panel.
+// First hop by push for the panels listed in `panels` (level 0): `items` are their source
+// columns, already on the device. max_strips bounds the strips of any listed panel.
 struct PushLaunch {
   const PushItem* items;
   uint32_t n_items;
-  const Chunk* chunks;
-  uint32_t n_chunks;
   const uint32_t* panels;
   uint32_t n_panels;
   uint32_t max_strips;
@@ -2546,10 +2495,6 @@ void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushLaunch& pl, hi
   if (pl.n_panels) {
     push_reset_kernel<W><<<dim3(grid_for(pl.max_strips, kWaves, 4096), pl.n_panels), kBlock, 0, st>>>(e->lv[0].get(),
                                                                                                     pl.panels);
-    MG_LAUNCH_CHECK();
-  }
-  if (pl.n_chunks) {
-    push_structural_kernel<<<pl.n_chunks, kBlock, 0, st>>>(pl.chunks, e->lv[0].get());
     MG_LAUNCH_CHECK();
   }
   if (pl.n_items) {
@@ -2826,8 +2771,8 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
       hipEvent_t* const evk = w->timing ? &ev[4 * k] : nullptr;
       if (k == 0 && push)
         launch_push_first_hop(st, &w->e,
-                              PushLaunch{w->push_items.get(), n_items, w->e.chunks[0].get(), w->e.n_chunks[0],
-                                         w->e.push_panels.get(), 1u, w->e.panels[0].n_strips},
+                              PushLaunch{w->push_items.get(), n_items, w->e.push_panels.get(), 1u,
+                                         w->e.panels[0].n_strips},
                               evk);
       else
         launch_level(st, &w->e, k, evk);
@@ -3050,8 +2995,8 @@ void record_batch(hipStream_t st, mg_walk_batch* b) {
   for (uint32_t k = 0; k < e.max_hops; ++k) {
     if (k == 0 && e.n_push_panels)
       launch_push_first_hop(st, &e,
-                            PushLaunch{e.push_items.get(), e.n_push_items, e.push_chunks.get(), e.n_push_chunks,
-                                       e.push_panels.get(), e.n_push_panels, e.push_max_strips},
+                            PushLaunch{e.push_items.get(), e.n_push_items, e.push_panels.get(), e.n_push_panels,
+                                       e.push_max_strips},
                             nullptr);
     launch_level(st, &e, k, nullptr);  // the panels whose first hop is swept
   }
