@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__(kSegBlock) walk_seg_kernel(const Chunk* __rest
 // R mod 8, so a light strip still spreads over the waves), source rows whose V bit is clear
 // skipped, one barrier, and the write-out shared with the sweep kernel.
 #ifndef MG_LEAN_UNROLL
-#define MG_LEAN_UNROLL 4
+#define MG_LEAN_UNROLL 2
 #endif
 constexpr int kLeanWaves = 8;
 constexpr int kLeanBlock = kLeanWaves * kWave;
@@ -1152,30 +1152,44 @@ struct LeanSet {
   uint64_t rows[kLeanUnroll];  // target rows of the gathered source row (0: none)
 };
 
-// Issues the gathers of the next kLeanUnroll sources of `m` (wave-uniform: lanes of `job` still
-// to gather): a scalar row address plus the lane's word. Jobs of one source that follow each
-// other in a run (the job list is column-major within a tile) share one gather: their rows are
-// collected into one mask and their lanes cleared from `m`.
-__device__ inline void lean_issue(LeanSet& s, uint64_t& m, uint32_t job, gptr<const uint64_t> x_in, uint32_t lane) {
+// Groups of a batch (`job`: one job per lane, `ok`: the lanes to gather). Jobs of one source
+// that follow each other in a run of kLeanRun lanes (the job list is column-major within a
+// tile) form one group, gathered once; `rows` becomes, in the group's first lane, the target
+// rows of the whole group. Returns the first lanes of the groups. Vector work only: the
+// scalar unit, which issues one instruction per cycle for the whole CU, is kept for the
+// gathers' own bookkeeping.
+__device__ inline uint64_t lean_groups(uint32_t job, uint64_t ok, uint32_t lane, uint64_t& rows) {
+  const uint32_t u = job >> 6;
+  rows = 1ull << (job & 63u);
+  const uint32_t u_prev = static_cast<uint32_t>(__shfl_up(static_cast<int>(u), 1, kWave));
+  const bool cont = (lane % kLeanRun) != 0 && ((ok >> lane) & 1) && ((ok >> (lane - 1)) & 1) && u_prev == u;
+  const uint64_t cm = ballot(cont);
+  if (cm) {
+    const bool next_cont = lane + 1 < kWave && ((cm >> (lane + 1)) & 1);
+    // One step per further job of the longest group: each lane ORs in its successor's rows.
+    for (uint64_t ck = cm; ck; ck &= ck >> 1) {
+      const uint64_t nxt = shfl64(rows, lane + 1 < kWave ? lane + 1 : lane);
+      if (next_cont) rows |= nxt;
+    }
+  }
+  return ok & ~cm;
+}
+
+// Issues the gathers of the next kLeanUnroll groups of `starts` (wave-uniform: first lanes of
+// the groups still to gather): a scalar row address plus the lane's word.
+__device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint32_t job, uint64_t rows,
+                                  gptr<const uint64_t> x_in, uint32_t lane) {
 #pragma unroll
   for (int k = 0; k < kLeanUnroll; ++k) {
     s.rows[k] = 0;
     s.v[k] = 0;
-    if (m) {
-      const int src = __ffsll(static_cast<unsigned long long>(m)) - 1;
-      m &= m - 1;
-      const uint32_t jk = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), src));
-      const uint32_t u = jk >> 6;
-      uint64_t rows = 1ull << (jk & 63u);
-      for (int l = src + 1; (l % static_cast<int>(kLeanRun)) != 0 && ((m >> l) & 1); ++l) {
-        const uint32_t jl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), l));
-        if ((jl >> 6) != u) break;
-        rows |= 1ull << (jl & 63u);
-        m &= ~(1ull << l);
-      }
+    if (starts) {
+      const int src = __ffsll(static_cast<unsigned long long>(starts)) - 1;
+      starts &= starts - 1;
+      const uint32_t u = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), src)) >> 6;
       const gptr<const uint64_t> row = x_in + static_cast<uint64_t>(u) * 64u;
       s.v[k] = row[lane];
-      s.rows[k] = rows;
+      s.rows[k] = read_lane(rows, src);
     }
   }
 }
@@ -1185,14 +1199,18 @@ __device__ inline void lean_issue(LeanSet& s, uint64_t& m, uint32_t job, gptr<co
 __device__ inline void lean_commit(const LeanSet& s, char* __restrict__ acc_lane, uint64_t keep, bool count_rows,
                                    uint32_t& jobs_ok) {
 #pragma unroll
-  for (int k = 0; k < kLeanUnroll; ++k)
-    if (s.rows[k]) {
+  for (int k = 0; k < kLeanUnroll; ++k) {
+    const uint64_t r = s.rows[k];
+    if (r) {
       const uint64_t val = s.v[k] & keep;
-      if (count_rows && ballot(val != 0)) jobs_ok += static_cast<uint32_t>(__popcll(s.rows[k]));
-      for (uint64_t r = s.rows[k]; r; r &= r - 1)
-        atomicOr(reinterpret_cast<unsigned long long*>(acc_lane + 512u * (__ffsll(static_cast<unsigned long long>(r)) - 1)),
+      if (count_rows && ballot(val != 0)) jobs_ok += static_cast<uint32_t>(__popcll(r));
+      atomicOr(reinterpret_cast<unsigned long long*>(acc_lane + 512u * (__ffsll(static_cast<unsigned long long>(r)) - 1)),
+               val);
+      for (uint64_t rest = r & (r - 1); rest; rest &= rest - 1)
+        atomicOr(reinterpret_cast<unsigned long long*>(acc_lane + 512u * (__ffsll(static_cast<unsigned long long>(rest)) - 1)),
                  val);
     }
+  }
 }
 
 __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu(kLeanWaves, kLeanWaves)))
@@ -1320,24 +1338,26 @@ __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu
       next_batch(p_strip, p_base, n_strip, n_base);
       const uint32_t job_n = load_batch(n_strip, n_base);
       const bool ok = check_v ? ((vw >> ((job >> 6) & 63u)) & 1) : job != kNoJob;
-      uint64_t m = ballot(ok);
-      if (check_v) edges += static_cast<uint32_t>(__popcll(m));
+      const uint64_t okm = ballot(ok);
+      if (check_v) edges += static_cast<uint32_t>(__popcll(okm));
       const bool count_rows = !check_v;
+      uint64_t rows;
+      uint64_t m = lean_groups(job, okm, lane, rows);
       LeanSet a, b;
-      lean_issue(a, m, job, x_in, lane);
+      lean_issue(a, m, job, rows, x_in, lane);
       const uint64_t vw_n = (check_v && job_n != kNoJob) ? v_in[job_n >> 12] : 0ull;
       for (;;) {
         if (!m) {
           lean_commit(a, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(b, m, job, x_in, lane);
+        lean_issue(b, m, job, rows, x_in, lane);
         lean_commit(a, acc_lane, keep, count_rows, edges);
         if (!m) {
           lean_commit(b, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(a, m, job, x_in, lane);
+        lean_issue(a, m, job, rows, x_in, lane);
         lean_commit(b, acc_lane, keep, count_rows, edges);
       }
       job = job_n;
