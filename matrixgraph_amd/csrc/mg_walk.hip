@@ -1176,9 +1176,9 @@ __device__ inline uint64_t lean_groups(uint32_t job, uint64_t ok, uint32_t lane,
 }
 
 // Issues the gathers of the next kLeanUnroll groups of `starts` (wave-uniform: first lanes of
-// the groups still to gather): a scalar row address plus the lane's word.
-__device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint32_t job, uint64_t rows,
-                                  gptr<const uint64_t> x_in, uint32_t lane) {
+// the groups still to gather): the row address (`row_addr`, one per lane of the batch, made by
+// vector instructions) is read out as a scalar, plus the lane's word.
+__device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint64_t row_addr, uint64_t rows, uint32_t lane) {
 #pragma unroll
   for (int k = 0; k < kLeanUnroll; ++k) {
     s.rows[k] = 0;
@@ -1186,8 +1186,7 @@ __device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint32_t job, ui
     if (starts) {
       const int src = __ffsll(static_cast<unsigned long long>(starts)) - 1;
       starts &= starts - 1;
-      const uint32_t u = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(job), src)) >> 6;
-      const gptr<const uint64_t> row = x_in + static_cast<uint64_t>(u) * 64u;
+      const gptr<const uint64_t> row = reinterpret_cast<gptr<const uint64_t>>(read_lane(row_addr, src));
       s.v[k] = row[lane];
       s.rows[k] = read_lane(rows, src);
     }
@@ -1343,21 +1342,22 @@ __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu
       const bool count_rows = !check_v;
       uint64_t rows;
       uint64_t m = lean_groups(job, okm, lane, rows);
+      const uint64_t row_addr = reinterpret_cast<uint64_t>(x_in) + static_cast<uint64_t>(job >> 6) * 512u;
       LeanSet a, b;
-      lean_issue(a, m, job, rows, x_in, lane);
+      lean_issue(a, m, row_addr, rows, lane);
       const uint64_t vw_n = (check_v && job_n != kNoJob) ? v_in[job_n >> 12] : 0ull;
       for (;;) {
         if (!m) {
           lean_commit(a, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(b, m, job, rows, x_in, lane);
+        lean_issue(b, m, row_addr, rows, lane);
         lean_commit(a, acc_lane, keep, count_rows, edges);
         if (!m) {
           lean_commit(b, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(a, m, job, rows, x_in, lane);
+        lean_issue(a, m, row_addr, rows, lane);
         lean_commit(b, acc_lane, keep, count_rows, edges);
       }
       job = job_n;
