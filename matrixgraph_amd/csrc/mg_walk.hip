@@ -118,6 +118,12 @@ struct Chunk {
 struct StripItem {
   uint32_t panel, strip;
 };
+// A strip split across chunks at some level: its rows of that level's output frontier and its
+// S word, zeroed at the start of every run (its pieces OR into them).
+struct ZeroItem {
+  uint64_t* x;
+  uint64_t* s;
+};
 // Source s of a panel: bit `col` of the X_0 row of vertex `vertex`.
 struct SourceItem {
   uint32_t panel, col, vertex;
@@ -1461,15 +1467,14 @@ __global__ void __launch_bounds__(kBlock) seg_combine_kernel(const PanelLevel* _
 }
 
 // Zero the strips that are written by several chunks (atomic OR targets) and their
-// structural masks.
+// structural masks, for every level at once, at the start of a run (nothing reads or writes a
+// level's output before its own launch).
 template <int W>
-__global__ void __launch_bounds__(kBlock) zero_strips_kernel(const StripItem* __restrict__ items,
-                                                             const PanelLevel* __restrict__ lv) {
+__global__ void __launch_bounds__(kBlock) zero_split_kernel(const ZeroItem* __restrict__ items) {
   constexpr int kWords = 64 * W;
-  const StripItem it = items[blockIdx.x];
-  uint64_t* dst = lv[it.panel].x_out + static_cast<uint64_t>(it.strip) * kWords;
-  for (int i = threadIdx.x; i < kWords; i += kBlock) dst[i] = 0;
-  if (threadIdx.x == 0) lv[it.panel].s_out[it.strip] = 0;
+  const ZeroItem it = items[blockIdx.x];
+  for (int i = threadIdx.x; i < kWords; i += kBlock) it.x[i] = 0;
+  if (threadIdx.x == 0) *it.s = 0;
 }
 
 // Masks, popcount and tile counts of the split strips once every piece has been OR-ed in.
@@ -2226,6 +2231,8 @@ struct Engine {
   std::vector<uint32_t> n_chunks;
   std::vector<DevBuf<StripItem>> split;
   std::vector<uint32_t> n_split;
+  DevBuf<ZeroItem> zero_items;  // the split strips of every level
+  uint32_t n_zero_items = 0;
   // X_0 setup
   DevBuf<PanelInit> inits;
   DevBuf<uint32_t> panel_strips;
@@ -2287,6 +2294,7 @@ void build_engine(Engine* e, hipStream_t st) {
   e->n_chunks.assign(e->max_hops, 0);
   e->split.resize(e->max_hops);
   e->n_split.assign(e->max_hops, 0);
+  std::vector<ZeroItem> zero_items;
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
     std::vector<Chunk> ch;
@@ -2331,7 +2339,10 @@ void build_engine(Engine* e, hipStream_t st) {
         }
       }
       if (!pushed)
-        for (uint32_t y : g.split) sp.push_back({pi, y});
+        for (uint32_t y : g.split) {
+          sp.push_back({pi, y});
+          zero_items.push_back({p.x[k + 1].get() + static_cast<uint64_t>(y) * 64 * e->W, p.s[k + 1].get() + y});
+        }
     }
     // Heaviest chunks first (chunk_costs) so the tail of the launch is made of light ones.
     // Among equal costs, the same chunk of every panel of one graph is adjacent, so the
@@ -2365,6 +2376,11 @@ void build_engine(Engine* e, hipStream_t st) {
     if (!sp.empty()) MG_HIP(hipMemcpyAsync(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice, st));
     MG_HIP(hipStreamSynchronize(st));  // lv, ch and sp are about to go out of scope
   }
+  e->n_zero_items = static_cast<uint32_t>(zero_items.size());
+  e->zero_items.alloc(std::max<size_t>(1, zero_items.size()));
+  if (!zero_items.empty())
+    MG_HIP(hipMemcpyAsync(e->zero_items.get(), zero_items.data(), sizeof(ZeroItem) * zero_items.size(),
+                          hipMemcpyHostToDevice, st));
   // X_0 fills: frontier and masks of every panel, plus all counters.
   std::vector<PanelInit> inits(P);
   std::vector<uint32_t> strips(P);
@@ -2445,10 +2461,6 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
   // The structural pass runs inside the numeric kernel: its event pair brackets nothing.
   if (ev) MG_HIP(hipEventRecord(ev[0], st));
   if (ev) MG_HIP(hipEventRecord(ev[1], st));
-  if (e->n_split[k]) {
-    zero_strips_kernel<W><<<e->n_split[k], kBlock, 0, st>>>(e->split[k].get(), e->lv[k].get());
-    MG_LAUNCH_CHECK();
-  }
   if (ev) MG_HIP(hipEventRecord(ev[2], st));
   const uint32_t G = (W == 64 && e->seg && !e->logging && k >= e->seg_from) ? e->seg : 0;
   if (e->n_chunks[k]) {
@@ -2563,6 +2575,10 @@ void launch_clear_x0(hipStream_t st, Engine* e) {
   clear_x0_kernel<W><<<2048, kBlock, 0, st>>>(e->inits.get(), e->panel_strips.get(),
                                               static_cast<uint32_t>(e->panels.size()));
   MG_LAUNCH_CHECK();
+  if (e->n_zero_items) {
+    zero_split_kernel<W><<<e->n_zero_items, kBlock, 0, st>>>(e->zero_items.get());
+    MG_LAUNCH_CHECK();
+  }
 }
 
 void launch_init(hipStream_t st, Engine* e, uint32_t n_sources) {
