@@ -1527,11 +1527,12 @@ __global__ void __launch_bounds__(kBlock) finalize_strips_kernel(const StripItem
 // X_0 is one-hot: only the A^T tile columns of the strips that hold sources carry numeric
 // work into X_1. The first hop of a walk can therefore read just those columns (through a
 // column index of A^T) and OR each source row into its target rows with global atomics,
-// instead of sweeping all strips with the LDS accumulator. Three launches produce the same
-// X_1, masks and counters as the sweep: push_reset (clear the previous run's words of X_1,
-// reset its masks), push_first_hop (the jobs, and S_1: only the source strips' columns add to
-// it) and push_finalize (N, V masks, popcount and strip counts, read back from the rows the
-// pushes marked in Z, and the tile counts per column).
+// instead of sweeping all strips with the LDS accumulator. Three or four launches produce the
+// same X_1, masks and counters as the sweep: push_reset (clear the previous run's words of
+// X_1, reset its masks), S_1 and the tile counts (either push_structural over the row-major
+// index, or push_first_hop from the source columns plus per-column counts in push_finalize),
+// push_first_hop (the jobs) and push_finalize (N, V masks, popcount and strip counts, read
+// back from the rows the pushes marked in Z).
 
 // Positions [begin, end) of the A^T column index that belong to source strip `strip`.
 struct PushItem {
@@ -1569,6 +1570,72 @@ __global__ void __launch_bounds__(kBlock) push_reset_kernel(const PanelLevel* __
   }
 }
 
+// S_1 and the tile-product counts of the first hop when the pushed columns hold many of the
+// tiles (batches of large queries): per A^T tile (y, t), S_1[y] |= S_0[t] & alive. One
+// workgroup per scheduler chunk; the chunk's strips are reduced in LDS and stored (pieces of
+// split strips OR into S_1, zeroed by push_reset). With few pushed tiles (a walk from random
+// sources) push_first_hop ORs S_1 from the source columns instead and push_finalize counts
+// the tile products per column.
+__global__ void __launch_bounds__(kBlock) push_structural_kernel(const Chunk* __restrict__ chunks,
+                                                                 const PanelLevel* __restrict__ lv) {
+  __shared__ unsigned long long s_sym[kChunkStrips];
+  __shared__ unsigned long long s_cnt[kBlockCounters];
+  const Chunk ch = chunks[blockIdx.x];
+  const PanelLevel& L = lv[ch.panel];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const gptr<const uint32_t> col = as_global(L.col);
+  const gptr<const uint32_t> row = as_global(L.row);
+  const gptr<const uint64_t> s_in = as_global(L.s_in);
+  const uint64_t alive = L.alive_in;
+  uint32_t cnt[kBlockCounters];
+#pragma unroll
+  for (int i = 0; i < kBlockCounters; ++i) cnt[i] = 0;
+  if (threadIdx.x < kBlockCounters) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < kChunkStrips) s_sym[threadIdx.x] = 0;
+  __syncthreads();
+  // kGroups groups of 64 tiles per wave step, their index loads issued together.
+  constexpr int kGroups = 4;
+  for (uint32_t g0 = ch.tile_begin + 64 * wave; g0 < ch.tile_end; g0 += 64 * kWaves * kGroups) {
+    uint32_t t[kGroups], y[kGroups];
+    uint64_t sv[kGroups];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      const uint32_t p = g0 + 64 * kWaves * q + lane;
+      const bool v = p < ch.tile_end;
+      t[q] = v ? __builtin_nontemporal_load(col + p) : 0xFFFFFFFFu;
+      y[q] = v ? __builtin_nontemporal_load(row + p) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) sv[q] = t[q] != 0xFFFFFFFFu ? (s_in[t[q]] & alive) : 0ull;
+    // A group mostly lies in one strip: lanes in lane 0's strip are reduced across the wave
+    // first, so the LDS ORs do not all queue on one address.
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) {
+      cnt[bTP] += __popcll(sv[q]);
+      cnt[bUsed] += sv[q] != 0;
+      const uint32_t y0 = read_first_lane(y[q]);
+      const bool same = y[q] == y0;
+      const uint64_t red = wave_or(same ? sv[q] : 0ull);
+      if (lane == 0 && red) atomicOr(&s_sym[y0 - ch.strip_begin], static_cast<unsigned long long>(red));
+      if (!same && sv[q]) atomicOr(&s_sym[y[q] - ch.strip_begin], static_cast<unsigned long long>(sv[q]));
+    }
+  }
+  __syncthreads();
+  const uint32_t n_cs = ch.strip_end - ch.strip_begin;
+  if (threadIdx.x < n_cs) {
+    const uint32_t y = ch.strip_begin + threadIdx.x;
+    const gptr<const uint32_t> off = as_global(L.off);
+    const bool split = ch.tile_begin != off[y] || ch.tile_end != off[y + 1];
+    if (split) {
+      if (s_sym[threadIdx.x]) atomicOr((unsigned long long*)(L.s_out + y), s_sym[threadIdx.x]);
+    } else {
+      as_global(L.s_out)[y] = s_sym[threadIdx.x];
+    }
+  }
+  flush_block_counters(cnt, s_cnt, as_global(L.counters), nullptr);
+}
+
 // Pushes the first n jobs of a wave list; a job is (target vertex << 32) | source vertex.
 // Each source row is loaded whole (lane = word) and its non-zero words are OR-ed into the
 // target row; the target row is marked in Z.
@@ -1599,13 +1666,14 @@ __device__ inline void push_drain(unsigned long long* __restrict__ jobs, uint32_
 }
 
 // One workgroup per item; its waves take the item's column positions in groups of 64
-// (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job. Every
-// tile (y, t) of the column also ORs S_0[t] & alive into S_1[y] (structural X_1; tiles of other
-// columns add nothing): a hub strip y is reached from most source columns, so S_1[y] is read
-// first and only a missing bit costs an atomic.
+// (lane = tile). Per set bit of a tile whose source vertex is non-empty in X_0: one job. With
+// s_from_columns, every tile (y, t) of the column also ORs S_0[t] & alive into S_1[y]
+// (structural X_1; tiles of other columns add nothing): a hub strip y is reached from most
+// source columns, so S_1[y] is read first and only a missing bit costs an atomic.
 template <int W>
 __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* __restrict__ items,
-                                                                const PanelLevel* __restrict__ lv) {
+                                                                const PanelLevel* __restrict__ lv,
+                                                                bool s_from_columns) {
   __shared__ unsigned long long jobs_all[kWaves][kPushCap];
   __shared__ unsigned long long s_cnt[kBlockCounters];
   const uint32_t lane = lane_id();
@@ -1622,7 +1690,7 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
   const gptr<uint64_t> z_out = as_global(L.z_out);
   const uint32_t t = it.strip;
   const uint64_t vm = as_global(L.v_in)[t];
-  const uint64_t sm = as_global(L.s_in)[t] & L.alive_in;
+  const uint64_t sm = s_from_columns ? (as_global(L.s_in)[t] & L.alive_in) : 0ull;
   const unsigned long long pad_job = static_cast<unsigned long long>(L.n_strips) * 64;  // zero source row
   unsigned long long* const jobs = jobs_all[wave];
   uint32_t cnt[kBlockCounters];
@@ -1701,11 +1769,13 @@ __global__ void __launch_bounds__(kBlock) push_first_hop_kernel(const PushItem* 
 }
 
 // One wave per strip: N and V masks and popcount of X_1 from the rows marked in Z, and the
-// per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1); per tile column
-// t = the strip, the tile products and used tiles: col_count[t] tiles with S_0[t] & alive.
+// per-strip tile counts (T_L from S_0, T_C from S_1, numeric tiles from N_1); with
+// column_counts, per tile column t = the strip, the tile products and used tiles: col_count[t]
+// tiles with S_0[t] & alive.
 template <int W>
 __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel* __restrict__ lv,
-                                                               const uint32_t* __restrict__ panels) {
+                                                               const uint32_t* __restrict__ panels,
+                                                               bool column_counts) {
   __shared__ unsigned long long s_cnt[kBlockCounters];
   const PanelLevel& L = lv[panels[blockIdx.y]];
   const uint32_t lane = lane_id();
@@ -1748,7 +1818,7 @@ __global__ void __launch_bounds__(kBlock) push_finalize_kernel(const PanelLevel*
       cnt[bOut] += __popcll(as_global(L.s_out)[y]);
       const uint64_t left = as_global(L.s_in)[y] & L.alive_in;
       cnt[bLeft] += __popcll(left);
-      const uint64_t cc = as_global(L.col_count)[y];
+      const uint64_t cc = column_counts ? as_global(L.col_count)[y] : 0ull;
       tp += static_cast<uint64_t>(__popcll(left)) * cc;
       used += left ? cc : 0ull;
     }
@@ -2241,9 +2311,12 @@ struct Engine {
   DevBuf<SourceItem> sources;
   uint32_t n_sources = 0;
   int prune = 0;
-  // First hop by push: panels whose level 0 is pushed (their level-0 chunks are left out of
-  // the level-0 launch), their column items and the panel list.
+  // First hop by push: panels whose level 0 is pushed (their level-0 chunks go to push_chunks,
+  // which only the structural pass sweeps), their column items and the panel list.
   std::vector<uint8_t> push0;
+  DevBuf<Chunk> push_chunks;
+  uint32_t n_push_chunks = 0;
+  bool push_s_from_columns = false;
   DevBuf<PushItem> push_items;
   uint32_t n_push_items = 0;
   DevBuf<uint32_t> push_panels;
@@ -2297,7 +2370,7 @@ void build_engine(Engine* e, hipStream_t st) {
   std::vector<ZeroItem> zero_items;
   for (uint32_t k = 0; k < e->max_hops; ++k) {
     std::vector<PanelLevel> lv(P);
-    std::vector<Chunk> ch;
+    std::vector<Chunk> ch, pch;
     std::vector<uint64_t> cost;  // per entry of ch
     std::vector<StripItem> sp;
     for (uint32_t pi = 0; pi < P; ++pi) {
@@ -2333,7 +2406,9 @@ void build_engine(Engine* e, hipStream_t st) {
       for (size_t i = 0; i < g.chunks.size(); ++i) {
         Chunk c = g.chunks[i];
         c.panel = pi;
-        if (!pushed) {
+        if (pushed) {
+          pch.push_back(c);
+        } else {
           ch.push_back(c);
           cost.push_back(g.chunk_cost[i]);
         }
@@ -2371,10 +2446,16 @@ void build_engine(Engine* e, hipStream_t st) {
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
     if (!ch.empty()) MG_HIP(hipMemcpyAsync(e->chunks[k].get(), ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, st));
+    if (k == 0) {
+      e->n_push_chunks = static_cast<uint32_t>(pch.size());
+      e->push_chunks.alloc(std::max<size_t>(1, pch.size()));
+      if (!pch.empty())
+        MG_HIP(hipMemcpyAsync(e->push_chunks.get(), pch.data(), sizeof(Chunk) * pch.size(), hipMemcpyHostToDevice, st));
+    }
     e->n_split[k] = static_cast<uint32_t>(sp.size());
     e->split[k].alloc(std::max<size_t>(1, sp.size()));
     if (!sp.empty()) MG_HIP(hipMemcpyAsync(e->split[k].get(), sp.data(), sizeof(StripItem) * sp.size(), hipMemcpyHostToDevice, st));
-    MG_HIP(hipStreamSynchronize(st));  // lv, ch and sp are about to go out of scope
+    MG_HIP(hipStreamSynchronize(st));  // lv, ch, pch and sp are about to go out of scope
   }
   e->n_zero_items = static_cast<uint32_t>(zero_items.size());
   e->zero_items.alloc(std::max<size_t>(1, zero_items.size()));
@@ -2508,14 +2589,22 @@ void launch_level(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
 }
 
 // First hop by push for the panels listed in `panels` (level 0): `items` are their source
-// columns, already on the device. max_strips bounds the strips of any listed panel.
+// columns, `chunks` their level-0 scheduler chunks (swept by push_structural unless
+// s_from_columns); all already on the device. max_strips bounds the strips of any listed panel.
 struct PushLaunch {
   const PushItem* items;
   uint32_t n_items;
+  const Chunk* chunks;
+  uint32_t n_chunks;
   const uint32_t* panels;
   uint32_t n_panels;
   uint32_t max_strips;
+  bool s_from_columns;
 };
+
+// S_1 from the source columns (atomics, one per missing bit) pays while the pushed columns hold
+// at most this fraction of the panels' tiles; above it the structural sweep of all tiles does.
+constexpr uint64_t kPushColumnS = 8;  // pushed tiles * kPushColumnS <= tiles
 
 template <int W>
 void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushLaunch& pl, hipEvent_t ev[4]) {
@@ -2529,13 +2618,17 @@ void launch_push_first_hop_w(hipStream_t st, Engine* e, const PushLaunch& pl, hi
                                                                                                     pl.panels);
     MG_LAUNCH_CHECK();
   }
+  if (!pl.s_from_columns && pl.n_chunks) {
+    push_structural_kernel<<<pl.n_chunks, kBlock, 0, st>>>(pl.chunks, e->lv[0].get());
+    MG_LAUNCH_CHECK();
+  }
   if (pl.n_items) {
-    push_first_hop_kernel<W><<<pl.n_items, kBlock, 0, st>>>(pl.items, e->lv[0].get());
+    push_first_hop_kernel<W><<<pl.n_items, kBlock, 0, st>>>(pl.items, e->lv[0].get(), pl.s_from_columns);
     MG_LAUNCH_CHECK();
   }
   if (pl.n_panels) {
     push_finalize_kernel<W><<<dim3(grid_for(pl.max_strips, kWaves, 16384), pl.n_panels), kBlock, 0, st>>>(
-        e->lv[0].get(), pl.panels);
+        e->lv[0].get(), pl.panels, pl.s_from_columns);
     MG_LAUNCH_CHECK();
   }
   if (ev) MG_HIP(hipEventRecord(ev[3], st));
@@ -2777,14 +2870,16 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
     // First hop: push from the source strips when their A^T columns hold at most half of the
     // tiles (the sweep reads every tile once; the push reads the source columns).
     uint32_t n_items = 0;
-    bool push = false;
+    bool push = false, s_cols = false;
     if (w->max_push_items) {
       std::vector<uint32_t> strips(n_sources);
       for (uint32_t i = 0; i < n_sources; ++i) strips[i] = gp.engine_vertex(sources[i]) / 64;
       std::sort(strips.begin(), strips.end());
       strips.erase(std::unique(strips.begin(), strips.end()), strips.end());
       const GraphPlan& g = w->e.graphs[0];
-      push = 2 * push_items_for(g, 0, strips, nullptr) <= g.at->n_nz_tile;
+      const uint64_t pushed_tiles = push_items_for(g, 0, strips, nullptr);
+      push = 2 * pushed_tiles <= g.at->n_nz_tile;
+      s_cols = kPushColumnS * pushed_tiles <= g.at->n_nz_tile;
       if (push) {
         std::vector<PushItem> items;
         push_items_for(g, 0, strips, &items);
@@ -2807,8 +2902,8 @@ extern "C" int mg_walk_run(mg_context* ctx, mg_walk* w, uint32_t n_sources, cons
       hipEvent_t* const evk = w->timing ? &ev[4 * k] : nullptr;
       if (k == 0 && push)
         launch_push_first_hop(st, &w->e,
-                              PushLaunch{w->push_items.get(), n_items, w->e.push_panels.get(), 1u,
-                                         w->e.panels[0].n_strips},
+                              PushLaunch{w->push_items.get(), n_items, w->e.chunks[0].get(), w->e.n_chunks[0],
+                                         w->e.push_panels.get(), 1u, w->e.panels[0].n_strips, s_cols},
                               evk);
       else
         launch_level(st, &w->e, k, evk);
@@ -3031,8 +3126,9 @@ void record_batch(hipStream_t st, mg_walk_batch* b) {
   for (uint32_t k = 0; k < e.max_hops; ++k) {
     if (k == 0 && e.n_push_panels)
       launch_push_first_hop(st, &e,
-                            PushLaunch{e.push_items.get(), e.n_push_items, e.push_panels.get(), e.n_push_panels,
-                                       e.push_max_strips},
+                            PushLaunch{e.push_items.get(), e.n_push_items, e.push_chunks.get(), e.n_push_chunks,
+                                       e.push_panels.get(), e.n_push_panels, e.push_max_strips,
+                                       e.push_s_from_columns},
                             nullptr);
     launch_level(st, &e, k, nullptr);  // the panels whose first hop is swept
   }
@@ -3144,6 +3240,7 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
       }
       std::vector<uint8_t> indexed(e.graphs.size(), 0);
       e.push0.assign(e.panels.size(), 0);
+      uint64_t pushed_sum = 0, pushed_panel_tiles = 0;
       for (uint32_t pi = 0; pi < e.panels.size(); ++pi) {
         std::vector<uint32_t>& st = strips[pi];
         std::sort(st.begin(), st.end());
@@ -3154,13 +3251,17 @@ extern "C" int mg_walk_batch_create(mg_context* ctx, const mg_tiled_matrix* cons
           build_column_index(g, ctx->stream);
           indexed[gi] = 1;
         }
-        if (2 * push_items_for(g, pi, st, nullptr) <= g.at->n_nz_tile) {
+        const uint64_t pushed_tiles = push_items_for(g, pi, st, nullptr);
+        if (2 * pushed_tiles <= g.at->n_nz_tile) {
           e.push0[pi] = 1;
+          pushed_sum += pushed_tiles;
+          pushed_panel_tiles += g.at->n_nz_tile;
           push_items_for(g, pi, st, &push_items);
           push_panels.push_back(pi);
           e.push_max_strips = std::max(e.push_max_strips, g.n_strips);
         }
       }
+      e.push_s_from_columns = kPushColumnS * pushed_sum <= pushed_panel_tiles;
     }
     build_engine(&e, ctx->stream);
     apply_seg(&e, default_segments(), ctx->stream);
