@@ -229,9 +229,25 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_kernel(
   }
 }
 
+// Product of two dense tiles through their common inner indices (`common`: the columns in use
+// in both, wave-uniform): for each such j, the rows r with L(r, j) set receive the columns c
+// with Rt(c, j) set (a ballot). The loop runs over the j the two tiles share, where
+// tile_product runs over every non-zero row of Rt; on R-MAT tiles above kWideMax bits that
+// is about half the steps.
+__device__ inline uint64_t tile_product_common(uint64_t l_row, uint64_t rt_row, uint64_t common) {
+  uint64_t out = 0;
+  for (; common; common &= common - 1) {
+    const int j = __ffsll(static_cast<unsigned long long>(common)) - 1;
+    const uint64_t cols = ballot((rt_row >> j) & 1);
+    if ((l_row >> j) & 1) out |= cols;
+  }
+  return out;
+}
+
 // 16-byte compact payload of a tile with at most kWideMax set bits: entry i = (row << 6) |
 // column in bits 12(i % 5).. of word i / 5, the entry count in bits 60..63 of word 1; count
-// kWideDense marks a denser tile.
+// kWideDense marks a denser tile, whose word 0 then holds the OR of its rows (the columns in
+// use: the inner indices j through which it meets the other operand's tiles).
 typedef uint64_t WidePayload __attribute__((ext_vector_type(2)));
 constexpr uint32_t kWideMax = 10;
 constexpr uint32_t kWideDense = 15;
@@ -249,6 +265,8 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
     const uint64_t word = data[static_cast<uint64_t>(i) * kTile + lane_id()];
     const uint64_t total = wave_sum(__popcll(word));
     WidePayload pl = {0, static_cast<uint64_t>(kWideDense) << 60};
+    const uint64_t cols = wave_or(word);
+    if (total > kWideMax) pl.x = cols;
     if (total <= kWideMax) {
       uint64_t w[2] = {0, 0};
       uint32_t n = 0;
@@ -373,12 +391,13 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
       // whose A row has bit j.
       const bool a_dense = na == kWideDense, b_dense = nb == kWideDense;
       for (uint64_t dm = ballot(dense && !a_dense); dm;) {
-        uint64_t rt_row[kPairBatch];
+        uint64_t rt_row[kPairBatch], used[kPairBatch];
         WidePayload ent[kPairBatch];
         uint32_t n_pairs = 0;  // wave-uniform
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
           rt_row[q] = 0;
+          used[q] = 0;
           ent[q] = WidePayload{0, 0};
           if (dm) {
             const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
@@ -387,6 +406,7 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
             rt_row[q] = data_bt[kb * kTile + lane];
             ent[q].x = read_lane(pa.x, k);
             ent[q].y = read_lane(pa.y, k);
+            used[q] = read_lane(pb.x, k);  // columns in use of the dense Bt tile
             ++n_pairs;
           }
         }
@@ -396,18 +416,20 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
           const uint32_t n = wide_count(ent[q]);
           for (uint32_t u = 0; u < n; ++u) {
             const uint32_t e = wide_entry(ent[q], u);
+            if (!((used[q] >> (e & 63u)) & 1)) continue;  // no Bt row has bit j
             const uint64_t col = ballot((rt_row[q] >> (e & 63u)) & 1);
             if (lane == (e >> 6)) acc |= col;
           }
         }
       }
       for (uint64_t dm = ballot(dense && a_dense && !b_dense); dm;) {
-        uint64_t l_row[kPairBatch];
+        uint64_t l_row[kPairBatch], used[kPairBatch];
         WidePayload ent[kPairBatch];
         uint32_t n_pairs = 0;  // wave-uniform
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
           l_row[q] = 0;
+          used[q] = 0;
           ent[q] = WidePayload{0, 0};
           if (dm) {
             const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
@@ -416,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
             l_row[q] = data_a[ka * kTile + lane];
             ent[q].x = read_lane(pb.x, k);
             ent[q].y = read_lane(pb.y, k);
+            used[q] = read_lane(pa.x, k);  // columns in use of the dense A tile
             ++n_pairs;
           }
         }
@@ -425,18 +448,20 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
           const uint32_t n = wide_count(ent[q]);
           for (uint32_t v = 0; v < n; ++v) {
             const uint32_t e = wide_entry(ent[q], v);
+            if (!((used[q] >> (e & 63u)) & 1)) continue;  // no A row has bit j
             acc |= ((l_row[q] >> (e & 63u)) & 1ull) << (e >> 6);
           }
         }
       }
-      // Both tiles dense: the full-tile wave product.
+      // Both tiles dense: the wave product through the inner indices both tiles use.
       for (uint64_t dm = ballot(dense && a_dense && b_dense); dm;) {
-        uint64_t l_row[kPairBatch], rt_row[kPairBatch];
+        uint64_t l_row[kPairBatch], rt_row[kPairBatch], common[kPairBatch];
         uint32_t n_pairs = 0;  // wave-uniform
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
           l_row[q] = 0;
           rt_row[q] = 0;
+          common[q] = 0;
           if (dm) {
             const int k = __ffsll(static_cast<unsigned long long>(dm)) - 1;
             dm &= dm - 1;
@@ -444,12 +469,13 @@ __global__ void __launch_bounds__(kBlock) fill_tiles_payload_kernel(
             const uint64_t kb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ib), k));
             l_row[q] = data_a[ka * kTile + lane];
             rt_row[q] = data_bt[kb * kTile + lane];
+            common[q] = read_lane(pa.x, k) & read_lane(pb.x, k);
             ++n_pairs;
           }
         }
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q)
-          if (static_cast<uint32_t>(q) < n_pairs) acc |= tile_product(l_row[q], rt_row[q]);
+          if (static_cast<uint32_t>(q) < n_pairs) acc |= tile_product_common(l_row[q], rt_row[q], common[q]);
       }
     }
     wave_lds_sync();
