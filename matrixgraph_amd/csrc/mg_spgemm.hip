@@ -257,12 +257,15 @@ __device__ inline uint32_t wide_entry(WidePayload p, uint32_t i) {
   return static_cast<uint32_t>((i < 5 ? p.x : p.y) >> (12 * (i < 5 ? i : i - 5))) & 0xFFFu;
 }
 
-// Wide payload of every tile, one wave per tile (lane r holds row r).
-__global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restrict__ data, uint32_t n_tiles,
+// Wide payload of every tile of both operands, one wave per tile (lane r holds row r): tiles
+// [0, n_a) of data_a, then the n_bt tiles of data_bt, at the same positions of `payload`.
+__global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restrict__ data_a, uint32_t n_a,
+                                                         const uint64_t* __restrict__ data_bt, uint32_t n_bt,
                                                          WidePayload* __restrict__ payload) {
   const uint32_t n_waves = gridDim.x * kWavesPerBlock;
-  for (uint32_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
-    const uint64_t word = data[static_cast<uint64_t>(i) * kTile + lane_id()];
+  const uint64_t n_tiles = static_cast<uint64_t>(n_a) + n_bt;
+  for (uint64_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
+    const uint64_t word = i < n_a ? data_a[i * kTile + lane_id()] : data_bt[(i - n_a) * kTile + lane_id()];
     const uint64_t total = wave_sum(__popcll(word));
     WidePayload pl = {0, static_cast<uint64_t>(kWideDense) << 60};
     const uint64_t cols = wave_or(word);
@@ -283,6 +286,15 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
     }
     if (lane_id() == 0) payload[i] = pl;
   }
+}
+
+// Payloads of A's tiles at pl[0, n_a) and of Bt's at pl[n_a, n_a + n_bt), one launch.
+void launch_payloads(hipStream_t st, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, WidePayload* pl) {
+  const uint64_t n = static_cast<uint64_t>(a->n_nz_tile) + bt->n_nz_tile;
+  if (!n) return;
+  payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile, bt->data.get(),
+                                                                        bt->n_nz_tile, pl);
+  MG_LAUNCH_CHECK();
 }
 
 // fill_tiles_kernel for operands whose tiles are mostly sparse, from their wide payloads
@@ -595,9 +607,19 @@ __global__ void add_counts_kernel(uint32_t* __restrict__ dst, const uint32_t* __
 
 // --- symbolic pass ---------------------------------------------------------
 
-// Row form of B from Bt: histogram of Bt's column index.
-__global__ void col_hist_kernel(const uint32_t* __restrict__ col, uint32_t n, uint32_t* __restrict__ hist) {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) atomicAdd(hist + col[i], 1u);
+// Row form of B from Bt: histogram of Bt's column index (entries [0, n_bt)); in the same
+// launch, the rows of B that A references (its column index, entries [n_bt, n_bt + n_a)).
+__global__ void col_hist_mark_kernel(const uint32_t* __restrict__ col_bt, uint32_t n_bt,
+                                     const uint32_t* __restrict__ col_a, uint32_t n_a, uint32_t* __restrict__ hist,
+                                     uint32_t* __restrict__ ref) {
+  const uint64_t n = static_cast<uint64_t>(n_bt) + n_a;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (i < n_bt)
+      atomicAdd(hist + col_bt[i], 1u);
+    else
+      ref[col_a[i - n_bt]] = 1u;
+  }
 }
 __global__ void col_scatter_kernel(const uint32_t* __restrict__ row, const uint32_t* __restrict__ col, uint32_t n,
                                    const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
@@ -606,18 +628,6 @@ __global__ void col_scatter_kernel(const uint32_t* __restrict__ row, const uint3
     const uint32_t t = col[i];
     out[start[t] + atomicAdd(cursor + t, 1u)] = row[i];
   }
-}
-__global__ void mark_referenced_kernel(const uint32_t* __restrict__ col, uint32_t n, uint8_t* __restrict__ ref) {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * blockDim.x) ref[col[i]] = 1;
-}
-__global__ void __launch_bounds__(kBlock) used_right_kernel(const uint8_t* __restrict__ ref,
-                                                            const uint32_t* __restrict__ off_b, uint32_t n,
-                                                            unsigned long long* __restrict__ out) {
-  uint64_t local = 0;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    if (ref[t]) local += off_b[t + 1] - off_b[t];
-  local = wave_sum(local);
-  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
 }
 
 // Block-wide exclusive scan of one u32 per thread (256 threads).
@@ -644,20 +654,24 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
 }
 
 // Per output row x: union over t in A.row(x) of B.row(t) in an LDS bitmap.
-// EMIT = false: count[x] = |union|, tp[x] = sum |B.row(t)|.
+// EMIT = false: count[x] = |union|, tp[x] = sum |B.row(t)|; with `sums`, the block's totals are
+// added to sums[0] (tile products), sums[1] (tiles of the B rows that A references, ref[x]:
+// row x of B, as x runs over the strips) and sums[2] (C tiles).
 // EMIT = true : write the union as ascending col_idx at off_c[x], row_idx = x.
 template <bool EMIT>
 __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
     uint32_t n_strips, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ off_b, const uint32_t* __restrict__ col_b, uint32_t* __restrict__ count,
     unsigned long long* __restrict__ tp, const uint32_t* __restrict__ off_c, uint32_t* __restrict__ row_c,
-    uint32_t* __restrict__ col_c) {
+    uint32_t* __restrict__ col_c, const uint32_t* __restrict__ ref = nullptr,
+    unsigned long long* __restrict__ sums = nullptr) {
   extern __shared__ uint64_t s_bm[];
   __shared__ uint32_t s_tmp[kWavesPerBlock];
   __shared__ unsigned long long s_tp;
   const uint32_t n_words = (n_strips + 63) / 64;
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
+  uint64_t blk_tp = 0, blk_used = 0, blk_c = 0;  // thread 0's totals over the block's rows
   for (uint32_t x = blockIdx.x; x < n_strips; x += gridDim.x) {
     for (uint32_t w = threadIdx.x; w < n_words; w += kBlock) s_bm[w] = 0;
     if (threadIdx.x == 0) s_tp = 0;
@@ -687,6 +701,9 @@ __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
       if (threadIdx.x == 0) {
         count[x] = total;
         tp[x] = s_tp;
+        blk_tp += s_tp;
+        blk_c += total;
+        if (ref && ref[x]) blk_used += off_b[x + 1] - off_b[x];
       }
     } else {
       uint32_t pos = off_c[x] + before;
@@ -703,22 +720,11 @@ __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
     }
     __syncthreads();
   }
-}
-
-__global__ void __launch_bounds__(kBlock) sum_u32_to_u64_kernel(const uint32_t* __restrict__ v, uint32_t n,
-                                                                unsigned long long* __restrict__ out) {
-  uint64_t local = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) local += v[i];
-  local = wave_sum(local);
-  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
-}
-
-__global__ void __launch_bounds__(kBlock) sum_u64_kernel(const unsigned long long* __restrict__ v, uint32_t n,
-                                                         unsigned long long* __restrict__ out) {
-  uint64_t local = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) local += v[i];
-  local = wave_sum(local);
-  if (lane_id() == 0 && local) atomicAdd(out, static_cast<unsigned long long>(local));
+  if (!EMIT && sums && threadIdx.x == 0) {
+    if (blk_tp) atomicAdd(sums, static_cast<unsigned long long>(blk_tp));
+    if (blk_used) atomicAdd(sums + 1, static_cast<unsigned long long>(blk_used));
+    if (blk_c) atomicAdd(sums + 2, static_cast<unsigned long long>(blk_c));
+  }
 }
 
 constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
@@ -748,16 +754,7 @@ void launch_payload_product(mg_context* ctx, const mg_tiled_matrix* a, const mg_
   hipStream_t st = ctx->stream;
   WidePayload* pl_a = pl;
   WidePayload* pl_bt = pl + a->n_nz_tile;
-  if (a->n_nz_tile) {
-    payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile,
-                                                                                     pl_a);
-    MG_LAUNCH_CHECK();
-  }
-  if (bt->n_nz_tile) {
-    payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(), bt->n_nz_tile,
-                                                                                      pl_bt);
-    MG_LAUNCH_CHECK();
-  }
+  launch_payloads(st, a, bt, pl);
   const uint32_t grid = grid_for(c->n_nz_tile, kWavesPerBlock, 65536);
   if (rec.pairs)
     fill_tiles_payload_kernel<true><<<grid, kBlock, 0, st>>>(
@@ -789,50 +786,43 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
   if (n_words > kMaxLdsBitmapWords) fail(MG_ERR_UNSUPPORTED, std::string(who) + ": n_strips > 524288 not supported");
   hipStream_t s = ctx->stream;
   S.n = n;
-  StreamBuf<uint32_t> hist(n, s), cursor(n, s);
+  // One zeroed block: the totals (tile products, right tiles used, C tiles; u64), then the
+  // histogram, the scatter cursors and the referenced-row flags of the strips (u32 each).
+  StreamBuf<unsigned long long> zero(3 + (3ull * n + 1) / 2, s);
+  unsigned long long* const sums = zero.get();
+  uint32_t* const hist = reinterpret_cast<uint32_t*>(sums + 3);
+  uint32_t* const cursor = hist + n;
+  uint32_t* const ref = cursor + n;
+  MG_HIP(hipMemsetAsync(zero.get(), 0, zero.bytes(), s));
   S.off_b.alloc(n + 1ull, s);
   S.col_b.alloc(bt->n_nz_tile, s);
-  MG_HIP(hipMemsetAsync(hist.get(), 0, sizeof(uint32_t) * n, s));
-  MG_HIP(hipMemsetAsync(cursor.get(), 0, sizeof(uint32_t) * n, s));
-  if (bt->n_nz_tile) {
-    col_hist_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile, hist.get());
+  if (bt->n_nz_tile || a->n_nz_tile) {
+    col_hist_mark_kernel<<<grid_for(static_cast<uint64_t>(bt->n_nz_tile) + a->n_nz_tile, 256), 256, 0, s>>>(
+        bt->col_idx.get(), bt->n_nz_tile, a->col_idx.get(), a->n_nz_tile, hist, ref);
     MG_LAUNCH_CHECK();
   }
-  exclusive_scan_u32(ctx, hist.get(), S.off_b.get(), n);
+  exclusive_scan_u32(ctx, hist, S.off_b.get(), n);
   if (bt->n_nz_tile) {
     col_scatter_kernel<<<grid_for(bt->n_nz_tile, 256), 256, 0, s>>>(bt->row_idx.get(), bt->col_idx.get(),
-                                                                    bt->n_nz_tile, S.off_b.get(), cursor.get(),
+                                                                    bt->n_nz_tile, S.off_b.get(), cursor,
                                                                     S.col_b.get());
     MG_LAUNCH_CHECK();
   }
   S.cnt.alloc(n, s);
   S.off_c.alloc(n + 1ull, s);
   S.tp.alloc(n, s);
-  StreamBuf<unsigned long long> sums(3, s);  // tile products, right tiles used, C tiles
-  MG_HIP(hipMemsetAsync(sums.get(), 0, sizeof(unsigned long long) * 3, s));
   const size_t lds = sizeof(uint64_t) * n_words;
   const unsigned grid = grid_for(n, 1, 16384);
+  // The row kernel also adds up the totals. C's tile index is 32-bit (the reference's
+  // n_nz_tile is u32): the 64-bit C tile total is read with the other sums and checked before
+  // anything uses the 32-bit scan.
   symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
                                                        S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
-                                                       nullptr);
-  MG_LAUNCH_CHECK();
-  // C's tile index is 32-bit (the reference's n_nz_tile is u32): the 64-bit total is read with
-  // the other sums and checked before anything uses the 32-bit scan.
-  sum_u32_to_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(S.cnt.get(), n, sums.get() + 2);
+                                                       nullptr, ref, sums);
   MG_LAUNCH_CHECK();
   exclusive_scan_u32(ctx, S.cnt.get(), S.off_c.get(), n);
-  sum_u64_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(S.tp.get(), n, sums.get());
-  MG_LAUNCH_CHECK();
-  StreamBuf<uint8_t> ref(n, s);
-  MG_HIP(hipMemsetAsync(ref.get(), 0, n, s));
-  if (a->n_nz_tile) {
-    mark_referenced_kernel<<<grid_for(a->n_nz_tile, 256), 256, 0, s>>>(a->col_idx.get(), a->n_nz_tile, ref.get());
-    MG_LAUNCH_CHECK();
-  }
-  used_right_kernel<<<grid_for(n, 256, 1024), 256, 0, s>>>(ref.get(), S.off_b.get(), n, sums.get() + 1);
-  MG_LAUNCH_CHECK();
   unsigned long long h_sums[3] = {0, 0, 0};
-  MG_HIP(hipMemcpyAsync(h_sums, sums.get(), sizeof(h_sums), hipMemcpyDeviceToHost, s));
+  MG_HIP(hipMemcpyAsync(h_sums, sums, sizeof(h_sums), hipMemcpyDeviceToHost, s));
   MG_HIP(hipStreamSynchronize(s));  // the one synchronisation of the pass: the totals
   const uint64_t n_c64 = h_sums[2];
   MG_REQUIRE(n_c64 <= 0xFFFFFFFFull, std::string(who) + ": " + std::to_string(n_c64) +
@@ -1115,16 +1105,7 @@ void pruned_prepare(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
   P.col_s.alloc(n_s, s);
   structural_emit(ctx, a, P.S, P.row_s.get(), P.col_s.get());
   P.pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
-  if (a->n_nz_tile) {
-    payload_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(), a->n_nz_tile,
-                                                                                   P.pl);
-    MG_LAUNCH_CHECK();
-  }
-  if (bt->n_nz_tile) {
-    payload_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(bt->data.get(), bt->n_nz_tile,
-                                                                                     P.pl + a->n_nz_tile);
-    MG_LAUNCH_CHECK();
-  }
+  launch_payloads(s, a, bt, P.pl);
 }
 
 // Features of every C row into feat (device, n_strips x kRowFeatures).
