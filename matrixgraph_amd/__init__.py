@@ -288,8 +288,8 @@ def multiply_numeric(ctx: Context, a: TiledMatrix, bt: TiledMatrix, c: TiledMatr
 
 
 # Tile pairs whose two tiles both hold at least this many set bits are faster on the matrix
-# cores than on the vector ALUs (C = A*A of planted communities: 0.92x at ~41 bits per tile,
-# 1.6x at ~123, 1.86x at ~410).
+# cores than on the vector ALUs (C = A*A of planted communities: 0.48x at ~41 bits per tile,
+# 1.32x at ~123, 1.76x at ~410).
 DENSE_MIN_BITS = 128
 
 
