@@ -373,10 +373,13 @@ __global__ void grid_block_scatter_kernel(const uint64_t* __restrict__ keys, uin
 
 // Tile -> edge-list expansion in (source, destination) order without a sort. The tiles of
 // each strip x are cut into pieces of kPieceTiles consecutive tiles (ascending column y); one
-// wave per piece, lane = row r. Counts are laid out in the output order (strip x, row r, piece)
-// so that their exclusive scan gives every (row, piece) its first output slot; the write pass
-// then walks the piece's tiles in order, so each lane emits its row's edges sorted by
-// destination. drop_diagonal leaves out the bits with x*64 + r == y*64 + c.
+// wave per piece. Counts (lane = row r, over the piece's tiles) are laid out in the output
+// order (strip x, row r, piece) so that their exclusive scan gives every (row, piece) its
+// first output slot. The write pass takes lane = tile: each lane reads its tile 16 rows (one
+// 128-byte line) at a time, and for each row a prefix sum of the lanes' bit counts places each
+// lane's edges, so consecutive lanes write the row's run of the piece front to back in (tile,
+// column) order, which is the destination order. drop_diagonal leaves out the bits with
+// x*64 + r == y*64 + c.
 constexpr uint32_t kPieceTiles = 64;
 
 struct PieceRef {
@@ -396,13 +399,10 @@ __global__ void __launch_bounds__(256) expand_pieces_kernel(const uint32_t* __re
   }
 }
 
-template <bool kWrite>
-__global__ void __launch_bounds__(256) expand_rows_kernel(const PieceRef* __restrict__ pieces, uint32_t n_pieces,
-                                                          const uint32_t* __restrict__ col_idx,
-                                                          const uint64_t* __restrict__ data, int drop_diagonal,
-                                                          uint32_t* __restrict__ count,
-                                                          const uint32_t* __restrict__ start,
-                                                          uint64_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) count_rows_kernel(const PieceRef* __restrict__ pieces, uint32_t n_pieces,
+                                                         const uint32_t* __restrict__ col_idx,
+                                                         const uint64_t* __restrict__ data, int drop_diagonal,
+                                                         uint32_t* __restrict__ count) {
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
   for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_pieces;
@@ -410,33 +410,63 @@ __global__ void __launch_bounds__(256) expand_rows_kernel(const PieceRef* __rest
     const PieceRef pc = pieces[i];
     const uint64_t slot = static_cast<uint64_t>(pc.count_base) + static_cast<uint64_t>(lane) * pc.n_pieces_in_strip +
                           pc.piece;
-    uint64_t pos = kWrite ? start[slot] : 0;
     uint32_t c_total = 0;
     for (uint32_t k = 0; k < pc.n_tiles; ++k) {
       const uint32_t t = pc.first_tile + k;
-      const uint32_t y = col_idx[t];
       uint64_t w = data[static_cast<uint64_t>(t) * 64 + lane];
-      if (drop_diagonal && y == pc.strip) w &= ~(1ull << lane);
-      if (kWrite) {
-        // Row by row, the whole wave writes the row's pairs as one run (lane = column), so
-        // every store covers consecutive addresses.
-        for (uint64_t rows = ballot(w != 0); rows; rows &= rows - 1) {
-          const int r = __ffsll(static_cast<unsigned long long>(rows)) - 1;
-          const uint64_t m = read_lane(w, r);
-          const uint64_t p = read_lane(pos, r);
-          if ((m >> lane) & 1) {
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-            out[p + below] = ((static_cast<uint64_t>(pc.strip) * 64 + static_cast<uint64_t>(r)) << 32) |
-                             (static_cast<uint64_t>(y) * 64 + lane);
-          }
-          if (lane == static_cast<uint32_t>(r)) pos += __popcll(m);
+      if (drop_diagonal && col_idx[t] == pc.strip) w &= ~(1ull << lane);
+      c_total += __popcll(w);
+    }
+    count[slot] = c_total;
+  }
+}
+
+__global__ void __launch_bounds__(256) write_rows_kernel(const PieceRef* __restrict__ pieces, uint32_t n_pieces,
+                                                         const uint32_t* __restrict__ col_idx,
+                                                         const uint64_t* __restrict__ data, int drop_diagonal,
+                                                         const uint32_t* __restrict__ start,
+                                                         uint64_t* __restrict__ out) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  const uint32_t lane = lane_id();
+  const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
+  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_pieces;
+       i += n_waves) {
+    const PieceRef pc = pieces[i];
+    const bool has = lane < pc.n_tiles;  // lane = tile k of the piece
+    const uint32_t t = pc.first_tile + (has ? lane : 0u);
+    const uint64_t y = has ? col_idx[t] : 0u;
+    const bool diag = drop_diagonal && has && y == pc.strip;
+    const u64x2* const tile = reinterpret_cast<const u64x2*>(data + static_cast<uint64_t>(t) * 64);
+    for (uint32_t r0 = 0; r0 < 64; r0 += 16) {
+      // Rows r0 .. r0 + 15 of this lane's tile: one 128-byte line per lane.
+      u64x2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = has ? tile[r0 / 2 + q] : u64x2{0ull, 0ull};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const uint32_t r = r0 + q;
+        uint64_t w = (q & 1) ? v[q / 2].y : v[q / 2].x;
+        if (diag) w &= ~(1ull << r);
+        if (!ballot(w != 0)) continue;
+        // Exclusive prefix of the lanes' bit counts: this lane's first position in the run.
+        const uint32_t n = static_cast<uint32_t>(__popcll(w));
+        uint32_t incl = n;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d, kWave));
+          if (lane >= static_cast<uint32_t>(d)) incl += o;
         }
-      } else {
-        c_total += __popcll(w);
+        const uint64_t slot = static_cast<uint64_t>(pc.count_base) +
+                              static_cast<uint64_t>(r) * pc.n_pieces_in_strip + pc.piece;
+        uint64_t pos = static_cast<uint64_t>(start[slot]) + (incl - n);
+        const uint64_t src = (static_cast<uint64_t>(pc.strip) * 64 + r) << 32;
+        while (w) {
+          const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(w)) - 1);
+          w &= w - 1;
+          out[pos++] = src | (y * 64 + c);
+        }
       }
     }
-    if (!kWrite) count[slot] = c_total;
   }
 }
 
@@ -547,8 +577,8 @@ std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_mat
   const uint32_t n_slots = n_pieces * 64;
   DevBuf<uint32_t> count(n_slots), start(n_slots + 1ull);
   const unsigned grid = grid_for(n_pieces, 4, 65536);
-  expand_rows_kernel<false><<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
-                                                  drop_diagonal ? 1 : 0, count.get(), nullptr, nullptr);
+  count_rows_kernel<<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
+                                          drop_diagonal ? 1 : 0, count.get());
   MG_LAUNCH_CHECK();
   // The scan below is 32-bit: bound the 64-bit total first (the other edge-list paths stop at
   // 2^31 edges too).
@@ -562,8 +592,8 @@ std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_mat
   el->n_edges = total;
   el->keys.alloc(total ? total : 1);
   if (total) {
-    expand_rows_kernel<true><<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
-                                                   drop_diagonal ? 1 : 0, nullptr, start.get(), el->keys.get());
+    write_rows_kernel<<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
+                                            drop_diagonal ? 1 : 0, start.get(), el->keys.get());
     MG_LAUNCH_CHECK();
   }
   MG_HIP(hipStreamSynchronize(st));
