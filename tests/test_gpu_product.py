@@ -81,8 +81,10 @@ def test_dense_and_closed_form_tiles(ctx):
 @pytest.mark.parametrize("seed", [7, 8])
 def test_tiles_around_compact_limit(ctx, seed):
     """Tiles of 1..12 bits (the numeric pass multiplies tiles of up to 10 bits from their
-    compact entry lists, one lane per pair, and denser ones as whole tiles), 40-bit and full
-    tiles, with empty tiles between them: C = A * A equals a brute-force Boolean product."""
+    compact entry lists, one lane per pair, and denser ones as whole tiles through the inner
+    indices both tiles use, so partial and empty overlaps of their column sets occur), 40-bit
+    and full tiles, with empty tiles between them: C = A * A equals a brute-force Boolean
+    product."""
     rng = np.random.default_rng(seed)
     ns = 8
     n = ns * 64
