@@ -630,7 +630,8 @@ __global__ void col_scatter_kernel(const uint32_t* __restrict__ row, const uint3
   }
 }
 
-// Block-wide exclusive scan of one u32 per thread (256 threads).
+// Block-wide exclusive scan of one u32 per thread (kWaves waves).
+template <int kWaves>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
@@ -643,7 +644,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
   if (lane == kWave - 1) s_tmp[wave] = incl;
   __syncthreads();
   uint32_t base = 0, tot = 0;
-  for (int w = 0; w < kWavesPerBlock; ++w) {
+  for (int w = 0; w < kWaves; ++w) {
     const uint32_t t = s_tmp[w];
     if (w < static_cast<int>(wave)) base += t;
     tot += t;
@@ -658,27 +659,29 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
 // added to sums[0] (tile products), sums[1] (tiles of the B rows that A references, ref[x]:
 // row x of B, as x runs over the strips) and sums[2] (C tiles).
 // EMIT = true : write the union as ascending col_idx at off_c[x], row_idx = x.
-template <bool EMIT>
-__global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
+// BLOCK threads per row: 1024 when there are too few rows to fill the chip with 256.
+template <bool EMIT, int BLOCK = kBlock>
+__global__ void __launch_bounds__(BLOCK) symbolic_row_kernel(
     uint32_t n_strips, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ off_b, const uint32_t* __restrict__ col_b, uint32_t* __restrict__ count,
     unsigned long long* __restrict__ tp, const uint32_t* __restrict__ off_c, uint32_t* __restrict__ row_c,
     uint32_t* __restrict__ col_c, const uint32_t* __restrict__ ref = nullptr,
     unsigned long long* __restrict__ sums = nullptr) {
+  constexpr int kWaves = BLOCK / kWave;
   extern __shared__ uint64_t s_bm[];
-  __shared__ uint32_t s_tmp[kWavesPerBlock];
+  __shared__ uint32_t s_tmp[kWaves];
   __shared__ unsigned long long s_tp;
   const uint32_t n_words = (n_strips + 63) / 64;
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / kWave;
   uint64_t blk_tp = 0, blk_used = 0, blk_c = 0;  // thread 0's totals over the block's rows
   for (uint32_t x = blockIdx.x; x < n_strips; x += gridDim.x) {
-    for (uint32_t w = threadIdx.x; w < n_words; w += kBlock) s_bm[w] = 0;
+    for (uint32_t w = threadIdx.x; w < n_words; w += BLOCK) s_bm[w] = 0;
     if (threadIdx.x == 0) s_tp = 0;
     __syncthreads();
     const uint32_t a0 = off_a[x], a1 = off_a[x + 1];
     uint64_t my_tp = 0;
-    for (uint32_t k = a0 + wave; k < a1; k += kWavesPerBlock) {
+    for (uint32_t k = a0 + wave; k < a1; k += kWaves) {
       const uint32_t t = col_a[k];
       const uint32_t r0 = off_b[t], r1 = off_b[t + 1];
       my_tp += r1 - r0;
@@ -690,13 +693,13 @@ __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
     if (!EMIT && lane == 0 && my_tp) atomicAdd(&s_tp, static_cast<unsigned long long>(my_tp));
     __syncthreads();
     // Contiguous word range per thread so emitted columns come out ascending.
-    const uint32_t per = (n_words + kBlock - 1) / kBlock;
+    const uint32_t per = (n_words + BLOCK - 1) / BLOCK;
     const uint32_t w0 = threadIdx.x * per;
     const uint32_t w1 = (w0 + per < n_words) ? w0 + per : n_words;
     uint32_t c = 0;
     for (uint32_t w = w0; w < w1; ++w) c += __popcll(s_bm[w]);
     uint32_t total = 0;
-    const uint32_t before = block_exclusive_scan(c, s_tmp, &total);
+    const uint32_t before = block_exclusive_scan<kWaves>(c, s_tmp, &total);
     if (!EMIT) {
       if (threadIdx.x == 0) {
         count[x] = total;
@@ -728,6 +731,9 @@ __global__ void __launch_bounds__(kBlock) symbolic_row_kernel(
 }
 
 constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
+// Below this many rows (one workgroup per row), the symbolic row kernel runs 1024 threads per
+// row instead of 256, so that the few workgroups still fill the CUs.
+constexpr uint32_t kWideRowsBelow = 2048;
 
 void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const uint32_t* off_bt,
                        const uint32_t* row_c, const uint32_t* col_c, const uint32_t* col_a,
@@ -816,9 +822,14 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
   // The row kernel also adds up the totals. C's tile index is 32-bit (the reference's
   // n_nz_tile is u32): the 64-bit C tile total is read with the other sums and checked before
   // anything uses the 32-bit scan.
-  symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
-                                                       S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
-                                                       nullptr, ref, sums);
+  if (n < kWideRowsBelow)
+    symbolic_row_kernel<false, 1024><<<grid, 1024, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(),
+                                                             S.off_b.get(), S.col_b.get(), S.cnt.get(), S.tp.get(),
+                                                             nullptr, nullptr, nullptr, ref, sums);
+  else
+    symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
+                                                         S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
+                                                         nullptr, ref, sums);
   MG_LAUNCH_CHECK();
   exclusive_scan_u32(ctx, S.cnt.get(), S.off_c.get(), n);
   unsigned long long h_sums[3] = {0, 0, 0};
@@ -837,9 +848,14 @@ void structural_emit(mg_context* ctx, const mg_tiled_matrix* a, const Structural
                      uint32_t* col_c) {
   if (!S.n_c) return;
   const size_t lds = sizeof(uint64_t) * ((S.n + 63) / 64);
-  symbolic_row_kernel<true><<<grid_for(S.n, 1, 16384), kBlock, lds, ctx->stream>>>(
-      S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
-      row_c, col_c);
+  if (S.n < kWideRowsBelow)
+    symbolic_row_kernel<true, 1024><<<grid_for(S.n, 1, 16384), 1024, lds, ctx->stream>>>(
+        S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
+        row_c, col_c);
+  else
+    symbolic_row_kernel<true><<<grid_for(S.n, 1, 16384), kBlock, lds, ctx->stream>>>(
+        S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
+        row_c, col_c);
   MG_LAUNCH_CHECK();
 }
 
