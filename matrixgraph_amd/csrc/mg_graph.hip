@@ -375,11 +375,12 @@ __global__ void grid_block_scatter_kernel(const uint64_t* __restrict__ keys, uin
 // each strip x are cut into pieces of kPieceTiles consecutive tiles (ascending column y); one
 // wave per piece. Counts (lane = row r, over the piece's tiles) are laid out in the output
 // order (strip x, row r, piece) so that their exclusive scan gives every (row, piece) its
-// first output slot. The write pass takes lane = tile: each lane reads its tile 16 rows (one
-// 128-byte line) at a time, and for each row a prefix sum of the lanes' bit counts places each
-// lane's edges, so consecutive lanes write the row's run of the piece front to back in (tile,
-// column) order, which is the destination order. drop_diagonal leaves out the bits with
-// x*64 + r == y*64 + c.
+// first output slot. The write pass takes one wave per (piece, quarter of 16 rows) and lane =
+// tile: each lane reads its tile's 16 rows (one 128-byte line), and for each row a prefix sum of
+// the lanes' bit counts places each lane's edges, so the row's run of the piece is written in
+// (tile, column) order, which is the destination order: by each lane for its own few bits, or,
+// when the row holds more than four bits per tile that has any, tile by tile by the whole wave
+// (lane = column). drop_diagonal leaves out the bits with x*64 + r == y*64 + c.
 constexpr uint32_t kPieceTiles = 64;
 
 struct PieceRef {
@@ -429,15 +430,21 @@ __global__ void __launch_bounds__(256) write_rows_kernel(const PieceRef* __restr
   typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   const uint32_t lane = lane_id();
   const uint32_t n_waves = gridDim.x * (blockDim.x / kWave);
-  for (uint32_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_pieces;
+  const uint64_t n_items = static_cast<uint64_t>(n_pieces) * 4;  // (piece, 16-row quarter)
+  for (uint64_t i = read_first_lane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); i < n_items;
        i += n_waves) {
-    const PieceRef pc = pieces[i];
+    const PieceRef pc = pieces[i / 4];
+    const uint32_t r0 = 16 * static_cast<uint32_t>(i % 4);
     const bool has = lane < pc.n_tiles;  // lane = tile k of the piece
     const uint32_t t = pc.first_tile + (has ? lane : 0u);
     const uint64_t y = has ? col_idx[t] : 0u;
     const bool diag = drop_diagonal && has && y == pc.strip;
     const u64x2* const tile = reinterpret_cast<const u64x2*>(data + static_cast<uint64_t>(t) * 64);
-    for (uint32_t r0 = 0; r0 < 64; r0 += 16) {
+    // First output position of the quarter's rows' runs (lane q: row r0 + q), read out per row.
+    const uint32_t row_start = lane < 16 ? start[static_cast<uint64_t>(pc.count_base) +
+                                                 static_cast<uint64_t>(r0 + lane) * pc.n_pieces_in_strip + pc.piece]
+                                         : 0u;
+    {
       // Rows r0 .. r0 + 15 of this lane's tile: one 128-byte line per lane.
       u64x2 v[8];
 #pragma unroll
@@ -447,7 +454,8 @@ __global__ void __launch_bounds__(256) write_rows_kernel(const PieceRef* __restr
         const uint32_t r = r0 + q;
         uint64_t w = (q & 1) ? v[q / 2].y : v[q / 2].x;
         if (diag) w &= ~(1ull << r);
-        if (!ballot(w != 0)) continue;
+        const uint64_t act = ballot(w != 0);
+        if (!act) continue;
         // Exclusive prefix of the lanes' bit counts: this lane's first position in the run.
         const uint32_t n = static_cast<uint32_t>(__popcll(w));
         uint32_t incl = n;
@@ -456,14 +464,31 @@ __global__ void __launch_bounds__(256) write_rows_kernel(const PieceRef* __restr
           const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d, kWave));
           if (lane >= static_cast<uint32_t>(d)) incl += o;
         }
-        const uint64_t slot = static_cast<uint64_t>(pc.count_base) +
-                              static_cast<uint64_t>(r) * pc.n_pieces_in_strip + pc.piece;
-        uint64_t pos = static_cast<uint64_t>(start[slot]) + (incl - n);
+        uint64_t pos = static_cast<uint64_t>(static_cast<uint32_t>(
+                           __builtin_amdgcn_readlane(static_cast<int>(row_start), q))) +
+                       (incl - n);
         const uint64_t src = (static_cast<uint64_t>(pc.strip) * 64 + r) << 32;
-        while (w) {
-          const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(w)) - 1);
-          w &= w - 1;
-          out[pos++] = src | (y * 64 + c);
+        const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
+        if (total > 4u * static_cast<uint32_t>(__popcll(act))) {
+          // Dense row: tile by tile, the whole wave writes the tile's edges (lane = column).
+          for (uint64_t a = act; a; a &= a - 1) {
+            const int k = __ffsll(static_cast<unsigned long long>(a)) - 1;
+            const uint64_t m = read_lane(w, k);
+            const uint64_t p = read_lane(pos, k);
+            const uint64_t yk = read_lane(y, k);
+            if ((m >> lane) & 1) {
+              const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+              out[p + below] = src | (yk * 64 + lane);
+            }
+          }
+        } else {
+          // Sparse row: each lane writes its few edges itself.
+          while (w) {
+            const uint64_t c = static_cast<uint64_t>(__ffsll(static_cast<unsigned long long>(w)) - 1);
+            w &= w - 1;
+            out[pos++] = src | (y * 64 + c);
+          }
         }
       }
     }
@@ -592,8 +617,8 @@ std::unique_ptr<mg_edge_list> tiles_to_edges(mg_context* ctx, const mg_tiled_mat
   el->n_edges = total;
   el->keys.alloc(total ? total : 1);
   if (total) {
-    write_rows_kernel<<<grid, 256, 0, st>>>(pieces.get(), n_pieces, m->col_idx.get(), m->data.get(),
-                                            drop_diagonal ? 1 : 0, start.get(), el->keys.get());
+    write_rows_kernel<<<grid_for(static_cast<uint64_t>(n_pieces) * 4, 4, 65536), 256, 0, st>>>(
+        pieces.get(), n_pieces, m->col_idx.get(), m->data.get(), drop_diagonal ? 1 : 0, start.get(), el->keys.get());
     MG_LAUNCH_CHECK();
   }
   MG_HIP(hipStreamSynchronize(st));
