@@ -659,8 +659,12 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
 // added to sums[0] (tile products), sums[1] (tiles of the B rows that A references, ref[x]:
 // row x of B, as x runs over the strips) and sums[2] (C tiles).
 // EMIT = true : write the union as ascending col_idx at off_c[x], row_idx = x.
-// BLOCK threads per row: 1024 when there are too few rows to fill the chip with 256.
-template <bool EMIT, int BLOCK = kBlock>
+// BLOCK threads per row (kRowBlock = 1024: a heavy row, such as a dense frontier's tile row
+// against thousands of B rows, is the whole pass when few rows carry the work; 1024 threads
+// took the 3-hop density-mixed product walk's symbolic passes from 39 to 16 ms and C = A*A of
+// R-MAT 14 from 0.64 to 0.62 ms, with the two-hop of R-MAT 18 unchanged).
+constexpr int kRowBlock = 1024;
+template <bool EMIT, int BLOCK = kRowBlock>
 __global__ void __launch_bounds__(BLOCK) symbolic_row_kernel(
     uint32_t n_strips, const uint32_t* __restrict__ off_a, const uint32_t* __restrict__ col_a,
     const uint32_t* __restrict__ off_b, const uint32_t* __restrict__ col_b, uint32_t* __restrict__ count,
@@ -731,9 +735,6 @@ __global__ void __launch_bounds__(BLOCK) symbolic_row_kernel(
 }
 
 constexpr uint32_t kMaxLdsBitmapWords = 8192;  // 64 KiB: n_strips <= 524288
-// Below this many rows (one workgroup per row), the symbolic row kernel runs 1024 threads per
-// row instead of 256, so that the few workgroups still fill the CUs.
-constexpr uint32_t kWideRowsBelow = 2048;
 
 void launch_fill_tiles(hipStream_t s, uint32_t n_c, const uint32_t* off_a, const uint32_t* off_bt,
                        const uint32_t* row_c, const uint32_t* col_c, const uint32_t* col_a,
@@ -822,14 +823,9 @@ void structural_pass(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_m
   // The row kernel also adds up the totals. C's tile index is 32-bit (the reference's
   // n_nz_tile is u32): the 64-bit C tile total is read with the other sums and checked before
   // anything uses the 32-bit scan.
-  if (n < kWideRowsBelow)
-    symbolic_row_kernel<false, 1024><<<grid, 1024, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(),
-                                                             S.off_b.get(), S.col_b.get(), S.cnt.get(), S.tp.get(),
-                                                             nullptr, nullptr, nullptr, ref, sums);
-  else
-    symbolic_row_kernel<false><<<grid, kBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(),
-                                                         S.col_b.get(), S.cnt.get(), S.tp.get(), nullptr, nullptr,
-                                                         nullptr, ref, sums);
+  symbolic_row_kernel<false><<<grid, kRowBlock, lds, s>>>(n, a->offset_row.get(), a->col_idx.get(),
+                                                          S.off_b.get(), S.col_b.get(), S.cnt.get(), S.tp.get(),
+                                                          nullptr, nullptr, nullptr, ref, sums);
   MG_LAUNCH_CHECK();
   exclusive_scan_u32(ctx, S.cnt.get(), S.off_c.get(), n);
   unsigned long long h_sums[3] = {0, 0, 0};
@@ -848,14 +844,9 @@ void structural_emit(mg_context* ctx, const mg_tiled_matrix* a, const Structural
                      uint32_t* col_c) {
   if (!S.n_c) return;
   const size_t lds = sizeof(uint64_t) * ((S.n + 63) / 64);
-  if (S.n < kWideRowsBelow)
-    symbolic_row_kernel<true, 1024><<<grid_for(S.n, 1, 16384), 1024, lds, ctx->stream>>>(
-        S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
-        row_c, col_c);
-  else
-    symbolic_row_kernel<true><<<grid_for(S.n, 1, 16384), kBlock, lds, ctx->stream>>>(
-        S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
-        row_c, col_c);
+  symbolic_row_kernel<true><<<grid_for(S.n, 1, 16384), kRowBlock, lds, ctx->stream>>>(
+      S.n, a->offset_row.get(), a->col_idx.get(), S.off_b.get(), S.col_b.get(), nullptr, nullptr, S.off_c.get(),
+      row_c, col_c);
   MG_LAUNCH_CHECK();
 }
 
