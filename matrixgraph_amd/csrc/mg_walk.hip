@@ -1183,8 +1183,9 @@ __device__ inline uint64_t lean_groups(uint32_t job, uint64_t ok, uint32_t lane,
 
 // Issues the gathers of the next kLeanUnroll groups of `starts` (wave-uniform: first lanes of
 // the groups still to gather): the row address (`row_addr`, one per lane of the batch, made by
-// vector instructions) is read out as a scalar, plus the lane's word.
-__device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint64_t row_addr, uint64_t rows, uint32_t lane) {
+// vector instructions) is read out as a scalar, and each lane loads word `word` of the row (its
+// own word, or a live one when its own is dead).
+__device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint64_t row_addr, uint64_t rows, uint32_t word) {
 #pragma unroll
   for (int k = 0; k < kLeanUnroll; ++k) {
     s.rows[k] = 0;
@@ -1193,7 +1194,7 @@ __device__ inline void lean_issue(LeanSet& s, uint64_t& starts, uint64_t row_add
       const int src = __ffsll(static_cast<unsigned long long>(starts)) - 1;
       starts &= starts - 1;
       const gptr<const uint64_t> row = reinterpret_cast<gptr<const uint64_t>>(read_lane(row_addr, src));
-      s.v[k] = row[lane];
+      s.v[k] = row[word];
       s.rows[k] = read_lane(rows, src);
     }
   }
@@ -1218,6 +1219,9 @@ __device__ inline void lean_commit(const LeanSet& s, char* __restrict__ acc_lane
   }
 }
 
+// kPartial: some panel of the level has dead words (its gathers then read only the cache lines of
+// live words); false keeps the plain lane-per-word loads and their registers.
+template <bool kPartial>
 __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu(kLeanWaves, kLeanWaves)))
     walk_lean_kernel(const Chunk* __restrict__ chunks, const PanelLevel* __restrict__ lv) {
   constexpr int kWords = 64 * 64;
@@ -1242,6 +1246,11 @@ __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu
   const uint64_t alive_in = L.alive_in;
   const uint64_t alive_next = L.alive_next;
   const uint64_t keep = ((alive_in >> lane) & 1) ? ~0ull : 0ull;
+  // Word this lane gathers: its own if live, else the first live word, so that a row of a
+  // panel with few live words touches only their cache lines (the value is masked by keep).
+  const uint32_t gather_word =
+      (!kPartial || keep) ? lane
+                          : (alive_in ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(alive_in)) - 1) : 0u);
   const bool lane_next = (alive_next >> lane) & 1;  // this lane's word propagates further
   // Per-strip values of the chunk, one strip per lane: row pointers and the job offsets of
   // the strip boundaries (clamped to the chunk's tile range).
@@ -1350,20 +1359,20 @@ __global__ void __launch_bounds__(kLeanBlock) __attribute__((amdgpu_waves_per_eu
       uint64_t m = lean_groups(job, okm, lane, rows);
       const uint64_t row_addr = reinterpret_cast<uint64_t>(x_in) + static_cast<uint64_t>(job >> 6) * 512u;
       LeanSet a, b;
-      lean_issue(a, m, row_addr, rows, lane);
+      lean_issue(a, m, row_addr, rows, gather_word);
       const uint64_t vw_n = (check_v && job_n != kNoJob) ? v_in[job_n >> 12] : 0ull;
       for (;;) {
         if (!m) {
           lean_commit(a, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(b, m, row_addr, rows, lane);
+        lean_issue(b, m, row_addr, rows, gather_word);
         lean_commit(a, acc_lane, keep, count_rows, edges);
         if (!m) {
           lean_commit(b, acc_lane, keep, count_rows, edges);
           break;
         }
-        lean_issue(a, m, row_addr, rows, lane);
+        lean_issue(a, m, row_addr, rows, gather_word);
         lean_commit(b, acc_lane, keep, count_rows, edges);
       }
       job = job_n;
@@ -2297,6 +2306,7 @@ struct Engine {
   bool logging = false;
   std::vector<DevBuf<PanelLevel>> lv;
   std::vector<std::vector<PanelLevel>> host_lv;  // host copies of lv
+  std::vector<uint8_t> partial;  // per level: some active panel has dead words (alive_in != all)
   std::vector<DevBuf<Chunk>> chunks;
   std::vector<uint32_t> n_chunks;
   std::vector<DevBuf<StripItem>> split;
@@ -2362,6 +2372,7 @@ void build_engine(Engine* e, hipStream_t st) {
   e->counters.alloc(static_cast<size_t>(P) * (e->max_hops + 1) * kCounterStride);
   e->lv.resize(e->max_hops);
   e->host_lv.clear();
+  e->partial.clear();
   e->chunks.resize(e->max_hops);
   e->host_chunks.assign(e->max_hops, {});
   e->n_chunks.assign(e->max_hops, 0);
@@ -2442,6 +2453,9 @@ void build_engine(Engine* e, hipStream_t st) {
     e->lv[k].alloc(P);
     MG_HIP(hipMemcpyAsync(e->lv[k].get(), lv.data(), sizeof(PanelLevel) * P, hipMemcpyHostToDevice, st));
     e->host_lv.push_back(lv);
+    bool partial = false;
+    for (const auto& L : lv) partial |= L.x_in && L.alive_in != ~0ull;
+    e->partial.push_back(partial ? 1 : 0);
     e->n_chunks[k] = static_cast<uint32_t>(ch.size());
     e->host_chunks[k] = ch;
     e->chunks[k].alloc(std::max<size_t>(1, ch.size()));
@@ -2550,7 +2564,12 @@ void launch_level_w(hipStream_t st, Engine* e, uint32_t k, hipEvent_t ev[4]) {
       const uint32_t nb = e->n_chunks[k];
       switch (G) {
         case 0: walk_numeric64_kernel<<<nb, kNumBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get(), log); break;
-        case 1: walk_lean_kernel<<<nb, kLeanBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
+        case 1:
+          if (e->partial[k])
+            walk_lean_kernel<true><<<nb, kLeanBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+          else
+            walk_lean_kernel<false><<<nb, kLeanBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get());
+          break;
         case 2: walk_seg_kernel<2><<<nb * 2, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
         case 4: walk_seg_kernel<4><<<nb * 4, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
         case 8: walk_seg_kernel<8><<<nb * 8, kSegBlock, 0, st>>>(e->chunks[k].get(), e->lv[k].get()); break;
