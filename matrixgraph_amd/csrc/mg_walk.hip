@@ -160,10 +160,10 @@ __global__ void __launch_bounds__(kBlock) fill_zero_kernel(const FillRange* __re
   }
 }
 
-// Before a run: zero the strips of X_0 that the previous run set (their numeric mask is
-// non-zero) and reset the X_0 masks. X_0 buffers start zeroed at allocation. One wave per
-// strip, over the strips of all panels in turn: every lane reads the strip's mask before
-// lane 0 resets it, so no barrier is needed.
+// Before a run: zero the rows of X_0 that the previous run set (the rows marked in V_0 of the
+// strips whose numeric mask is non-zero: the source rows) and reset the X_0 masks. X_0
+// buffers start zeroed at allocation. One wave per strip, over the strips of all panels in
+// turn: every lane reads the strip's masks before lane 0 resets them, so no barrier is needed.
 template <int W>
 __global__ void __launch_bounds__(kBlock) clear_x0_kernel(const PanelInit* __restrict__ panels,
                                                           const uint32_t* __restrict__ n_strips, uint32_t n_panels) {
@@ -176,8 +176,12 @@ __global__ void __launch_bounds__(kBlock) clear_x0_kernel(const PanelInit* __res
     const uint32_t ns = n_strips[p];
     for (uint32_t y = first; y < ns; y += n_waves) {
       if (P.n0[y] != 0) {
+        // Only the rows that hold a source (V_0 marks them) are non-zero.
         uint64_t* dst = P.x0 + static_cast<uint64_t>(y) * kWords;
-        for (int i = lane; i < kWords; i += kWave) dst[i] = 0;
+        for (uint64_t rows = P.v0[y]; rows; rows &= rows - 1) {
+          const uint32_t r = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(rows)) - 1);
+          for (int i = lane; i < W; i += kWave) dst[r * W + i] = 0;
+        }
       }
       if (lane == 0) {
         P.n0[y] = 0;
