@@ -258,10 +258,12 @@ __device__ inline uint32_t wide_entry(WidePayload p, uint32_t i) {
 }
 
 // Wide payload of every tile of both operands, one wave per tile (lane r holds row r): tiles
-// [0, n_a) of data_a, then the n_bt tiles of data_bt, at the same positions of `payload`.
+// [0, n_a) of data_a, then the n_bt tiles of data_bt, at the same positions of `payload`; with
+// `bits`, also each tile's set-bit count at the same position.
 __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restrict__ data_a, uint32_t n_a,
                                                          const uint64_t* __restrict__ data_bt, uint32_t n_bt,
-                                                         WidePayload* __restrict__ payload) {
+                                                         WidePayload* __restrict__ payload,
+                                                         uint32_t* __restrict__ bits = nullptr) {
   const uint32_t n_waves = gridDim.x * kWavesPerBlock;
   const uint64_t n_tiles = static_cast<uint64_t>(n_a) + n_bt;
   for (uint64_t i = read_first_lane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); i < n_tiles; i += n_waves) {
@@ -284,16 +286,21 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
       pl.x = w[0];
       pl.y = w[1] | (static_cast<uint64_t>(n) << 60);
     }
-    if (lane_id() == 0) payload[i] = pl;
+    if (lane_id() == 0) {
+      payload[i] = pl;
+      if (bits) bits[i] = static_cast<uint32_t>(total);
+    }
   }
 }
 
-// Payloads of A's tiles at pl[0, n_a) and of Bt's at pl[n_a, n_a + n_bt), one launch.
-void launch_payloads(hipStream_t st, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, WidePayload* pl) {
+// Payloads of A's tiles at pl[0, n_a) and of Bt's at pl[n_a, n_a + n_bt), one launch (with
+// `bits`, the tiles' set-bit counts at the same positions).
+void launch_payloads(hipStream_t st, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, WidePayload* pl,
+                     uint32_t* bits = nullptr) {
   const uint64_t n = static_cast<uint64_t>(a->n_nz_tile) + bt->n_nz_tile;
   if (!n) return;
   payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile, bt->data.get(),
-                                                                        bt->n_nz_tile, pl);
+                                                                        bt->n_nz_tile, pl, bits);
   MG_LAUNCH_CHECK();
 }
 
@@ -1101,6 +1108,7 @@ struct PrunedPlan {
   Structural S;
   StreamBuf<uint32_t> row_s, col_s;
   WidePayload* pl = nullptr;
+  StreamBuf<uint32_t> bits;  // set bits per tile of A, then of Bt (from the payload pass)
 };
 
 void pruned_prepare(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, PrunedPlan& P,
@@ -1112,34 +1120,43 @@ void pruned_prepare(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_ma
   P.col_s.alloc(n_s, s);
   structural_emit(ctx, a, P.S, P.row_s.get(), P.col_s.get());
   P.pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
-  launch_payloads(s, a, bt, P.pl);
+  P.bits.alloc(static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1, s);
+  launch_payloads(s, a, bt, P.pl, P.bits.get());
 }
 
 // Features of every C row into feat (device, n_strips x kRowFeatures).
+// known_bits: the tiles' set-bit counts of A then Bt when a payload pass already made them
+// (else they are counted here).
 void row_features_into(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, const Structural& S,
-                       float* feat) {
+                       float* feat, const uint32_t* known_bits = nullptr) {
   hipStream_t s = ctx->stream;
   const uint32_t n = S.n;
-  StreamBuf<uint32_t> bits_a(a->n_nz_tile, s), bits_b(bt->n_nz_tile, s);
+  StreamBuf<uint32_t> counted(known_bits ? 0 : static_cast<size_t>(a->n_nz_tile) + bt->n_nz_tile + 1, s);
   StreamBuf<unsigned long long> brow(n, s);
   MG_HIP(hipMemsetAsync(brow.get(), 0, sizeof(unsigned long long) * std::max<uint32_t>(1, n), s));
-  if (a->n_nz_tile) {
-    tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(), a->n_nz_tile,
-                                                                                       bits_a.get());
-    MG_LAUNCH_CHECK();
+  if (!known_bits) {
+    if (a->n_nz_tile) {
+      tile_bits_kernel<<<grid_for(a->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(a->data.get(),
+                                                                                         a->n_nz_tile, counted.get());
+      MG_LAUNCH_CHECK();
+    }
+    if (bt->n_nz_tile) {
+      tile_bits_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(
+          bt->data.get(), bt->n_nz_tile, counted.get() + a->n_nz_tile);
+      MG_LAUNCH_CHECK();
+    }
   }
+  const uint32_t* bits_a = known_bits ? known_bits : counted.get();
+  const uint32_t* bits_b = bits_a + a->n_nz_tile;
   if (bt->n_nz_tile) {
-    tile_bits_kernel<<<grid_for(bt->n_nz_tile, kWavesPerBlock, 65536), kBlock, 0, s>>>(bt->data.get(), bt->n_nz_tile,
-                                                                                        bits_b.get());
-    MG_LAUNCH_CHECK();
     // Set bits of each row t of B = of the Bt tiles in column t.
-    brow_bits_kernel<<<grid_for(bt->n_nz_tile, kBlock), kBlock, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile,
-                                                                       bits_b.get(), brow.get());
+    brow_bits_kernel<<<grid_for(bt->n_nz_tile, kBlock), kBlock, 0, s>>>(bt->col_idx.get(), bt->n_nz_tile, bits_b,
+                                                                       brow.get());
     MG_LAUNCH_CHECK();
   }
   if (n) {
     row_features_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, s>>>(
-        n, a->offset_row.get(), a->col_idx.get(), bits_a.get(), brow.get(), S.cnt.get(), S.tp.get(), feat);
+        n, a->offset_row.get(), a->col_idx.get(), bits_a, brow.get(), S.cnt.get(), S.tp.get(), feat);
     MG_LAUNCH_CHECK();
   }
 }
@@ -1305,7 +1322,7 @@ extern "C" int mg_multiply_pruned_model(mg_context* ctx, const mg_tiled_matrix* 
       StreamBuf<float> feat(static_cast<uint64_t>(n) * kRowFeatures, s), y(static_cast<uint64_t>(n) * n_out, s),
           pred(n, s);
       StreamBuf<unsigned long long> total(1, s);
-      row_features_into(ctx, a, bt, P.S, feat.get());
+      row_features_into(ctx, a, bt, P.S, feat.get(), P.bits.get());
       if (mg_mlp_forward(ctx, model, feat.get(), n, y.get()) != MG_OK)
         fail(MG_ERR_INTERNAL, std::string("mg_multiply_pruned_model: ") + mg_last_error());
       MG_HIP(hipMemsetAsync(total.get(), 0, sizeof(unsigned long long), s));
