@@ -43,8 +43,8 @@ def popcount(a):
     return int(np.unpackbits(np.ascontiguousarray(a).view(np.uint8)).sum())
 
 
-# 3000 sources fill 47 of a 64-word panel's words: the hop kernel's variant for levels with dead
-# words gathers only the live words' cache lines.
+# 3000 sources fill 47 of a 64-word panel's words (the other 17 words stay zero); the batch
+# tests cover panels whose finished queries' words are dead.
 @pytest.mark.parametrize("scale,seed,n_src,hops", [(10, 1, 64, 4), (12, 4, 1, 3), (12, 5, 65, 4), (14, 2, 4096, 4),
                                                    (14, 6, 1000, 2), (13, 7, 3000, 4)])
 def test_walk_matches_oracle(ctx, scale, seed, n_src, hops):
