@@ -239,6 +239,15 @@ typedef struct mg_product_stats {
  * the temporaries are stream-ordered pool allocations. stats may be NULL. */
 int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                          mg_tiled_matrix** c, mg_product_stats* stats);
+/* Keeps m's per-tile payloads (16 bytes) and set-bit counts (4 bytes) with the handle, so the
+ * products that take m as an operand (mg_multiply_numeric*, mg_multiply_pruned*) copy them
+ * instead of reading m's tiles again: a walk by products, X_{k+1} = X_k * A, takes A^T at every
+ * hop. No counterpart in the reference (its products re-read every tile). The kept copy is
+ * dropped by the next library call that writes m's tiles (a numeric pass into m); after
+ * writing m's arrays through the pointers of mg_tiled_matrix_device_arrays, call it again.
+ * A call while the kept copy is current does nothing; otherwise it synchronises the context's
+ * stream. */
+int mg_tiled_matrix_prepare_operand(mg_context* ctx, mg_tiled_matrix* m);
 /* Numeric pass into a C produced by mg_multiply_symbolic (OR-accumulates).
  * Replaces PPRQuery::FillTilesUnifiedMemory (ppr_query.cu:459-780). Asynchronous on the
  * context's stream; keeps 16 bytes per operand tile in the context's scratch, which grows in

@@ -258,6 +258,11 @@ class TiledMatrix:
         _check(lib().mg_tiled_matrix_popcount(self.ctx.h, self.h, C.byref(d)), "mg_tiled_matrix_popcount")
         return d.value
 
+    def prepare_operand(self) -> None:
+        """Keeps this matrix's per-tile payloads and bit counts for the products it enters as an
+        operand (mg_tiled_matrix_prepare_operand): a walk by products reuses A^T's at every hop."""
+        _check(lib().mg_tiled_matrix_prepare_operand(self.ctx.h, self.h), "mg_tiled_matrix_prepare_operand")
+
     def to_edges(self) -> "EdgeList":
         """Every set bit as an edge (row, column), sorted, on n_strips*64 vertices."""
         h = C.c_void_p()

@@ -91,6 +91,7 @@ SIGNATURES = {
     "mg_tiled_matrix_device_arrays": (i32, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
     "mg_tiled_matrix_digest": (i32, [vp, vp, P_u64]),
     "mg_tiled_matrix_popcount": (i32, [vp, vp, P_u64]),
+    "mg_tiled_matrix_prepare_operand": (i32, [vp, vp]),
     "mg_tiled_matrix_destroy": (i32, [vp]),
     "mg_io_read_block_metadata": (i32, [C.c_char_p, P_u32, P_u32, P_u32]),
     "mg_io_read_block": (i32, [C.c_char_p, vp, vp, vp, vp, vp]),

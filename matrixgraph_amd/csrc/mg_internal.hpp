@@ -198,6 +198,13 @@ struct mg_tiled_matrix {
   mgamd::DevBuf<uint32_t> row_idx;     // n_nz_tile
   mgamd::DevBuf<uint32_t> col_idx;     // n_nz_tile
   mgamd::DevBuf<uint64_t> data;        // 64 * n_nz_tile
+  // Bumped by every library call that writes this matrix's tiles (the numeric passes into C).
+  uint64_t version = 0;
+  // mg_tiled_matrix_prepare_operand: the product path's per-tile payloads (16 B) and set-bit
+  // counts, valid while prep_version == version.
+  mgamd::DevBuf<uint64_t> prep_payload;
+  mgamd::DevBuf<uint32_t> prep_bits;
+  uint64_t prep_version = ~0ull;
 };
 
 struct mg_edge_list {

@@ -293,14 +293,36 @@ __global__ void __launch_bounds__(kBlock) payload_kernel(const uint64_t* __restr
   }
 }
 
+bool prepared(const mg_tiled_matrix* m) { return m->prep_version == m->version && m->prep_payload.get(); }
+
 // Payloads of A's tiles at pl[0, n_a) and of Bt's at pl[n_a, n_a + n_bt), one launch (with
-// `bits`, the tiles' set-bit counts at the same positions).
+// `bits`, the tiles' set-bit counts at the same positions). An operand prepared by
+// mg_tiled_matrix_prepare_operand is copied from its kept payloads instead (16 + 4 bytes per
+// tile rather than a read of its 512-byte tiles).
 void launch_payloads(hipStream_t st, const mg_tiled_matrix* a, const mg_tiled_matrix* bt, WidePayload* pl,
                      uint32_t* bits = nullptr) {
-  const uint64_t n = static_cast<uint64_t>(a->n_nz_tile) + bt->n_nz_tile;
+  const bool pa = prepared(a) && a->n_nz_tile, pb = prepared(bt) && bt->n_nz_tile;
+  const uint32_t na = a->n_nz_tile, nb = bt->n_nz_tile;
+  if (pa) {
+    MG_HIP(hipMemcpyAsync(pl, a->prep_payload.get(), sizeof(WidePayload) * na, hipMemcpyDeviceToDevice, st));
+    if (bits) MG_HIP(hipMemcpyAsync(bits, a->prep_bits.get(), sizeof(uint32_t) * na, hipMemcpyDeviceToDevice, st));
+  }
+  if (pb) {
+    MG_HIP(hipMemcpyAsync(pl + na, bt->prep_payload.get(), sizeof(WidePayload) * nb, hipMemcpyDeviceToDevice, st));
+    if (bits)
+      MG_HIP(hipMemcpyAsync(bits + na, bt->prep_bits.get(), sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, st));
+  }
+  // The rest in one launch: both operands, or the one that is not prepared.
+  const uint64_t n = (pa ? 0ull : na) + (pb ? 0ull : nb);
   if (!n) return;
-  payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), a->n_nz_tile, bt->data.get(),
-                                                                        bt->n_nz_tile, pl, bits);
+  if (!pa && !pb)
+    payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), na, bt->data.get(), nb, pl,
+                                                                          bits);
+  else if (!pa)
+    payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(a->data.get(), na, nullptr, 0, pl, bits);
+  else
+    payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(bt->data.get(), nb, nullptr, 0, pl + na,
+                                                                          bits ? bits + na : nullptr);
   MG_LAUNCH_CHECK();
 }
 
@@ -1087,6 +1109,26 @@ extern "C" int mg_multiply_symbolic(mg_context* ctx, const mg_tiled_matrix* a, c
   });
 }
 
+extern "C" int mg_tiled_matrix_prepare_operand(mg_context* ctx, mg_tiled_matrix* m) {
+  return guarded([&] {
+    MG_REQUIRE(ctx && m, "mg_tiled_matrix_prepare_operand: NULL argument");
+    MG_REQUIRE(m->tile_size == 64, "mg_tiled_matrix_prepare_operand: tile_size must be 64");
+    if (prepared(m)) return;  // kept copy still current
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    const uint32_t n = m->n_nz_tile;
+    m->prep_payload.alloc(2ull * std::max<uint32_t>(1, n));
+    m->prep_bits.alloc(std::max<uint32_t>(1, n));
+    if (n) {
+      payload_kernel<<<grid_for(n, kWavesPerBlock, 65536), kBlock, 0, st>>>(
+          m->data.get(), n, nullptr, 0, reinterpret_cast<WidePayload*>(m->prep_payload.get()), m->prep_bits.get());
+      MG_LAUNCH_CHECK();
+    }
+    MG_HIP(hipStreamSynchronize(st));
+    m->prep_version = m->version;
+  });
+}
+
 extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, const mg_tiled_matrix* bt,
                                    mg_tiled_matrix* c) {
   return guarded([&] {
@@ -1094,6 +1136,7 @@ extern "C" int mg_multiply_numeric(mg_context* ctx, const mg_tiled_matrix* a, co
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric: operands differ in n_strips");
     DeviceGuard g(ctx->device);
+    ++c->version;
     if (c->n_nz_tile == 0) return;
     WidePayload* pl = reinterpret_cast<WidePayload*>(ctx_scratch(ctx, payload_words(a, bt)));
     launch_payload_product(ctx, a, bt, c, pl, DenseRecord{});
@@ -1373,6 +1416,7 @@ extern "C" int mg_multiply_numeric_noalloc(mg_context* ctx, const mg_tiled_matri
     MG_REQUIRE(ctx && a && bt && c, "mg_multiply_numeric_noalloc: NULL argument");
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric_noalloc: operands differ in n_strips");
+    ++c->version;
     const size_t words = payload_words(a, bt);
     if (words * sizeof(uint64_t) > kNoAllocScratchBytes)
       invalid("mg_multiply_numeric_noalloc: " + std::to_string(a->n_nz_tile) + " + " +
@@ -1395,6 +1439,7 @@ extern "C" int mg_multiply_numeric_dense(mg_context* ctx, const mg_tiled_matrix*
     MG_REQUIRE(a->n_strips == bt->n_strips && c->n_strips == a->n_strips,
                "mg_multiply_numeric_dense: operands differ in n_strips");
     DeviceGuard g(ctx->device);
+    ++c->version;
     hipStream_t st = ctx->stream;
     StreamBuf<uint32_t> ba(a->n_nz_tile, st), bb(bt->n_nz_tile, st);
     if (a->n_nz_tile) {

@@ -350,6 +350,7 @@ def pruned_walk(ctx, at, sources, hops, mode, mlp=None, margin=CAPACITY_MARGIN):
     import matrixgraph_amd as M
     from . import workloads as WL
 
+    at.prepare_operand()  # A^T's payloads, kept for every hop's product
     x = WL.sources_matrix(ctx, at.n_strips * 64, sources)
     st = {"products": 0, "fallbacks": 0, "passes": 0, "scratch_bytes": 0, "output_bytes": 0, "seconds": 0.0,
           "rows": 0, "rows_mispredicted": 0, "predicted_tiles": 0, "numeric_tiles": 0, "capacity_tiles": 0,

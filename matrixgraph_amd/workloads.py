@@ -141,6 +141,7 @@ def product_walk(ctx, at, n_vertices: int, sources, hops: int, min_bits):
 
     import matrixgraph_amd as M
 
+    at.prepare_operand()  # A^T's payloads, kept for every hop's product
     xs = [sources_matrix(ctx, n_vertices, sources)]
     stats = []
     for _ in range(hops):

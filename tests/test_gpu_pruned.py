@@ -157,3 +157,32 @@ def test_learned_capacity_walk_equals_exact_and_structural(ctx):
                                                          e.col_idx[keep], e.data[keep]))
         assert xw.digest() == d_e
         w.close()
+
+
+def test_prepared_operand_gives_the_same_products(ctx):
+    """Operands whose payloads are kept (mg_tiled_matrix_prepare_operand) give the same plain,
+    pruned and model-free products; a numeric pass into a prepared matrix drops its kept copy
+    (prepared while its tiles were still zero, then filled: later products see the filled tiles)."""
+    M = mg()
+    el = M.EdgeList.rmat(ctx, 12, 16, 0.57, 0.19, 0.19, 5)
+    a = el.to_tiled_matrix(transpose=False)
+    at = el.to_tiled_matrix(transpose=True)
+    c0, _ = M.multiply(ctx, a, at)
+    p0, _ = M.multiply_pruned(ctx, a, at, M.PRUNED_STRUCTURAL)
+    want_c, want_p = c0.digest(), p0.digest()
+    a.prepare_operand()
+    at.prepare_operand()
+    at.prepare_operand()  # current already: nothing to do
+    c1, _ = M.multiply(ctx, a, at)
+    p1, _ = M.multiply_pruned(ctx, a, at, M.PRUNED_STRUCTURAL)
+    assert c1.digest() == want_c and p1.digest() == want_p
+    c2, _ = M.multiply_symbolic(ctx, a, at)
+    c2.prepare_operand()  # all tiles zero at this point
+    M.multiply_numeric(ctx, a, at, c2)
+    ctx.synchronize()
+    assert c2.digest() == want_c
+    d_prep, _ = M.multiply(ctx, c2, at)
+    d_ref, _ = M.multiply(ctx, c0, at)
+    assert d_prep.digest() == d_ref.digest()
+    for m in (c0, p0, c1, p1, c2, d_prep, d_ref):
+        m.close()
